@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the element internal-force path on one MI355X.
+
+A step is one residual's worth of the hot path on device-resident state: the ghost mirror
+update (GlobalGhost) plus one FInternalAct-equivalent force evaluation (element kernel +
+deterministic node gather) of the 256 x 192 active-strain membrane (98 304 triangles,
+3 through-thickness points, 3 C33 Newton iterations).  The headline value is FP64-plain
+elem-QP force evaluations per second; the FP32-mixed path is measured in the same run.
+
+Printed: one JSON line (see the keys below).  --gpus N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "biofem-ia_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "elem-QP force evals/s + GB/s (FP64 plain vs FP32-mixed); p50/p99 case latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def byte_model(m, n_qp):
+    """Algorithmic bytes of one force evaluation (compulsory traffic in this layout).
+
+    element kernel reads: coordinates 24 B/node; per element the 12-slot patch (48 B,
+    nen slots for irregular elements), packed meta (4 B), geom0 cache a1,a2,a3,a3_1,a3_2
+    (120 B), dA0 (8 B), fibre (24 B), gamma_scale (8 B).
+    gather kernel writes: Fint 24 B/node.
+    """
+    nen = m.nen().astype(np.int64)
+    per_elem = int((4 * nen).sum()) + m.n_elmt * (4 + 120 + 8 + 24 + 8)
+    elem_kernel = 24 * m.n_nodes + per_elem
+    path = elem_kernel + 24 * m.n_nodes
+    overhead = int((2 * 3 * 8 * nen).sum()) + 4 * int(nen.sum()) + 4 * (m.n_v + 1)
+    return elem_kernel, path, overhead
+
+
+def run_gpu(fe, lib, L, prec, steps, warmup):
+    import torch
+    dx, dy, dz, df = fe.device_buffers()
+    ctx = fe.ctx
+
+    def step():
+        L.check("ghost", lib.bf_ghost_update_device(ctx, dx, dy, dz, None))
+        L.check("eval", lib.bf_force_eval_device(ctx, dx, dy, dz, df, 0, prec, None))
+
+    for _ in range(warmup):
+        step()
+    L.check("sync", lib.bf_sync(ctx))
+    torch.cuda.synchronize()
+    L.check("timing", lib.bf_timing_enable(ctx, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    L.check("sync", lib.bf_sync(ctx))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n = C.c_longlong()
+    te, tg = C.c_double(), C.c_double()
+    L.check("timing_read", lib.bf_timing_read(ctx, C.byref(n), C.byref(te), C.byref(tg)))
+    L.check("timing", lib.bf_timing_enable(ctx, 0))
+    e, code, warn = fe.last_error()
+    if code == L.BF_ELEM_NONFINITE:
+        raise RuntimeError(f"non-finite element force at element {e}")
+    return wall, n.value, te.value / max(n.value, 1), tg.value / max(n.value, 1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=192)
+    ap.add_argument("--cpu", type=int, default=1, help="time the CPU oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores for the OpenMP leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the path would shard by element partition with a halo exchange, not run here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from biofem_amd import _lib as L
+    from biofem_amd import configs
+    from biofem_amd.fem import ActStrainFE
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.zeros(1, device="cuda:0")
+    lib = L.lib()
+    log(f"building workload membrane {args.nx}x{args.ny}")
+    m, p, (x, y, z) = configs.membrane_large(args.nx, args.ny)
+    E, q = m.n_elmt, p.n_qp
+    fe = ActStrainFE(m, p, device=0)
+    # one host-API evaluation stages the current coordinates in the context's device buffers
+    f64 = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
+    fmx = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
+    fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
+    elem_bytes, path_bytes, overhead_bytes = byte_model(m, q)
+
+    res = {}
+    for name, prec in (("fp64", L.BF_PREC_FP64), ("mixed", L.BF_PREC_MIXED)):
+        log(f"timing {name}: warmup {args.warmup}, steps {args.steps}")
+        wall, n, t_elem, t_gather = run_gpu(fe, lib, L, prec, args.steps, args.warmup)
+        ms = 1e3 * wall / args.steps
+        res[name] = dict(
+            ms_per_step=ms,
+            elem_qp_per_s=E * q * args.steps / wall,
+            kernel_ms_element=t_elem,
+            kernel_ms_gather=t_gather,
+            timed_launches=n,
+            gbps_element_kernel=elem_bytes / (t_elem * 1e-3) / 1e9,
+            gbps_path=path_bytes / ((t_elem + t_gather) * 1e-3) / 1e9,
+        )
+        log(f"{name}: {ms:.4f} ms/step, element kernel {t_elem:.4f} ms, gather {t_gather:.4f} ms")
+
+    cpu = None
+    parity = {}
+    if args.cpu:
+        from oracle import bf_oracle as O
+        log("CPU oracle baseline (single thread, full workload, geom0 prebuilt)")
+        cached = O.Cached(m, p, omp=True)
+        fref = np.zeros(3 * m.n_nodes)
+        xs, ys, zs = (np.ascontiguousarray(a) for a in (x, y, z))
+        t0 = time.perf_counter()
+        st = cached.eval(xs, ys, zs, fref, nthreads=1)
+        t1 = time.perf_counter() - t0
+        nth = args.cpu_threads or min(os.cpu_count() or 1, 64)
+        fomp = np.zeros(3 * m.n_nodes)
+        t0 = time.perf_counter()
+        cached.eval(xs, ys, zs, fomp, nthreads=nth)
+        tomp = time.perf_counter() - t0
+        cached.close()
+        cpu = {"value": E * q / t1, "unit": "elem-QP/s", "cores": 1, "kind": "port",
+               "sample": f"one full FInternalAct of the {args.nx}x{args.ny} membrane "
+                         f"({E} elements x {q} qp, {t1:.1f} s), oracle -O3 x86-64-v3, "
+                         "geom0 prebuilt",
+               "omp": {"value": E * q / tomp, "threads": nth, "seconds": tomp}}
+        nrm = np.linalg.norm(fref)
+        parity = {"fp64_rel_err": float(np.linalg.norm(f64 - fref) / nrm),
+                  "mixed_rel_err": float(np.linalg.norm(fmx - fref) / nrm),
+                  "omp_bitwise_equal": bool(np.array_equal(fomp, fref)),
+                  "oracle_status": list(st)}
+        log(f"cpu {t1:.2f} s single, {tomp:.2f} s x{nth}; parity {parity}")
+
+    r64 = res["fp64"]
+    achieved = r64["gbps_element_kernel"]
+    out = {
+        "metric": METRIC,
+        "value": r64["elem_qp_per_s"],
+        "unit": "elem-QP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"flat active-strain membrane {args.nx}x{args.ny} ({E} triangles, "
+                        f"{m.n_nodes} nodes incl. ghosts), n_qp={q}, C33 iters={p.c33_iters}, "
+                        "FP64 plain; step = ghost update + force evaluation",
+            "elements": E, "nodes": m.n_nodes, "n_qp": q, "c33_iters": p.c33_iters,
+            "elem_qp_per_step": E * q,
+        },
+        "gbps": r64["gbps_path"],
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "bf_element_kernel<double>",
+            "algorithmic_bytes_per_launch": elem_bytes,
+            "kernel_ms": r64["kernel_ms_element"],
+        },
+        "cpu_baseline": cpu,
+        "fp32_mixed": dict(res["mixed"], speedup_vs_fp64=res["mixed"]["elem_qp_per_s"] / r64["elem_qp_per_s"],
+                           roofline_frac=res["mixed"]["gbps_element_kernel"] / HBM_PEAK_GBPS),
+        "fp64_detail": r64,
+        "bytes": {"element_kernel": elem_bytes, "path": path_bytes, "overhead_element_buffer": overhead_bytes},
+        "parity": parity,
+    }
+    fe.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

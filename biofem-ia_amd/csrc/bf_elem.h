@@ -1,0 +1,440 @@
+// bf_elem.h -- per-element active-strain Kirchhoff-Love force arithmetic, shared by the
+// gfx950 kernels (bf_force.hip) and by the CPU consistency harness in tests/.
+//
+// Everything here is templated on the per-quadrature-point real type R (double for the
+// plain FP64 path, float for the FP32-mixed path).  Node coordinates, the midsurface
+// geometry sums and the nodal force accumulation stay in double in both paths.
+//
+// The reference evaluates the constitutive chain with literal index loops
+// (src/active_strain.c:1381-1648: a 6-deep loop for Ce^-1, a 6-deep loop for S and an
+// 8-deep, 6561-term loop for CC^{2222}).  Here the same tensors are produced by staged
+// 3x3 products and closed forms of the contractions:
+//   Ce^-1     = T Ci T^T,          T  = Fa^{..} g0_{..}        (:1394-1417)
+//   Je        = 1/sqrt(det(Ce^-1) det(g0_{..}))              (:1458-1461; det Ce = 1/det Ce^-1)
+//   S         = 1/2 (H^T Se H + Se^T H H),  H = Fa^-1_{..} g0^{..}   (:1509-1528)
+//   CC^{2222} = 1/4 [2(beta-alpha) b^2 + 4 beta b (P.r) - 4 alpha B(h,P)(h.r)
+//                    + 2 beta (P.r)^2 - alpha B(P,P) C^22 - alpha (P.r)^2]   (:1614-1643)
+//               with h = H e3, P = H h, r = Ce^-1 e3, b = h.Ce^-1 h, B(u,v) = u.Ce^-1 v.
+// The staged forms are exact algebraic identities of the reference's sums (given the
+// symmetry of Ce^-1, g0 and Fa^-1 that the reference's own construction produces); they
+// differ from the loops only by rounding order.
+#pragma once
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define BF_HD __host__ __device__ __forceinline__
+#else
+#define BF_HD inline
+#endif
+
+#include <math.h>
+
+namespace bf {
+
+constexpr int kSentinel = 1000000;  // empty patch slot (src/io.c:306)
+constexpr int kMaxNen = 18;         // widest patch handled (valence 12)
+constexpr int kMaxQp = 5;
+
+// element kinds packed in meta bits 0..1
+enum : int { kRegular = 0, kIrregular = 1, kCap = 2 };
+
+// per-element status bits (OR-ed into a flag word, first element kept by atomicMin)
+enum : int { kStatCC3333 = 1, kStatNonFinite = 2 };
+
+template <class R>
+struct V3 {
+  R x, y, z;
+};
+
+template <class R>
+BF_HD V3<R> mk(R x, R y, R z) {
+  V3<R> v;
+  v.x = x;
+  v.y = y;
+  v.z = z;
+  return v;
+}
+template <class R>
+BF_HD V3<R> add(V3<R> a, V3<R> b) {
+  return mk(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template <class R>
+BF_HD V3<R> scale(R s, V3<R> a) {
+  return mk(s * a.x, s * a.y, s * a.z);
+}
+template <class R>
+BF_HD V3<R> cross(V3<R> v1, V3<R> v2) {  // src/math.c:29-37
+  return mk(v1.y * v2.z - v2.y * v1.z, -v1.x * v2.z + v2.x * v1.z, v1.x * v2.y - v2.x * v1.y);
+}
+template <class R>
+BF_HD R dot(V3<R> a, V3<R> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <class R>
+BF_HD R norm(V3<R> a) {
+  return sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+template <class R>
+BF_HD V3<R> unit(V3<R> a) {
+  R n = norm(a);
+  return mk(a.x / n, a.y / n, a.z / n);
+}
+template <class T, class R>
+BF_HD V3<T> cvt(V3<R> a) {
+  return mk((T)a.x, (T)a.y, (T)a.z);
+}
+
+// symmetric 3x3, storage (00, 11, 22, 01, 02, 12)
+template <class R>
+struct Sym {
+  R a00, a11, a22, a01, a02, a12;
+};
+template <class R>
+struct M33 {
+  R m[3][3];
+};
+
+template <class R>
+BF_HD R sdet(const Sym<R>& A) {  // DET3x3 row expansion, src/math.c:150-156
+  return A.a00 * (A.a11 * A.a22 - A.a12 * A.a12) - A.a01 * (A.a01 * A.a22 - A.a12 * A.a02) +
+         A.a02 * (A.a01 * A.a12 - A.a11 * A.a02);
+}
+template <class R>
+BF_HD Sym<R> sinv(const Sym<R>& T, R* det_out) {  // cofactor inverse, src/math.c:77-91
+  const R T11 = T.a00, T12 = T.a01, T13 = T.a02, T22 = T.a11, T23 = T.a12, T33 = T.a22;
+  const R det = T11 * (T33 * T22 - T23 * T23) - T12 * (T33 * T12 - T23 * T13) +
+                T13 * (T23 * T12 - T22 * T13);
+  const R c = R(1) / det;
+  Sym<R> I;
+  I.a00 = c * (T33 * T22 - T23 * T23);
+  I.a01 = -c * (T33 * T12 - T23 * T13);
+  I.a02 = c * (T23 * T12 - T22 * T13);
+  I.a11 = c * (T33 * T11 - T13 * T13);
+  I.a12 = -c * (T23 * T11 - T12 * T13);
+  I.a22 = c * (T22 * T11 - T12 * T12);
+  if (det_out) *det_out = det;
+  return I;
+}
+template <class R>
+BF_HD M33<R> full(const Sym<R>& S) {
+  M33<R> M;
+  M.m[0][0] = S.a00; M.m[0][1] = S.a01; M.m[0][2] = S.a02;
+  M.m[1][0] = S.a01; M.m[1][1] = S.a11; M.m[1][2] = S.a12;
+  M.m[2][0] = S.a02; M.m[2][1] = S.a12; M.m[2][2] = S.a22;
+  return M;
+}
+template <class R>
+BF_HD M33<R> mul(const M33<R>& A, const M33<R>& B) {
+  M33<R> C;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) C.m[i][j] = A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j] + A.m[i][2] * B.m[2][j];
+  return C;
+}
+// Gram matrix of three vectors g_i . g_j (ComputeMetricTensor, active_strain.c:769-779)
+template <class R>
+BF_HD Sym<R> gram(V3<R> g0, V3<R> g1, V3<R> g2) {
+  Sym<R> G;
+  G.a00 = dot(g0, g0);
+  G.a11 = dot(g1, g1);
+  G.a22 = dot(g2, g2);
+  G.a01 = dot(g0, g1);
+  G.a02 = dot(g0, g2);
+  G.a12 = dot(g1, g2);
+  return G;
+}
+
+// Compute_a3_alpha_ (active_strain.c:745-762): un-normalised derivative of a1 x a2 over |a1 x a2|
+template <class R>
+BF_HD void a3_alpha(V3<R> a1, V3<R> a2, V3<R> Aaa, V3<R> Aab, V3<R> Abb, V3<R>& a31, V3<R>& a32) {
+  const R s = norm(cross(a1, a2));
+  a31 = add(cross(Aaa, a2), cross(a1, Aab));
+  a32 = add(cross(Aab, a2), cross(a1, Abb));
+  a31 = mk(a31.x / s, a31.y / s, a31.z / s);
+  a32 = mk(a32.x / s, a32.y / s, a32.z / s);
+}
+
+// Midsurface data of one configuration (SubdivGeomQP without the IN* arrays).
+struct MidGeom {
+  V3<double> a1, a2, a3, gc1, gc2, Aaa, Abb, Aab;
+};
+
+// a3 = unit(a1 x a2) and the contravariant in-plane basis (active_strain.c:535-541)
+BF_HD void geom_tail(MidGeom& G) {
+  G.a3 = unit(cross(G.a1, G.a2));
+  G.gc1 = cross(G.a2, G.a3);
+  G.gc1 = scale(1.0 / dot(G.a1, G.gc1), G.gc1);
+  G.gc2 = cross(G.a3, G.a1);
+  G.gc2 = scale(1.0 / dot(G.a2, G.gc2), G.gc2);
+}
+
+// Midsurface geometry of a patch (ElemUpdateGeomSubdivFromCoords_, active_strain.c:477-541
+// regular, :668-699 irregular): second-derivative sums with Nab rows, tangents with Na rows,
+// each accumulated from 0 in slot order.  `xyz(i)` returns the coordinates of slot i.
+template <class CoordFn>
+BF_HD MidGeom mid_geom(int nen, const double* na0, const double* na1, const double* nb0,
+                       const double* nb1, const double* nb2, CoordFn xyz) {
+  MidGeom G;
+  G.Aaa = mk(0.0, 0.0, 0.0);
+  G.Abb = G.Aaa;
+  G.Aab = G.Aaa;
+  G.a1 = G.Aaa;
+  G.a2 = G.Aaa;
+  for (int i = 0; i < nen; i++) {
+    const V3<double> X = xyz(i);
+    G.Aaa.x += nb0[i] * X.x; G.Aaa.y += nb0[i] * X.y; G.Aaa.z += nb0[i] * X.z;
+    G.Abb.x += nb1[i] * X.x; G.Abb.y += nb1[i] * X.y; G.Abb.z += nb1[i] * X.z;
+    G.Aab.x += nb2[i] * X.x; G.Aab.y += nb2[i] * X.y; G.Aab.z += nb2[i] * X.z;
+    G.a1.x += na0[i] * X.x; G.a1.y += na0[i] * X.y; G.a1.z += na0[i] * X.z;
+    G.a2.x += na1[i] * X.x; G.a2.y += na1[i] * X.y; G.a2.z += na1[i] * X.z;
+  }
+  geom_tail(G);
+  return G;
+}
+
+// Flat-triangle (CST) geometry of an LV apex-cap element (active_strain.c:448-472).
+BF_HD MidGeom cap_geom(V3<double> x1, V3<double> x2, V3<double> x3) {
+  MidGeom G;
+  G.a1 = mk(x2.x - x1.x, x2.y - x1.y, x2.z - x1.z);
+  G.a2 = mk(x3.x - x1.x, x3.y - x1.y, x3.z - x1.z);
+  geom_tail(G);
+  G.Aaa = mk(0.0, 0.0, 0.0);
+  G.Abb = G.Aaa;
+  G.Aab = G.Aaa;
+  return G;
+}
+
+// Activation description of one element: Fa_cart = diag(d) - gam * f f^T.
+struct Activation {
+  double d0, d1, d2;  // diagonal part
+  double gam;         // rank-1 coefficient
+  V3<double> f;       // fibre
+};
+
+// Per-element reference data cached once (geom0): a_alpha, a3 and a3,alpha of the
+// reference midsurface.
+struct RefGeom {
+  V3<double> a1, a2, a3, a31, a32;
+};
+
+// Result of the through-thickness integration of one element: the stress resultants
+//   N = sum_q w_q Sv_q   and   M = sum_q w_q theta_q Sv_q,  Sv = (S^11, S^22, S^12),
+// plus the final C33 per qp and status bits.
+template <class R>
+struct Resultants {
+  double N0, N1, N2, M0, M1, M2;
+  double c33[kMaxQp];
+  int status;
+};
+
+// Staged constitutive chain for one element (ElemUpdateG, ElemActDefGrad, ElemCGDefTens,
+// ElemC33Solve; active_strain.c:896-982, 1196-1378, 1381-1721, 2048-2138).
+//   cur: current a1,a2,a3,a3_1,a3_2   ref: reference ones   R: per-qp arithmetic type.
+template <class R>
+BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a2c,
+                                      const V3<double>& a3c, const V3<double>& a31c,
+                                      const V3<double>& a32c, const RefGeom& ref,
+                                      const Activation& act, double h0, double mu_d,
+                                      double K_d, int n_qp, int iters, const double* theta,
+                                      const double* wq) {
+  Resultants<R> out;
+  out.N0 = out.N1 = out.N2 = out.M0 = out.M1 = out.M2 = 0.0;
+  out.status = 0;
+  const R mu = (R)mu_d, K = (R)K_d;
+  const double hhalf = 0.5 * h0;
+  // activation in the Cartesian frame, its inverse:  Fa^-1 = diag(1/d) + kap f f^T
+  const bool rank1 = act.gam != 0.0;
+  const R gam = (R)act.gam;
+  const V3<R> f = cvt<R>(act.f);
+  const R ff = dot(f, f);
+  const R kap = rank1 ? gam / (R(1) - gam * ff) : R(0);
+  const R d0 = (R)act.d0, d1 = (R)act.d1, d2 = (R)act.d2;
+  const bool diag_id = (act.d0 == 1.0 && act.d1 == 1.0 && act.d2 == 1.0);
+  // bit `it` set once some qp failed the CC3333 guard at iteration `it`: the reference's
+  // ModElemC33 returns at the first failing qp (:1679), so that qp and every later qp keep
+  // their C33 in that iteration (iteration-outer loop order, :2058/:1660).
+  unsigned skip_mask = 0u;
+
+  for (int qp = 0; qp < n_qp; qp++) {
+    const double t3d = hhalf * theta[qp];
+    const R t3 = (R)t3d;
+    // ---- current metric (ElemUpdateG :940-957) ----
+    const V3<R> g0 = add(cvt<R>(a1c), scale(t3, cvt<R>(a31c)));
+    const V3<R> g1 = add(cvt<R>(a2c), scale(t3, cvt<R>(a32c)));
+    const V3<R> g2 = cvt<R>(a3c);
+    const Sym<R> gm = gram(g0, g1, g2);
+    const Sym<R> gmI = sinv(gm, (R*)nullptr);
+    // ---- reference metric (:961-978) ----
+    const V3<R> h0v = add(cvt<R>(ref.a1), scale(t3, cvt<R>(ref.a31)));
+    const V3<R> h1v = add(cvt<R>(ref.a2), scale(t3, cvt<R>(ref.a32)));
+    const V3<R> h2v = cvt<R>(ref.a3);
+    const Sym<R> gm0 = gram(h0v, h1v, h2v);
+    const Sym<R> G = sinv(gm0, (R*)nullptr);  // gm0.Cont
+    const R det0 = sdet(gm0);
+    // ---- activation (ElemActDefGrad :1272-1320) ----
+    // Fa.Cov = S^T Fa_cart S, Fa_inv.Cov = S^T Fa_cart^-1 S with S = [g0_1 g0_2 g0_3]
+    const R phi0 = dot(f, h0v), phi1 = dot(f, h1v), phi2 = dot(f, h2v);
+    Sym<R> FaCov, FiCov;
+    if (diag_id) {
+      FaCov = gm0;
+      FiCov = gm0;
+    } else {
+      const R e0 = R(1) / d0, e1 = R(1) / d1, e2 = R(1) / d2;
+      FaCov.a00 = d0 * h0v.x * h0v.x + d1 * h0v.y * h0v.y + d2 * h0v.z * h0v.z;
+      FaCov.a11 = d0 * h1v.x * h1v.x + d1 * h1v.y * h1v.y + d2 * h1v.z * h1v.z;
+      FaCov.a22 = d0 * h2v.x * h2v.x + d1 * h2v.y * h2v.y + d2 * h2v.z * h2v.z;
+      FaCov.a01 = d0 * h0v.x * h1v.x + d1 * h0v.y * h1v.y + d2 * h0v.z * h1v.z;
+      FaCov.a02 = d0 * h0v.x * h2v.x + d1 * h0v.y * h2v.y + d2 * h0v.z * h2v.z;
+      FaCov.a12 = d0 * h1v.x * h2v.x + d1 * h1v.y * h2v.y + d2 * h1v.z * h2v.z;
+      FiCov.a00 = e0 * h0v.x * h0v.x + e1 * h0v.y * h0v.y + e2 * h0v.z * h0v.z;
+      FiCov.a11 = e0 * h1v.x * h1v.x + e1 * h1v.y * h1v.y + e2 * h1v.z * h1v.z;
+      FiCov.a22 = e0 * h2v.x * h2v.x + e1 * h2v.y * h2v.y + e2 * h2v.z * h2v.z;
+      FiCov.a01 = e0 * h0v.x * h1v.x + e1 * h0v.y * h1v.y + e2 * h0v.z * h1v.z;
+      FiCov.a02 = e0 * h0v.x * h2v.x + e1 * h0v.y * h2v.y + e2 * h0v.z * h2v.z;
+      FiCov.a12 = e0 * h1v.x * h2v.x + e1 * h1v.y * h2v.y + e2 * h1v.z * h2v.z;
+    }
+    if (rank1) {
+      FaCov.a00 -= gam * phi0 * phi0; FaCov.a11 -= gam * phi1 * phi1; FaCov.a22 -= gam * phi2 * phi2;
+      FaCov.a01 -= gam * phi0 * phi1; FaCov.a02 -= gam * phi0 * phi2; FaCov.a12 -= gam * phi1 * phi2;
+      FiCov.a00 += kap * phi0 * phi0; FiCov.a11 += kap * phi1 * phi1; FiCov.a22 += kap * phi2 * phi2;
+      FiCov.a01 += kap * phi0 * phi1; FiCov.a02 += kap * phi0 * phi2; FiCov.a12 += kap * phi1 * phi2;
+    }
+    const M33<R> Gm = full(G), g0m = full(gm0);
+    // Fa.Cont = G FaCov G ; T = Fa.Cont gm0 (mixed)
+    const M33<R> FaCont = mul(mul(Gm, full(FaCov)), Gm);
+    const M33<R> T = mul(FaCont, g0m);
+    // H = Fa_inv.Cov G  (mixed), H2 = H H
+    const M33<R> H = mul(full(FiCov), Gm);
+    const M33<R> H2 = mul(H, H);
+    // ---- ElemCGDefTens (:1341-1363): C.Cov in-plane block from gm, C33 = 1;
+    //      C_inv.Cont in-plane block from gm.Cont, 1 at [2][2]
+    const R c00 = gm.a00, c01 = gm.a01, c11 = gm.a11;
+    R c33 = R(1);
+    R ci00 = gmI.a00, ci01 = gmI.a01, ci11 = gmI.a11, ci22 = R(1);
+    R S00 = 0, S11 = 0, S01 = 0;
+    for (int it = 0; it < iters; it++) {
+      // ---- Ce^-1 = T Ci T^T (ElemElasCGDefTens :1394-1417) ----
+      M33<R> TC;  // T * Ci, Ci block-diagonal
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        TC.m[i][0] = T.m[i][0] * ci00 + T.m[i][1] * ci01;
+        TC.m[i][1] = T.m[i][0] * ci01 + T.m[i][1] * ci11;
+        TC.m[i][2] = T.m[i][2] * ci22;
+      }
+      Sym<R> Ch;
+      Ch.a00 = TC.m[0][0] * T.m[0][0] + TC.m[0][1] * T.m[0][1] + TC.m[0][2] * T.m[0][2];
+      Ch.a11 = TC.m[1][0] * T.m[1][0] + TC.m[1][1] * T.m[1][1] + TC.m[1][2] * T.m[1][2];
+      Ch.a22 = TC.m[2][0] * T.m[2][0] + TC.m[2][1] * T.m[2][1] + TC.m[2][2] * T.m[2][2];
+      Ch.a01 = TC.m[0][0] * T.m[1][0] + TC.m[0][1] * T.m[1][1] + TC.m[0][2] * T.m[1][2];
+      Ch.a02 = TC.m[0][0] * T.m[2][0] + TC.m[0][1] * T.m[2][1] + TC.m[0][2] * T.m[2][2];
+      Ch.a12 = TC.m[1][0] * T.m[2][0] + TC.m[1][1] * T.m[2][1] + TC.m[1][2] * T.m[2][2];
+      // ---- Je (ElemElasStress :1458-1461) ----
+      const R detCh = sdet(Ch);
+      const R Je = R(1) / sqrt(detCh * det0);
+      const R JJ = Je * Je - Je;
+      // ---- Se = mu (G - Ch) + K Ch (Je^2 - Je) (:1473-1487) ----
+      const R kj = K * JJ;
+      Sym<R> Se;
+      Se.a00 = mu * (G.a00 - Ch.a00) + kj * Ch.a00;
+      Se.a11 = mu * (G.a11 - Ch.a11) + kj * Ch.a11;
+      Se.a22 = mu * (G.a22 - Ch.a22) + kj * Ch.a22;
+      Se.a01 = mu * (G.a01 - Ch.a01) + kj * Ch.a01;
+      Se.a02 = mu * (G.a02 - Ch.a02) + kj * Ch.a02;
+      Se.a12 = mu * (G.a12 - Ch.a12) + kj * Ch.a12;
+      // ---- S = 1/2 (H^T Se H + Se H2) (ElemTotStress :1509-1528) ----
+      const M33<R> SeH = mul(full(Se), H);
+      const M33<R> SeF = full(Se);
+      // S^ij = 1/2 (sum_p H[p][i] SeH[p][j] + sum_p Se[p][i] H2[p][j])
+      S00 = R(0.5) * (H.m[0][0] * SeH.m[0][0] + H.m[1][0] * SeH.m[1][0] + H.m[2][0] * SeH.m[2][0] +
+                      SeF.m[0][0] * H2.m[0][0] + SeF.m[1][0] * H2.m[1][0] + SeF.m[2][0] * H2.m[2][0]);
+      S11 = R(0.5) * (H.m[0][1] * SeH.m[0][1] + H.m[1][1] * SeH.m[1][1] + H.m[2][1] * SeH.m[2][1] +
+                      SeF.m[0][1] * H2.m[0][1] + SeF.m[1][1] * H2.m[1][1] + SeF.m[2][1] * H2.m[2][1]);
+      S01 = R(0.5) * (H.m[0][0] * SeH.m[0][1] + H.m[1][0] * SeH.m[1][1] + H.m[2][0] * SeH.m[2][1] +
+                      SeF.m[0][0] * H2.m[0][1] + SeF.m[1][0] * H2.m[1][1] + SeF.m[2][0] * H2.m[2][1]);
+      const R S22 = R(0.5) * (H.m[0][2] * SeH.m[0][2] + H.m[1][2] * SeH.m[1][2] + H.m[2][2] * SeH.m[2][2] +
+                              SeF.m[0][2] * H2.m[0][2] + SeF.m[1][2] * H2.m[1][2] + SeF.m[2][2] * H2.m[2][2]);
+      // ---- CC^{2222} closed form (ElemElsTangMatTens + ElemTotTangMatTens) ----
+      const R alpha = K * JJ - mu;
+      const R beta = R(0.5) * K * Je * (R(2) * Je - R(1));
+      const R hx = H.m[0][2], hy = H.m[1][2], hz = H.m[2][2];     // h = H e3
+      const R Px = H2.m[0][2], Py = H2.m[1][2], Pz = H2.m[2][2];  // P = H h
+      const R rx = Ch.a02, ry = Ch.a12, rz = Ch.a22;              // r = Ch e3
+      const R Chx = Ch.a00 * hx + Ch.a01 * hy + Ch.a02 * hz;       // Ch h
+      const R Chy = Ch.a01 * hx + Ch.a11 * hy + Ch.a12 * hz;
+      const R Chz = Ch.a02 * hx + Ch.a12 * hy + Ch.a22 * hz;
+      const R b = hx * Chx + hy * Chy + hz * Chz;
+      const R BhP = Px * Chx + Py * Chy + Pz * Chz;
+      const R CPx = Ch.a00 * Px + Ch.a01 * Py + Ch.a02 * Pz;
+      const R CPy = Ch.a01 * Px + Ch.a11 * Py + Ch.a12 * Pz;
+      const R CPz = Ch.a02 * Px + Ch.a12 * Py + Ch.a22 * Pz;
+      const R BPP = Px * CPx + Py * CPy + Pz * CPz;
+      const R Pr = Px * rx + Py * ry + Pz * rz;
+      const R hr = hx * rx + hy * ry + hz * rz;
+      const R CC = R(0.25) * (R(2) * (beta - alpha) * b * b + R(4) * beta * b * Pr -
+                              R(4) * alpha * BhP * hr + R(2) * beta * Pr * Pr -
+                              alpha * BPP * Ch.a22 - alpha * Pr * Pr);
+      // ---- ModElemC33 (:1669-1707) ----
+      if (!(fabs(CC) > R(1e-14))) {
+        skip_mask |= 1u << it;  // error ignored by the caller (:2086)
+        out.status |= kStatCC3333;
+      } else if (!((skip_mask >> it) & 1u)) {
+        R dC = -S22 / CC;
+        const R cold = c33;
+        R trial = cold + dC;
+        // det(C.Cov) of the block-diagonal C: C00 (C11 C33) - C01 (C01 C33)
+        if (c00 * (c11 * trial) - c01 * (c01 * trial) <= R(0)) {
+          R lo = 0, hi = 1;
+          for (int bk = 0; bk < 16; bk++) {
+            const R mid = R(0.5) * (lo + hi);
+            const R cm = cold + mid * dC;
+            if (c00 * (c11 * cm) - c01 * (c01 * cm) > R(0)) lo = mid; else hi = mid;
+          }
+          dC *= lo;
+          trial = cold + dC;
+        }
+        c33 = trial;
+      }
+      // ---- C_inv.Cont = INV(C.Cov) (:2101), block-diagonal cofactor form ----
+      {
+        const R det = c00 * (c33 * c11) - c01 * (c33 * c01);
+        const R c = R(1) / det;
+        ci00 = c * (c33 * c11);
+        ci01 = -c * (c33 * c01);
+        ci11 = c * (c33 * c00);
+        ci22 = c * (c11 * c00 - c01 * c01);
+      }
+    }
+    out.c33[qp] = (double)c33;
+    // ---- through-thickness resultants for ElemUpdFint (:1859-1914) ----
+    const double w = wq[qp];
+    const double wt = w * theta[qp];
+    out.N0 += w * (double)S00;
+    out.N1 += w * (double)S11;
+    out.N2 += w * (double)S01;
+    out.M0 += wt * (double)S00;
+    out.M1 += wt * (double)S11;
+    out.M2 += wt * (double)S01;
+  }
+  return out;
+}
+
+// Nodal force of one patch node a (ElemUpdFint :1879-1913 / :1934-1983), summed over qps:
+//   Fb_a = pref0 [ (Na0 N0 + Na1 N2) a1 + (Na1 N1 + Na0 N2) a2
+//                  - (h0/2) (c0 M0 + c1 M1 + 2 c2 M2) a3 ],
+//   c_k = Nab_k + (Na0 gc1 + Na1 gc2) . A_k,  A = (Aaa, Abb, Aab).
+BF_HD V3<double> node_force(double Na0, double Na1, double Nb0, double Nb1, double Nb2,
+                            const MidGeom& G, double pref0, double half_h0, double N0,
+                            double N1, double N2, double M0, double M1, double M2) {
+  const V3<double> ng = add(scale(Na0, G.gc1), scale(Na1, G.gc2));
+  const double c0 = Nb0 + dot(ng, G.Aaa);
+  const double c1 = Nb1 + dot(ng, G.Abb);
+  const double c2 = Nb2 + dot(ng, G.Aab);
+  const double k1 = Na0 * N0 + Na1 * N2;
+  const double k2 = Na1 * N1 + Na0 * N2;
+  const double k3 = -half_h0 * (c0 * M0 + c1 * M1 + 2.0 * c2 * M2);
+  return mk(pref0 * (k1 * G.a1.x + k2 * G.a2.x + k3 * G.a3.x),
+            pref0 * (k1 * G.a1.y + k2 * G.a2.y + k3 * G.a3.y),
+            pref0 * (k1 * G.a1.z + k2 * G.a2.z + k3 * G.a3.z));
+}
+
+}  // namespace bf

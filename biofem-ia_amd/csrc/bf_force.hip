@@ -1,0 +1,905 @@
+// bf_force.hip -- gfx950 kernels and C ABI of the element internal-force path.
+//
+// Pipeline of one force evaluation (replaces FInternalAct, src/active_strain.c:2226-2267):
+//   1. bf_element_kernel<R>: one lane per element.  Gathers the element's Loop patch from
+//      SoA coordinates, forms the current midsurface geometry (FP64), runs the staged
+//      active-strain constitutive chain per through-thickness point in R, and writes the
+//      3*nen nodal contributions to an element-force buffer laid out slot-major
+//      (Fe[(3*slot + c) * E + e]), so that the 64 lanes of a wave store contiguously.
+//   2. bf_gather_kernel: one lane per node.  Sums the node's contributions in ascending
+//      (element, slot) order from a CSR list built at upload, in FP64.  This is the
+//      summation order of VecSetValues(ADD_VALUES) on one rank (active_strain.c:2238-2260),
+//      deterministic run to run, and uses no atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/biofem_hip.h"
+#include "bf_elem.h"
+
+using namespace bf;
+
+// ------------------------------------------------------------------ constant tables ----
+// Loop box-spline derivatives at the element centre, verbatim 4-decimal values of
+// src/active_strain.c:69-88.
+__constant__ double c_Na[2][12] = {
+    {-0.0247, -0.0309, 0.0000, -0.4815, -0.1852, 0.0247, 0.4815, 0.0000, -0.0062, 0.0309,
+     0.1852, 0.0062},
+    {-0.0309, -0.0247, -0.1852, -0.4815, 0.0000, -0.0062, 0.0000, 0.4815, 0.0247, 0.0062,
+     0.1852, 0.0309}};
+__constant__ double c_Nab[3][12] = {
+    {0.1111, 0.2222, -0.2222, -0.2222, 0.4444, 0.1111, -0.2222, -0.8889, 0.0000, 0.2222,
+     0.4444, 0.0000},
+    {0.2222, 0.1111, 0.4444, -0.2222, -0.2222, 0.0000, -0.8889, -0.2222, 0.1111, 0.0000,
+     0.4444, 0.2222},
+    {0.1667, 0.1667, -0.1111, 0.2222, -0.1111, -0.0556, -0.4444, -0.4444, -0.0556, 0.0556,
+     0.5556, 0.0556}};
+// IN* rows per valence (filled from bf_loop_tables at context creation): [v][row][slot]
+__constant__ double c_IN[13][5][kMaxNen];
+
+// ------------------------------------------------------------------ kernel params ------
+struct RampParams {
+  double gamma, gamma_T, tau_up, tau_down, apd, pulse_t0, pulse_sigma, t, T_default;
+  int type;
+};
+
+struct KParams {
+  int E, NS, n_qp, iters;
+  int act_mode;    // 0: I - gamma f f (cart_fib_act), 1: manufactured diag, 2: isotropic diag
+  int has_delay;   // per-element activation delay present
+  double theta[kMaxQp], w[kMaxQp];
+  double h0, mu, K;
+  double gamma_t;  // GammaOfTimeDelayed(p, 0) evaluated on the host
+  double man_gamma;
+  RampParams ramp;
+};
+
+struct DevMesh {
+  const int* patch;    // [NS][E]
+  const int* meta;     // kind | val << 8 | nen << 16
+  const int* corner;   // [6][E] cap: nv1..3 then their patch slots (NULL if no cap)
+  const double* ref;   // [15][E] a1 a2 a3 a31 a32 of geom0
+  const double* dA0;
+  const double* fib;   // [3][E]
+  const double* gscale;
+  const double* delay;
+  double* Fe;          // [3*NS][E]
+  double* c33;         // [n_qp][E]
+  int* status;         // [4] first error elem, code, first warn elem, warn count
+};
+
+// GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
+__device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) {
+  if (p.type == 0) return p.gamma;
+  const double t = p.t - delay;
+  if (p.type == 3) {
+    const double tau_up = (p.tau_up > 0.0) ? p.tau_up : 0.02;
+    const double tau_down = (p.tau_down > 0.0) ? p.tau_down : 0.08;
+    const double apd = (p.apd > 0.0) ? p.apd : 0.30;
+    const double t_up = 4.0 * tau_up;
+    const double t_down = t_up + apd;
+    const double phi_up = 1.0 / (1.0 + exp(-(t - t_up) / tau_up));
+    const double phi_down = 1.0 / (1.0 + exp((t - t_down) / tau_down));
+    return p.gamma * phi_up * phi_down;
+  }
+  if (p.type == 4) {
+    const double s = (t - p.pulse_t0) / p.pulse_sigma;
+    return p.gamma * exp(-0.5 * s * s);
+  }
+  const double T = (p.gamma_T > 0.0) ? p.gamma_T : p.T_default;
+  if (t <= 0.0) return 0.0;
+  const double s = (p.type == 1) ? sin(M_PI * t / T) : sin(M_PI * 0.5 * t / T);
+  return p.gamma * s * s;
+}
+
+// ------------------------------------------------------------------ kernels ------------
+__device__ inline V3<double> ld3(const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, int n) {
+  return mk(x[n], y[n], z[n]);
+}
+
+// Reference midsurface cache (ElemUpdateGeom0Subdiv over all cells, dmplex_geom.c:859-865),
+// stored as a1, a2, a3, a3_1, a3_2 -- everything ElemUpdateG reads from geom0 (:925-931).
+__global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E, int NS,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       double* __restrict__ ref) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int meta = m.meta[e];
+  const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
+  MidGeom G;
+  auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
+  if (kind == kRegular) {
+    G = mid_geom(12, c_Na[0], c_Na[1], c_Nab[0], c_Nab[1], c_Nab[2], xyz);
+  } else if (kind == kIrregular) {
+    const double(*t)[kMaxNen] = c_IN[val];
+    G = mid_geom(nen, t[0], t[1], t[2], t[3], t[4], xyz);
+  } else {
+    G = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),
+                 ld3(x, y, z, m.corner[(size_t)2 * E + e]));
+  }
+  V3<double> a31, a32;
+  a3_alpha(G.a1, G.a2, G.Aaa, G.Aab, G.Abb, a31, a32);
+  const V3<double> v[5] = {G.a1, G.a2, G.a3, a31, a32};
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    ref[(size_t)(3 * k + 0) * E + e] = v[k].x;
+    ref[(size_t)(3 * k + 1) * E + e] = v[k].y;
+    ref[(size_t)(3 * k + 2) * E + e] = v[k].z;
+  }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z) {
+  const int E = kp.E;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int meta = m.meta[e];
+  const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
+  auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
+
+  // ---- current midsurface geometry (ElemUpdateGeomSubdiv, active_strain.c:727) ----
+  MidGeom G;
+  const double *na0, *na1, *nb0, *nb1, *nb2;
+  if (kind == kRegular) {
+    na0 = c_Na[0]; na1 = c_Na[1]; nb0 = c_Nab[0]; nb1 = c_Nab[1]; nb2 = c_Nab[2];
+    G = mid_geom(12, na0, na1, nb0, nb1, nb2, xyz);
+  } else if (kind == kIrregular) {
+    const double(*t)[kMaxNen] = c_IN[val];
+    na0 = t[0]; na1 = t[1]; nb0 = t[2]; nb1 = t[3]; nb2 = t[4];
+    G = mid_geom(nen, na0, na1, nb0, nb1, nb2, xyz);
+  } else {
+    na0 = na1 = nb0 = nb1 = nb2 = nullptr;
+    G = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),
+                 ld3(x, y, z, m.corner[(size_t)2 * E + e]));
+  }
+  V3<double> a31c, a32c;
+  a3_alpha(G.a1, G.a2, G.Aaa, G.Aab, G.Abb, a31c, a32c);
+
+  // ---- reference cache ----
+  RefGeom rg;
+  const double* rf = m.ref;
+  rg.a1 = mk(rf[(size_t)0 * E + e], rf[(size_t)1 * E + e], rf[(size_t)2 * E + e]);
+  rg.a2 = mk(rf[(size_t)3 * E + e], rf[(size_t)4 * E + e], rf[(size_t)5 * E + e]);
+  rg.a3 = mk(rf[(size_t)6 * E + e], rf[(size_t)7 * E + e], rf[(size_t)8 * E + e]);
+  rg.a31 = mk(rf[(size_t)9 * E + e], rf[(size_t)10 * E + e], rf[(size_t)11 * E + e]);
+  rg.a32 = mk(rf[(size_t)12 * E + e], rf[(size_t)13 * E + e], rf[(size_t)14 * E + e]);
+
+  // ---- activation (ElemActDefGrad, active_strain.c:1208-1257) ----
+  double g = kp.has_delay ? ramp_gamma(kp.ramp, m.delay[e]) : kp.gamma_t;
+  g *= m.gscale ? m.gscale[e] : 1.0;
+  Activation act;
+  act.f = mk(m.fib[e], m.fib[(size_t)E + e], m.fib[(size_t)2 * E + e]);
+  if (kp.act_mode == 0) {
+    act.d0 = act.d1 = act.d2 = 1.0;
+    act.gam = g;
+  } else if (kp.act_mode == 1) {
+    act.d0 = 1.0 - kp.man_gamma;
+    act.d1 = act.d2 = 1.0;
+    act.gam = 0.0;
+  } else {
+    act.d0 = act.d1 = 1.0 - g;
+    act.d2 = 1.0 / (act.d0 * act.d1);
+    act.gam = 0.0;
+  }
+
+  const Resultants<R> r = integrate_element<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0,
+                                               kp.mu, kp.K, kp.n_qp, kp.iters, kp.theta, kp.w);
+  for (int q = 0; q < kp.n_qp; q++) m.c33[(size_t)q * E + e] = r.c33[q];
+
+  // ---- nodal forces (ElemUpdFint, active_strain.c:1762-1995) ----
+  const double pref0 = 0.5 * kp.h0 * m.dA0[e];
+  const double hh = 0.5 * kp.h0;
+  double* Fe = m.Fe;
+  if (kind != kCap) {
+    for (int a = 0; a < nen; a++) {
+      const V3<double> f = node_force(na0[a], na1[a], nb0[a], nb1[a], nb2[a], G, pref0, hh, r.N0,
+                                      r.N1, r.N2, r.M0, r.M1, r.M2);
+      Fe[(size_t)(3 * a + 0) * E + e] = f.x;
+      Fe[(size_t)(3 * a + 1) * E + e] = f.y;
+      Fe[(size_t)(3 * a + 2) * E + e] = f.z;
+    }
+  } else {
+    // CST membrane into the patch slots holding nv1..nv3 (:1807-1851); other slots zero
+    for (int a = 0; a < nen; a++) {
+      Fe[(size_t)(3 * a + 0) * E + e] = 0.0;
+      Fe[(size_t)(3 * a + 1) * E + e] = 0.0;
+      Fe[(size_t)(3 * a + 2) * E + e] = 0.0;
+    }
+    const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+    for (int a = 0; a < 3; a++) {
+      const int s = m.corner[(size_t)(3 + a) * E + e];
+      if (s < 0) continue;
+      const V3<double> f = node_force(dN[a][0], dN[a][1], 0.0, 0.0, 0.0, G, pref0, hh, r.N0,
+                                      r.N1, r.N2, 0.0, 0.0, 0.0);
+      Fe[(size_t)(3 * s + 0) * E + e] += f.x;
+      Fe[(size_t)(3 * s + 1) * E + e] += f.y;
+      Fe[(size_t)(3 * s + 2) * E + e] += f.z;
+    }
+  }
+  // ---- status ----
+  const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) && isfinite(r.M0) &&
+                      isfinite(r.M1) && isfinite(r.M2);
+  if (!finite) {
+    atomicMin(&m.status[0], e);
+  }
+  if (r.status & kStatCC3333) {
+    atomicMin(&m.status[2], e);
+    atomicAdd(&m.status[3], 1);
+  }
+}
+
+// Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + sum over the
+// node's (element, slot) entries in ascending element order.  Ghost rows (n >= n_v) are
+// zeroed when not accumulating, as after the caller's VecSet(Fint, 0).
+__global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ ptr,
+                                                        const unsigned* __restrict__ ent,
+                                                        const double* __restrict__ Fe, int E,
+                                                        int n_v, int n_nodes, int accumulate,
+                                                        double* __restrict__ fint) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  if (n >= n_v) {
+    if (!accumulate) {
+      fint[3 * (size_t)n + 0] = 0.0;
+      fint[3 * (size_t)n + 1] = 0.0;
+      fint[3 * (size_t)n + 2] = 0.0;
+    }
+    return;
+  }
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (accumulate) {
+    ax = fint[3 * (size_t)n + 0];
+    ay = fint[3 * (size_t)n + 1];
+    az = fint[3 * (size_t)n + 2];
+  }
+  const int b = ptr[n], en = ptr[n + 1];
+  for (int j = b; j < en; j++) {
+    const unsigned v = ent[j];
+    const int e = (int)(v >> 5), s = (int)(v & 31u);
+    ax += Fe[(size_t)(3 * s + 0) * E + e];
+    ay += Fe[(size_t)(3 * s + 1) * E + e];
+    az += Fe[(size_t)(3 * s + 2) * E + e];
+  }
+  fint[3 * (size_t)n + 0] = ax;
+  fint[3 * (size_t)n + 1] = ay;
+  fint[3 * (size_t)n + 2] = az;
+}
+
+// GlobalGhost (src/bending.c:1763-1814): ghost n_v+g = x[b] + x[a] - x[c] per rule (b,a,c).
+__global__ void __launch_bounds__(256) bf_ghost_kernel(const int* __restrict__ rules, int G,
+                                                       int n_v, double* __restrict__ x,
+                                                       double* __restrict__ y,
+                                                       double* __restrict__ z) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int b = rules[3 * g], a = rules[3 * g + 1], c = rules[3 * g + 2];
+  x[n_v + g] = x[b] + x[a] - x[c];
+  y[n_v + g] = y[b] + y[a] - y[c];
+  z[n_v + g] = z[b] + z[a] - z[c];
+}
+
+__global__ void bf_status_reset(int* st) {
+  if (threadIdx.x == 0) {
+    st[0] = INT_MAX;
+    st[1] = 0;
+    st[2] = INT_MAX;
+    st[3] = 0;
+  }
+}
+
+// ------------------------------------------------------------------ host side ----------
+struct bf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  bool timing = false;
+  hipEvent_t own_ev[3] = {nullptr, nullptr, nullptr};
+  // timing ring: 3 events per evaluation, harvested when full or on bf_timing_read
+  static constexpr int kRing = 64;
+  std::vector<hipEvent_t> ring;
+  int ring_n = 0;
+  bool ring_on = false;
+  long long t_count = 0;
+  double t_elem = 0.0, t_gather = 0.0;
+  // mesh
+  int n_v = 0, n_nodes = 0, E = 0, NS = 12, n_elmt_base = 0, nnz = 0, n_ghost = 0;
+  bool has_mesh = false, has_params = false, has_delay = false, has_cap = false;
+  int* d_patch = nullptr;
+  int* d_meta = nullptr;
+  int* d_corner = nullptr;
+  double* d_ref = nullptr;
+  double* d_dA0 = nullptr;
+  double* d_fib = nullptr;
+  double* d_gscale = nullptr;
+  double* d_delay = nullptr;
+  double* d_Fe = nullptr;
+  double* d_c33 = nullptr;
+  int* d_ptr = nullptr;
+  unsigned* d_ent = nullptr;
+  int* d_rules = nullptr;
+  int* d_status = nullptr;
+  int* h_status = nullptr;  // pinned
+  double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_fint = nullptr;
+  // params
+  bf_params p{};
+  KParams kp{};
+};
+
+namespace {
+
+int g_tables_device = -1;
+
+#define BF_HIP(call)                                                                      \
+  do {                                                                                   \
+    hipError_t _e = (call);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      fprintf(stderr, "biofem_hip: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(_e), \
+              __FILE__, __LINE__);                                                       \
+      return (_e == hipErrorOutOfMemory) ? BF_ERR_NOMEM : BF_ERR_DEVICE;                 \
+    }                                                                                    \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  BF_HIP(hipMalloc((void**)p, n * sizeof(T)));
+  return BF_OK;
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_mesh(bf_ctx* c) {
+  dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
+  dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
+  dfree(c->d_ptr); dfree(c->d_ent); dfree(c->d_rules); dfree(c->d_x); dfree(c->d_y);
+  dfree(c->d_z); dfree(c->d_fint);
+  c->has_mesh = false;
+}
+
+int upload_tables(int device) {
+  if (g_tables_device == device) return BF_OK;
+  static double tab[13][5][kMaxNen];
+  memset(tab, 0, sizeof(tab));
+  std::vector<double> buf(5 * kMaxNen);
+  for (int v = 3; v <= 12; v++) {
+    if (bf_loop_tables(v, buf.data()) != BF_OK) return BF_ERR_ARG;
+    const int nen = v + 6;
+    for (int r = 0; r < 5; r++)
+      for (int j = 0; j < nen; j++) tab[v][r][j] = buf[(size_t)r * nen + j];
+  }
+  BF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_IN), tab, sizeof(tab)));
+  g_tables_device = device;
+  return BF_OK;
+}
+
+int harvest_ring(bf_ctx* c) {
+  for (int i = 0; i < c->ring_n; i++) {
+    BF_HIP(hipEventSynchronize(c->ring[3 * i + 2]));
+    float a = 0.f, b = 0.f;
+    BF_HIP(hipEventElapsedTime(&a, c->ring[3 * i], c->ring[3 * i + 1]));
+    BF_HIP(hipEventElapsedTime(&b, c->ring[3 * i + 1], c->ring[3 * i + 2]));
+    c->t_elem += a;
+    c->t_gather += b;
+    c->t_count++;
+  }
+  c->ring_n = 0;
+  return BF_OK;
+}
+
+int kernel_launch_check() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "biofem_hip: kernel launch failed: %s\n", hipGetErrorString(e));
+    return BF_ERR_DEVICE;
+  }
+  return BF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bf_version(void) { return "biofem-hip 0.1 gfx950"; }
+
+int bf_ctx_create(int device, bf_ctx** out) {
+  if (!out) return BF_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  BF_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(device));
+  bf_ctx* c = new (std::nothrow) bf_ctx();
+  if (!c) return BF_ERR_NOMEM;
+  c->device = device;
+  int rc = upload_tables(device);
+  if (rc != BF_OK) { delete c; return rc; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->own_ev[0]) != hipSuccess || hipEventCreate(&c->own_ev[1]) != hipSuccess ||
+      hipEventCreate(&c->own_ev[2]) != hipSuccess ||
+      hipMalloc((void**)&c->d_status, 4 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_status, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    bf_ctx_destroy(c);
+    return BF_ERR_DEVICE;
+  }
+  for (int i = 0; i < 3; i++) c->ev[i] = c->own_ev[i];
+  *out = c;
+  return BF_OK;
+}
+
+int bf_ctx_destroy(bf_ctx* c) {
+  if (!c) return BF_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_mesh(c);
+  dfree(c->d_status);
+  if (c->h_status) (void)hipHostFree(c->h_status);
+  for (auto& e : c->own_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->ring)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BF_OK;
+}
+
+int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
+  if (!c || !m) return BF_ERR_ARG;
+  if (m->n_v < 0 || m->n_nodes < m->n_v || m->n_elmt < 0 || !m->patch || !m->ire || !m->val ||
+      !m->x0 || !m->y0 || !m->z0 || !m->dA0 || !m->n_fib || m->patch_stride < 12 ||
+      m->n_nodes >= BF_PATCH_SENTINEL || m->n_elmt >= (1 << 26))
+    return BF_ERR_ARG;
+  if (m->n_elmt_base > 0 && (!m->nv1 || !m->nv2 || !m->nv3)) return BF_ERR_ARG;
+  if (m->n_ghost_rules < 0 || (m->n_ghost_rules > 0 && !m->ghost_rules) ||
+      m->n_v + m->n_ghost_rules > m->n_nodes)
+    return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  const int E = m->n_elmt, S = m->patch_stride;
+  // ---- validate and pack topology (the per-residual checks of active_strain.c:498, :577,
+  //      :599, :721 are done once here; they depend on topology only) ----
+  std::vector<int> meta((size_t)E), nenv((size_t)E);
+  int NS = 12;
+  bool cap = false;
+  for (int e = 0; e < E; e++) {
+    const bool is_cap = m->n_elmt_base > 0 && e >= m->n_elmt_base;
+    int kind, nen, v = m->val[e];
+    if (is_cap) {
+      kind = kCap;
+      nen = (m->ire[e] == 0) ? 12 : v + 6;  // assembly width of FInternalAct (:2245)
+      cap = true;
+    } else if (m->ire[e] == 0) {
+      kind = kRegular;
+      nen = 12;
+    } else if (m->ire[e] == 1) {
+      kind = kIrregular;
+      nen = v + 6;
+      if (v < 3 || v > 12) {
+        fprintf(stderr, "biofem_hip: element %d valence %d outside 3..12\n", e, v);
+        return (v + 6 > BF_MAX_NEN) ? BF_ERR_UNSUPPORTED : BF_ERR_ARG;
+      }
+    } else {
+      fprintf(stderr, "biofem_hip: element %d has ire=%d\n", e, m->ire[e]);
+      return BF_ERR_ARG;
+    }
+    if (nen > S || nen > BF_MAX_NEN) {
+      fprintf(stderr, "biofem_hip: element %d needs %d patch slots (stride %d)\n", e, nen, S);
+      return BF_ERR_UNSUPPORTED;
+    }
+    if (!is_cap)
+      for (int i = 0; i < nen; i++) {
+        const int n = m->patch[(size_t)S * e + i];
+        if (n == BF_PATCH_SENTINEL || n < 0 || n >= m->n_nodes) {
+          fprintf(stderr, "biofem_hip: element %d patch slot %d incomplete (node %d)\n", e, i, n);
+          return BF_ERR_ELEMENT;
+        }
+      }
+    NS = std::max(NS, nen);
+    nenv[e] = nen;
+    meta[e] = kind | ((v & 255) << 8) | (nen << 16);
+  }
+  std::vector<int> patch((size_t)NS * std::max(E, 1), 0);
+  for (int e = 0; e < E; e++)
+    for (int i = 0; i < NS; i++) {
+      int n = (i < S) ? m->patch[(size_t)S * e + i] : BF_PATCH_SENTINEL;
+      // slots beyond nen are never read; park them on node 0 so any index is in range
+      if (i >= nenv[e] || n == BF_PATCH_SENTINEL || n < 0 || n >= m->n_nodes) n = 0;
+      patch[(size_t)i * E + e] = n;
+    }
+  std::vector<int> corner;
+  if (cap) {
+    corner.assign((size_t)6 * E, 0);
+    for (int e = 0; e < E; e++) {
+      const int tri[3] = {m->nv1[e], m->nv2[e], m->nv3[e]};
+      for (int a = 0; a < 3; a++) {
+        corner[(size_t)a * E + e] = tri[a];
+        int slot = -1;
+        if (m->n_elmt_base > 0 && e >= m->n_elmt_base) {
+          const int nslot = m->val[e] + 6;  // :1811-1819
+          for (int i = 0; i < nslot && i < S; i++)
+            if (m->patch[(size_t)S * e + i] == tri[a]) { slot = i; break; }
+          if (slot < 0 || slot >= nenv[e]) {
+            fprintf(stderr, "biofem_hip: cap element %d corner %d missing from patch\n", e, tri[a]);
+            return BF_ERR_ELEMENT;
+          }
+        }
+        corner[(size_t)(3 + a) * E + e] = slot;
+      }
+    }
+  }
+  // ---- node -> (element, slot) CSR in ascending (element, slot) order (:2238-2260) ----
+  std::vector<int> ptr((size_t)m->n_v + 1, 0);
+  for (int e = 0; e < E; e++)
+    for (int i = 0; i < nenv[e]; i++) {
+      const int n = m->patch[(size_t)S * e + i];
+      if (n >= 0 && n < m->n_v) ptr[n + 1]++;
+    }
+  for (int n = 0; n < m->n_v; n++) ptr[n + 1] += ptr[n];
+  const int nnz = ptr[m->n_v];
+  std::vector<unsigned> ent((size_t)std::max(nnz, 1));
+  {
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < E; e++)
+      for (int i = 0; i < nenv[e]; i++) {
+        const int n = m->patch[(size_t)S * e + i];
+        if (n >= 0 && n < m->n_v) ent[fill[n]++] = ((unsigned)e << 5) | (unsigned)i;
+      }
+  }
+  // ---- per-element SoA data ----
+  std::vector<double> fib((size_t)3 * std::max(E, 1)), gs((size_t)std::max(E, 1), 1.0);
+  for (int e = 0; e < E; e++) {
+    fib[e] = m->n_fib[3 * (size_t)e];
+    fib[(size_t)E + e] = m->n_fib[3 * (size_t)e + 1];
+    fib[(size_t)2 * E + e] = m->n_fib[3 * (size_t)e + 2];
+    if (m->gamma_scale) gs[e] = m->gamma_scale[e];
+  }
+  free_mesh(c);
+  c->n_v = m->n_v;
+  c->n_nodes = m->n_nodes;
+  c->E = E;
+  c->NS = NS;
+  c->n_elmt_base = m->n_elmt_base;
+  c->nnz = nnz;
+  c->n_ghost = m->n_ghost_rules;
+  c->has_delay = m->activation_delay != nullptr;
+  c->has_cap = cap;
+  const size_t En = (size_t)std::max(E, 1), Nn = (size_t)std::max(m->n_nodes, 1);
+  int rc;
+#define ALLOC(p, n) \
+  if ((rc = dalloc(&(p), (n))) != BF_OK) { free_mesh(c); return rc; }
+  ALLOC(c->d_patch, (size_t)NS * En);
+  ALLOC(c->d_meta, En);
+  ALLOC(c->d_ref, 15 * En);
+  ALLOC(c->d_dA0, En);
+  ALLOC(c->d_fib, 3 * En);
+  ALLOC(c->d_gscale, En);
+  ALLOC(c->d_Fe, (size_t)3 * NS * En);
+  ALLOC(c->d_c33, (size_t)kMaxQp * En);
+  ALLOC(c->d_ptr, (size_t)m->n_v + 1);
+  ALLOC(c->d_ent, (size_t)std::max(nnz, 1));
+  ALLOC(c->d_x, Nn);
+  ALLOC(c->d_y, Nn);
+  ALLOC(c->d_z, Nn);
+  ALLOC(c->d_fint, 3 * Nn);
+  if (c->has_delay) ALLOC(c->d_delay, En);
+  if (cap) ALLOC(c->d_corner, 6 * En);
+  if (c->n_ghost > 0) ALLOC(c->d_rules, (size_t)3 * c->n_ghost);
+#undef ALLOC
+  hipStream_t s = c->stream;
+  BF_HIP(hipMemcpyAsync(c->d_patch, patch.data(), sizeof(int) * (size_t)NS * En, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_meta, meta.data(), sizeof(int) * (size_t)E, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_dA0, m->dA0, sizeof(double) * (size_t)E, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_fib, fib.data(), sizeof(double) * 3 * (size_t)E, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_gscale, gs.data(), sizeof(double) * (size_t)E, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_ptr, ptr.data(), sizeof(int) * ((size_t)m->n_v + 1), hipMemcpyHostToDevice, s));
+  if (nnz > 0)
+    BF_HIP(hipMemcpyAsync(c->d_ent, ent.data(), sizeof(unsigned) * (size_t)nnz, hipMemcpyHostToDevice, s));
+  if (c->has_delay)
+    BF_HIP(hipMemcpyAsync(c->d_delay, m->activation_delay, sizeof(double) * (size_t)E, hipMemcpyHostToDevice, s));
+  if (cap)
+    BF_HIP(hipMemcpyAsync(c->d_corner, corner.data(), sizeof(int) * 6 * (size_t)E, hipMemcpyHostToDevice, s));
+  if (c->n_ghost > 0)
+    BF_HIP(hipMemcpyAsync(c->d_rules, m->ghost_rules, sizeof(int) * 3 * (size_t)c->n_ghost, hipMemcpyHostToDevice, s));
+  // reference coordinates -> geom0 cache (scratch: the resident coordinate buffers)
+  BF_HIP(hipMemcpyAsync(c->d_x, m->x0, sizeof(double) * (size_t)m->n_nodes, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_y, m->y0, sizeof(double) * (size_t)m->n_nodes, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_z, m->z0, sizeof(double) * (size_t)m->n_nodes, hipMemcpyHostToDevice, s));
+  if (E > 0) {
+    DevMesh dm{};
+    dm.patch = c->d_patch;
+    dm.meta = c->d_meta;
+    dm.corner = c->d_corner;
+    hipLaunchKernelGGL(bf_geom0_kernel, dim3((E + 255) / 256), dim3(256), 0, s, dm, E, NS,
+                       c->d_x, c->d_y, c->d_z, c->d_ref);
+    if ((rc = kernel_launch_check()) != BF_OK) return rc;
+  }
+  BF_HIP(hipStreamSynchronize(s));
+  c->has_mesh = true;
+  return BF_OK;
+}
+
+int bf_params_set(bf_ctx* c, const bf_params* p) {
+  if (!c || !p) return BF_ERR_ARG;
+  if (p->n_qp < 1 || p->n_qp > 5 || p->c33_iters < 1 || p->c33_iters > 31 || !(p->h0 > 0.0))
+    return BF_ERR_ARG;
+  if (p->ramp_type < 0 || p->ramp_type > 4) return BF_ERR_ARG;
+  if (p->ramp_type == 4 && !(p->pulse_sigma > 0.0)) return BF_ERR_ARG;
+  c->p = *p;
+  KParams& k = c->kp;
+  k.n_qp = p->n_qp;
+  k.iters = p->c33_iters;
+  // SetGaussianQuadrature (active_strain.c:184-222)
+  static const double th[5][5] = {{0.0},
+                                  {-0.5773502691896257, 0.5773502691896257},
+                                  {-0.7745966692414834, 0.0, 0.7745966692414834},
+                                  {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526},
+                                  {-0.9061798459386640, -0.5384693101056831, 0.0, 0.5384693101056831, 0.9061798459386640}};
+  static const double wt[5][5] = {{2.0},
+                                  {1.0, 1.0},
+                                  {0.5555555555555556, 0.8888888888888888, 0.5555555555555556},
+                                  {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538},
+                                  {0.2369268850561891, 0.4786286704993665, 0.5688888888888889, 0.4786286704993665, 0.2369268850561891}};
+  for (int i = 0; i < kMaxQp; i++) {
+    k.theta[i] = i < p->n_qp ? th[p->n_qp - 1][i] : 0.0;
+    k.w[i] = i < p->n_qp ? wt[p->n_qp - 1][i] : 0.0;
+  }
+  k.h0 = p->h0;
+  k.mu = p->mu;
+  k.K = p->K;
+  k.act_mode = p->cart_fib_act ? 0 : (p->manufactured ? 1 : 2);
+  RampParams& r = k.ramp;
+  r.gamma = p->gamma;
+  r.gamma_T = p->gamma_T;
+  r.tau_up = p->tau_up;
+  r.tau_down = p->tau_down;
+  r.apd = p->apd;
+  r.pulse_t0 = p->pulse_t0;
+  r.pulse_sigma = p->pulse_sigma;
+  r.t = (double)p->ti * p->dt;
+  r.T_default = (double)p->tisteps * p->dt;
+  r.type = p->ramp_type;
+  k.gamma_t = ramp_gamma(r, 0.0);
+  {
+    const double t = p->ti * p->dt;  // active_strain.c:1241-1244
+    const double s = sin(M_PI * t / p->manufactured_T);
+    k.man_gamma = p->manufactured ? p->manufactured_gamma0 * pow(s, 2.0) : 0.0;
+  }
+  c->has_params = true;
+  return BF_OK;
+}
+
+static int launch_eval(bf_ctx* c, const double* x, const double* y, const double* z,
+                       double* fint, int accumulate, int precision, hipStream_t s) {
+  if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
+  if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED) return BF_ERR_ARG;
+  KParams kp = c->kp;
+  kp.E = c->E;
+  kp.NS = c->NS;
+  kp.has_delay = c->has_delay ? 1 : 0;
+  DevMesh dm{};
+  dm.patch = c->d_patch;
+  dm.meta = c->d_meta;
+  dm.corner = c->d_corner;
+  dm.ref = c->d_ref;
+  dm.dA0 = c->d_dA0;
+  dm.fib = c->d_fib;
+  dm.gscale = c->d_gscale;
+  dm.delay = c->d_delay;
+  dm.Fe = c->d_Fe;
+  dm.c33 = c->d_c33;
+  dm.status = c->d_status;
+  hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
+  hipEvent_t e0 = c->own_ev[0], e1 = c->own_ev[1], e2 = c->own_ev[2];
+  c->ev[0] = e0;
+  c->ev[1] = e1;
+  c->ev[2] = e2;
+  if (c->ring_on) {
+    if (c->ring_n == bf_ctx::kRing) {
+      int rc = harvest_ring(c);
+      if (rc != BF_OK) return rc;
+    }
+    e0 = c->ring[3 * c->ring_n];
+    e1 = c->ring[3 * c->ring_n + 1];
+    e2 = c->ring[3 * c->ring_n + 2];
+    c->ring_n++;
+  }
+  BF_HIP(hipEventRecord(e0, s));
+  if (c->E > 0) {
+    const dim3 grid((c->E + 255) / 256), block(256);
+    if (precision == BF_PREC_FP64)
+      hipLaunchKernelGGL(bf_element_kernel<double>, grid, block, 0, s, dm, kp, x, y, z);
+    else
+      hipLaunchKernelGGL(bf_element_kernel<float>, grid, block, 0, s, dm, kp, x, y, z);
+  }
+  BF_HIP(hipEventRecord(e1, s));
+  if (c->n_nodes > 0)
+    hipLaunchKernelGGL(bf_gather_kernel, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s,
+                       c->d_ptr, c->d_ent, c->d_Fe, c->E, c->n_v, c->n_nodes, accumulate, fint);
+  BF_HIP(hipEventRecord(e2, s));
+  if (c->ring_on) {
+    c->ev[0] = e0;  // bf_last_timing reads the latest triple
+    c->ev[1] = e1;
+    c->ev[2] = e2;
+  }
+  c->timing = true;
+  return kernel_launch_check();
+}
+
+static int collect_status(bf_ctx* c, int* elem, int* code, int* n_warn) {
+  BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  const int* st = c->h_status;
+  int el = -1, cd = 0;
+  if (st[0] != INT_MAX) { el = st[0]; cd = BF_ELEM_NONFINITE; }
+  else if (st[2] != INT_MAX) { el = st[2]; cd = BF_ELEM_CC3333; }
+  if (elem) *elem = el;
+  if (code) *code = cd;
+  if (n_warn) *n_warn = st[3];
+  return (st[0] != INT_MAX) ? BF_ERR_ELEMENT : BF_OK;
+}
+
+int bf_force_eval(bf_ctx* c, const double* x, const double* y, const double* z, double* fint,
+                  int accumulate, int precision) {
+  if (!c || !x || !y || !z || !fint) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const size_t N = (size_t)c->n_nodes;
+  BF_HIP(hipMemcpyAsync(c->d_x, x, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_y, y, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_z, z, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  if (accumulate)
+    BF_HIP(hipMemcpyAsync(c->d_fint, fint, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
+  int rc = launch_eval(c, c->d_x, c->d_y, c->d_z, c->d_fint, accumulate, precision, s);
+  if (rc != BF_OK) return rc;
+  BF_HIP(hipMemcpyAsync(fint, c->d_fint, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
+  return collect_status(c, nullptr, nullptr, nullptr);
+}
+
+int bf_force_eval_device(bf_ctx* c, const double* d_x, const double* d_y, const double* d_z,
+                         double* d_fint, int accumulate, int precision, void* stream) {
+  if (!c || !d_x || !d_y || !d_z || !d_fint) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return launch_eval(c, d_x, d_y, d_z, d_fint, accumulate, precision, s);
+}
+
+int bf_ghost_update_device(bf_ctx* c, double* d_x, double* d_y, double* d_z, void* stream) {
+  if (!c || !d_x || !d_y || !d_z) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  if (c->n_ghost == 0) return BF_OK;
+  BF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipLaunchKernelGGL(bf_ghost_kernel, dim3((c->n_ghost + 255) / 256), dim3(256), 0, s,
+                     c->d_rules, c->n_ghost, c->n_v, d_x, d_y, d_z);
+  return kernel_launch_check();
+}
+
+int bf_ghost_update(bf_ctx* c, double* x, double* y, double* z) {
+  if (!c || !x || !y || !z) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const size_t N = (size_t)c->n_nodes;
+  BF_HIP(hipMemcpyAsync(c->d_x, x, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_y, y, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_z, z, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  int rc = bf_ghost_update_device(c, c->d_x, c->d_y, c->d_z, s);
+  if (rc != BF_OK) return rc;
+  BF_HIP(hipMemcpyAsync(x, c->d_x, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  BF_HIP(hipMemcpyAsync(y, c->d_y, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  BF_HIP(hipMemcpyAsync(z, c->d_z, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  BF_HIP(hipStreamSynchronize(s));
+  return BF_OK;
+}
+
+int bf_c33_download(bf_ctx* c, double* c33) {
+  if (!c || !c33) return BF_ERR_ARG;
+  if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  const int q = c->kp.n_qp, E = c->E;
+  std::vector<double> buf((size_t)q * std::max(E, 1));
+  BF_HIP(hipMemcpyAsync(buf.data(), c->d_c33, sizeof(double) * (size_t)q * E, hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  for (int e = 0; e < E; e++)
+    for (int k = 0; k < q; k++) c33[(size_t)e * q + k] = buf[(size_t)k * E + e];
+  return BF_OK;
+}
+
+int bf_update_thickness(bf_ctx* c, double* thickness) {
+  if (!c || !thickness) return BF_ERR_ARG;
+  if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
+  const int q = c->kp.n_qp, E = c->E;
+  std::vector<double> c33((size_t)q * std::max(E, 1));
+  int rc = bf_c33_download(c, c33.data());
+  if (rc != BF_OK) return rc;
+  for (int e = 0; e < E; e++) {  // active_strain.c:2211-2216
+    double sum = 0.0;
+    for (int k = 0; k < q; k++) sum += c33[(size_t)e * q + k];
+    const double C33 = (q > 0) ? sum / (double)q : 1.0;
+    const double lam = (C33 > 0.0) ? std::sqrt(C33) : 1.0;
+    thickness[e] = c->p.h0 * lam;
+  }
+  return BF_OK;
+}
+
+int bf_last_error(bf_ctx* c, int* elem, int* code, int* n_warn) {
+  if (!c) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  collect_status(c, elem, code, n_warn);
+  return BF_OK;
+}
+
+int bf_last_timing(bf_ctx* c, float* ms_element, float* ms_gather) {
+  if (!c) return BF_ERR_ARG;
+  if (!c->timing) return BF_ERR_STATE;
+  BF_HIP(hipEventSynchronize(c->ev[2]));
+  float a = 0.f, b = 0.f;
+  BF_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+  BF_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+  if (ms_element) *ms_element = a;
+  if (ms_gather) *ms_gather = b;
+  return BF_OK;
+}
+
+int bf_timing_enable(bf_ctx* c, int on) {
+  if (!c) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  if (on && c->ring.empty()) {
+    c->ring.assign(3 * bf_ctx::kRing, nullptr);
+    for (auto& e : c->ring) BF_HIP(hipEventCreate(&e));
+  }
+  if (!on && c->ring_on) {
+    int rc = harvest_ring(c);
+    if (rc != BF_OK) return rc;
+  }
+  c->ring_on = on != 0;
+  c->ring_n = 0;
+  c->t_count = 0;
+  c->t_elem = c->t_gather = 0.0;
+  return BF_OK;
+}
+
+int bf_timing_read(bf_ctx* c, long long* count, double* ms_element, double* ms_gather) {
+  if (!c) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  int rc = harvest_ring(c);
+  if (rc != BF_OK) return rc;
+  if (count) *count = c->t_count;
+  if (ms_element) *ms_element = c->t_elem;
+  if (ms_gather) *ms_gather = c->t_gather;
+  return BF_OK;
+}
+
+int bf_device_buffers(bf_ctx* c, double** d_x, double** d_y, double** d_z, double** d_fint) {
+  if (!c) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  if (d_x) *d_x = c->d_x;
+  if (d_y) *d_y = c->d_y;
+  if (d_z) *d_z = c->d_z;
+  if (d_fint) *d_fint = c->d_fint;
+  return BF_OK;
+}
+
+int bf_sync(bf_ctx* c) {
+  if (!c) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  return BF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+/*
+ * biofem_hip.h -- C ABI of the MI355X (gfx950) element internal-force library.
+ *
+ * Drop-in boundary for BioFEM-IA's active-strain force evaluation.  The reference entry
+ * point is
+ *     PetscErrorCode FInternalAct(FE *fem);          include/active_strain.h:68,
+ *                                                    src/active_strain.c:2226-2267
+ * called once per residual by FormFunctionFEM (src/main.c:834-836).  Its reads (current
+ * coordinates x_bp/y_bp/z_bp incl. ghosts, the patch table, ire/val, dA0, n_fib,
+ * gamma_scale, activation_delay, the geom0 cache and the globals h0/ti/dt/mu/K) become the
+ * arguments below; its output Vec Fint becomes an interleaved [3 * n_nodes] double array.
+ * The existing CUDA bridge (include/cuda_bridge.h:10-15) is the pattern: extern "C",
+ * plain pointers, an error code per call.
+ *
+ * All functions return BF_OK (0) or a negative BF_ERR_* code.  Nothing throws across the
+ * ABI.  Host pointers are borrowed for the duration of the call only.  A context owns its
+ * device memory and may be used from one host thread at a time.
+ */
+#ifndef BIOFEM_HIP_H
+#define BIOFEM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (mapping to the reference's PetscErrorCode classes) ---- */
+#define BF_OK 0
+#define BF_ERR_ARG (-1)         /* PETSC_ERR_ARG_* : bad size, NULL, option out of range */
+#define BF_ERR_DEVICE (-2)      /* PETSC_ERR_LIB   : HIP runtime failure */
+#define BF_ERR_NOMEM (-3)       /* PETSC_ERR_MEM   : device allocation failed */
+#define BF_ERR_ELEMENT (-4)     /* PETSC_ERR_PLIB  : per-element failure, see bf_last_error */
+#define BF_ERR_STATE (-5)       /* call order (e.g. eval before mesh upload) */
+#define BF_ERR_UNSUPPORTED (-6) /* PETSC_ERR_SUP   : e.g. patch wider than 18 nodes */
+
+/* per-element codes reported by bf_last_error() */
+#define BF_ELEM_STENCIL 2   /* patch slot empty (active_strain.c:498-500, :577-579) */
+#define BF_ELEM_IRE 3       /* ire not 0/1 (active_strain.c:721-723) */
+#define BF_ELEM_NEN 4       /* nen above the patch width (active_strain.c:599-600) */
+#define BF_ELEM_CC3333 6    /* |CC3333| <= 1e-14 (active_strain.c:1674-1681); non-fatal,
+                               the C33 update is skipped as in the reference */
+#define BF_ELEM_NONFINITE 7 /* NaN/Inf in an element force */
+
+/* precision modes of the force evaluation */
+#define BF_PREC_FP64 0  /* plain FP64 throughout */
+#define BF_PREC_MIXED 1 /* FP32 per-quadrature-point geometry/constitutive arithmetic,
+                           FP64 coordinates, midsurface sums and nodal accumulation */
+
+#define BF_PATCH_SENTINEL 1000000 /* empty patch slot (src/io.c:306) */
+#define BF_MAX_NEN 18             /* widest patch (valence 12) */
+
+typedef struct bf_ctx bf_ctx;
+
+/* Mesh arrays the force path reads (the IBMNodes fields of src/active_strain.c). */
+typedef struct {
+  int n_v;           /* real nodes; forces on nodes >= n_v are dropped (active_strain.c:2254) */
+  int n_nodes;       /* real + ghost nodes (length of every coordinate array) */
+  int n_elmt;        /* elements evaluated */
+  int n_elmt_base;   /* > 0: elements >= n_elmt_base are LV apex-cap CST elements */
+  int patch_stride;  /* row width of `patch`: 16 in the reference layout, at most 18 */
+  const int *patch;  /* [patch_stride * n_elmt], BF_PATCH_SENTINEL for empty slots */
+  const int *ire;    /* [n_elmt] 0 regular, 1 irregular */
+  const int *val;    /* [n_elmt] valence (6 for regular elements) */
+  const int *nv1, *nv2, *nv3;       /* [n_elmt] corners; needed only when n_elmt_base > 0 */
+  const double *x0, *y0, *z0;       /* reference coordinates [n_nodes] (x_bp0 ...) */
+  const double *dA0;                /* [n_elmt] reference flat areas (io.c:1646-1669) */
+  const double *n_fib;              /* [3 * n_elmt] fibre directions, Cmpnts layout */
+  const double *gamma_scale;        /* [n_elmt] or NULL (= 1) */
+  const double *activation_delay;   /* [n_elmt] or NULL (= 0) */
+  int n_ghost_rules;                /* ghosts recomputed by bf_ghost_update */
+  const int *ghost_rules;           /* [3 * n_ghost_rules]: ghost n_v+g = x[a] + x[b] - x[c] */
+} bf_mesh_desc;
+
+/* Options and globals of the path (src/active_strain.c:90-110, 174-230, 1156-1179). */
+typedef struct {
+  int n_qp;          /* -num_gaussian_quad_points, 1..5 */
+  int c33_iters;     /* -C33_subitr_nums, >= 1 */
+  int cart_fib_act;  /* -cart_fib_act: Fa = I - gamma f (x) f */
+  int manufactured;  /* manufactured-solution activation (active_strain.c:1240-1250) */
+  double h0, mu, K;  /* thickness, shear modulus (-mu), bulk modulus (-bulk_modulus) */
+  double gamma;      /* -muscle_act_gamma */
+  int ramp_type;     /* -gamma_ramp_type 0..3; 4 = Gaussian pulse (no reference counterpart) */
+  double gamma_T, tau_up, tau_down, apd; /* -gamma_T -gamma_tau_up -gamma_tau_down -gamma_apd */
+  double pulse_t0, pulse_sigma;          /* ramp type 4: gamma exp(-((t-t0)/sigma)^2 / 2) */
+  double manufactured_gamma0, manufactured_T;
+  int ti, tisteps;   /* time-step index and count (globals) */
+  double dt;
+} bf_params;
+
+/* ---- context ---- */
+int bf_ctx_create(int device, bf_ctx **out);
+int bf_ctx_destroy(bf_ctx *ctx);
+
+/* Upload topology, reference coordinates and per-element data; builds the geom0 cache
+ * (replaces ElemUpdateGeom0Subdiv over all cells in FEM_DMPlexGeomSetup,
+ * src/dmplex_geom.c:859-865).  Validates every patch slot (the reference's
+ * per-residual stencil check, active_strain.c:498/:577) once, here. */
+int bf_mesh_upload(bf_ctx *ctx, const bf_mesh_desc *mesh);
+
+/* Options/globals (GetUserActParams, SetGaussianQuadrature; globals h0, ti, dt). */
+int bf_params_set(bf_ctx *ctx, const bf_params *p);
+
+/* Replaces FInternalAct (active_strain.c:2226-2267) on one rank: current coordinates in,
+ * interleaved forces out.  fint has 3 * n_nodes entries; accumulate = 0 writes
+ * Fint = sum of element contributions (the caller's VecSet(Fint, 0) + ADD_VALUES), with
+ * ghost rows zero; accumulate = 1 adds to fint and leaves ghost rows untouched.  Node
+ * sums run in ascending element order, as VecSetValues(ADD_VALUES) does on one rank.
+ * Host pointers; the call is synchronous. */
+int bf_force_eval(bf_ctx *ctx, const double *x, const double *y, const double *z,
+                  double *fint, int accumulate, int precision);
+
+/* Same on device-resident arrays, asynchronous on `stream` (a hipStream_t, NULL = the
+ * context's stream).  Status is collected by bf_last_error(). */
+int bf_force_eval_device(bf_ctx *ctx, const double *d_x, const double *d_y, const double *d_z,
+                         double *d_fint, int accumulate, int precision, void *stream);
+
+/* GlobalGhost (src/bending.c:1763-1814) as an O(n_ghosts) gather:
+ * x[n_v + g] = x[a] + x[b] - x[c] for every ghost rule.  Host and device forms. */
+int bf_ghost_update(bf_ctx *ctx, double *x, double *y, double *z);
+int bf_ghost_update_device(bf_ctx *ctx, double *d_x, double *d_y, double *d_z, void *stream);
+
+/* C33 state of the last evaluation, [n_elmt * n_qp] (ead->C[qp].Cov[2][2]). */
+int bf_c33_download(bf_ctx *ctx, double *c33);
+
+/* UpdateElementThickness (active_strain.c:2198-2224): thickness[ec] = h0 sqrt(mean C33). */
+int bf_update_thickness(bf_ctx *ctx, double *thickness);
+
+/* First failure of the last evaluation: element index (-1 if none), BF_ELEM_* code,
+ * number of elements that raised the non-fatal CC3333 condition.  Synchronises. */
+int bf_last_error(bf_ctx *ctx, int *elem, int *code, int *n_warn);
+
+/* Wall-clock duration of the last bf_force_eval* kernels on the device, in ms
+ * (HIP events on the context stream): element kernel and node-gather kernel. */
+int bf_last_timing(bf_ctx *ctx, float *ms_element, float *ms_gather);
+
+/* Kernel timing over many evaluations: bf_timing_enable(ctx, 1) resets the totals and
+ * records HIP events around the element and gather kernels of every later evaluation;
+ * bf_timing_read() synchronises and returns the count and total milliseconds. */
+int bf_timing_enable(bf_ctx *ctx, int on);
+int bf_timing_read(bf_ctx *ctx, long long *count, double *ms_element, double *ms_gather);
+
+/* Device pointers owned by the context, for callers that keep state resident:
+ * coordinates and force buffers of n_nodes (3 * n_nodes for forces). */
+int bf_device_buffers(bf_ctx *ctx, double **d_x, double **d_y, double **d_z, double **d_fint);
+int bf_sync(bf_ctx *ctx);
+
+/* ---- host topology (inputs of the path; no device needed) ---- */
+/* AreaNormal (src/io.c:1646-1669): dA[e] = 0.5 |(x2-x1) x (x3-x1)|. */
+int bf_topo_area0(int n_tri, const int *nv1, const int *nv2, const int *nv3, const double *x,
+                  const double *y, const double *z, double *dA);
+/* IrrVer (src/bending.c:1188-1267). */
+int bf_topo_irrver(int n_v, int n_elmt, int n_elmt_base, const int *nv1, const int *nv2,
+                   const int *nv3, int n_bnodes, const int *bnodes, int *ire, int *irv,
+                   int *val);
+/* Patch (src/bending.c:1270-1667) over n_tri triangles (real first, then ghost). */
+int bf_topo_patch(int n_v, int n_nodes, int n_elmt, int n_elmt_base, int n_tri, const int *nv1,
+                  const int *nv2, const int *nv3, const int *ire, const int *irv,
+                  const int *val, int stride, int *patch);
+/* GlobalGhostInit (src/bending.c:1670-1760): appends 2 ghost triangles per boundary edge
+ * and writes the ghost rules (b, a, c). */
+int bf_topo_ghost_ring(int n_v, int n_elmt, int n_elmt_base, int n_edge, const int *n_bnodes,
+                       const int *bnodes, int *nv1, int *nv2, int *nv3, int *rules);
+/* IN* rows of a valence-v irregular patch (active_strain.c:594-666), out[5][v+6]. */
+int bf_loop_tables(int v, double *out);
+
+const char *bf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -32,6 +32,7 @@ BF_ELEM_CC3333 = 6
 BF_ELEM_NONFINITE = 7
 BF_PREC_FP64 = 0
 BF_PREC_MIXED = 1
+BF_PREC_FP64_FAST = 2
 SENTINEL = 1000000
 MAX_NEN = 18
 

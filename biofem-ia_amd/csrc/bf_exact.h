@@ -1,0 +1,308 @@
+// bf_exact.h -- per-element active-strain force in the reference's operation order.
+//
+// Every sum, product and inverse below is evaluated in the order written in
+// src/active_strain.c (ElemUpdateG :896-982, ElemActDefGrad :1196-1329, ElemCGDefTens
+// :1332-1378, ElemC33Solve :2048-2138 with its five sub-steps :1381-1721, ElemUpdFint
+// :1762-1995) and src/math.c.  Compiled without floating-point contraction, it rounds
+// exactly as the reference's C does, so this path reproduces the CPU result bit for bit
+// (up to the libm calls of the activation ramp).  It is the plain FP64 path; the staged
+// arithmetic of bf_elem.h is the fast path and states its error against this one.
+//
+// Differences from the reference that do not change any rounded value:
+//  * the qp loop is outermost (the reference loops qp inside each C33 iteration); qps
+//    do not interact except through the CC3333 guard, handled by `skip_mask` as below;
+//  * CCe entries are recomputed where the 8-deep CC^{2222} sum reads them instead of
+//    being stored (same expression, same rounding);
+//  * quantities no force depends on (C.Cont, C_inv.Cov, Fa_inv.Cont, g^i) are skipped.
+#pragma once
+
+#include "bf_elem.h"
+
+namespace bf {
+namespace exact {
+
+struct Mat {
+  double m[3][3];
+};
+
+BF_HD void matmult(const Mat& A, const Mat& B, Mat& C) {  // src/math.c:95-120
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double s = 0.0;
+      for (int k = 0; k < 3; k++) s += A.m[i][k] * B.m[k][j];
+      C.m[i][j] = s;
+    }
+}
+BF_HD void inv(const Mat& T, Mat& Ti) {  // src/math.c:77-91
+  const double T11 = T.m[0][0], T12 = T.m[0][1], T13 = T.m[0][2];
+  const double T21 = T.m[1][0], T22 = T.m[1][1], T23 = T.m[1][2];
+  const double T31 = T.m[2][0], T32 = T.m[2][1], T33 = T.m[2][2];
+  const double det = T11 * (T33 * T22 - T32 * T23) - T21 * (T33 * T12 - T32 * T13) +
+                     T31 * (T23 * T12 - T22 * T13);
+  const double c = 1 / det;
+  Ti.m[0][0] = c * (T33 * T22 - T32 * T23);
+  Ti.m[0][1] = -c * (T33 * T12 - T32 * T13);
+  Ti.m[0][2] = c * (T23 * T12 - T22 * T13);
+  Ti.m[1][0] = -c * (T33 * T21 - T31 * T23);
+  Ti.m[1][1] = c * (T33 * T11 - T31 * T13);
+  Ti.m[1][2] = -c * (T23 * T11 - T21 * T13);
+  Ti.m[2][0] = c * (T32 * T21 - T31 * T22);
+  Ti.m[2][1] = -c * (T32 * T11 - T31 * T12);
+  Ti.m[2][2] = c * (T22 * T11 - T21 * T12);
+}
+BF_HD double det3(const Mat& A) {  // src/math.c:150-156
+  return A.m[0][0] * (A.m[1][1] * A.m[2][2] - A.m[1][2] * A.m[2][1]) -
+         A.m[0][1] * (A.m[1][0] * A.m[2][2] - A.m[1][2] * A.m[2][0]) +
+         A.m[0][2] * (A.m[1][0] * A.m[2][1] - A.m[1][1] * A.m[2][0]);
+}
+BF_HD void raise2(const Mat& gInv, const Mat& A, Mat& out) {  // src/math.c:159-172
+  Mat t;
+  matmult(gInv, A, t);
+  matmult(t, gInv, out);
+}
+BF_HD void lower2(const Mat& gCov, const Mat& A, Mat& out) {  // src/math.c:174-186
+  Mat t;
+  matmult(gCov, A, t);
+  matmult(t, gCov, out);
+}
+// ComputeMetricTensor (active_strain.c:769-779) from basis rows
+BF_HD void metric(const V3<double> g[3], Mat& gm) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) gm.m[i][j] = dot(g[i], g[j]);
+}
+
+// Element-level inputs of the exact pipeline.
+struct ElemIn {
+  MidGeom cur;                 // current midsurface (geom[0])
+  V3<double> a31c, a32c;       // Compute_a3_alpha_ of the current midsurface
+  RefGeom ref;                 // a1, a2, a3, a3_1, a3_2 of geom0
+  double Fa_cart[3][3];        // active deformation in the Cartesian frame (:1227-1256)
+  double h0, mu, K;
+  int n_qp, iters;
+  const double* theta;
+  const double* wq;
+};
+
+// Stress S^{00}, S^{11}, S^{01} per qp after the C33 loop, final C33 per qp, status bits.
+struct ElemOut {
+  double Sv[kMaxQp][3];
+  double c33[kMaxQp];
+  int status;
+};
+
+BF_HD void integrate(const ElemIn& in, ElemOut& out) {
+  out.status = 0;
+  const double hhalf = 0.5 * in.h0;  // :913
+  Mat Fc, Finv_c;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) Fc.m[i][j] = in.Fa_cart[i][j];
+  inv(Fc, Finv_c);  // :1291 (same inverse for every qp)
+  unsigned skip_mask = 0u;
+  for (int qp = 0; qp < in.n_qp; qp++) {
+    const double theta3 = hhalf * in.theta[qp];  // :936
+    // ---- ElemUpdateG :940-978 ----
+    V3<double> g[3], g0[3];
+    g[0] = add(in.cur.a1, scale(theta3, in.a31c));
+    g[1] = add(in.cur.a2, scale(theta3, in.a32c));
+    g[2] = in.cur.a3;
+    Mat gm, gmI;
+    metric(g, gm);
+    inv(gm, gmI);
+    g0[0] = add(in.ref.a1, scale(theta3, in.ref.a31));
+    g0[1] = add(in.ref.a2, scale(theta3, in.ref.a32));
+    g0[2] = in.ref.a3;
+    Mat gm0, G;
+    metric(g0, gm0);
+    inv(gm0, G);
+    // ---- ElemActDefGrad :1272-1320 ----
+    Mat S, ST, tmp, FaCov, FiCov, FaCont;
+    for (int j = 0; j < 3; j++) {
+      S.m[0][j] = g0[j].x;
+      S.m[1][j] = g0[j].y;
+      S.m[2][j] = g0[j].z;
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) ST.m[i][j] = S.m[j][i];
+    matmult(ST, Fc, tmp);
+    matmult(tmp, S, FaCov);
+    matmult(ST, Finv_c, tmp);
+    matmult(tmp, S, FiCov);
+    raise2(G, FaCov, FaCont);
+    // ---- ElemCGDefTens :1341-1363 ----
+    Mat CCov, Ci;
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        CCov.m[i][j] = gm.m[i][j];
+        Ci.m[i][j] = gmI.m[i][j];
+      }
+    for (int a = 0; a < 2; a++) {
+      CCov.m[a][2] = 0.0;
+      CCov.m[2][a] = 0.0;
+      Ci.m[a][2] = 0.0;
+      Ci.m[2][a] = 0.0;
+    }
+    CCov.m[2][2] = 1.0;
+    Ci.m[2][2] = 1.0;
+    double Sc[3][3];
+    for (int it = 0; it < in.iters; it++) {
+      // ---- ElemElasCGDefTens :1389-1430 ----
+      Mat Ch, CeiCov, CeCont, CeCov;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double acc = 0.0;
+          for (int mm = 0; mm < 3; mm++)
+            for (int pp = 0; pp < 3; pp++)
+              for (int qq = 0; qq < 3; qq++)
+                for (int l = 0; l < 3; l++)
+                  acc += FaCont.m[i][mm] * gm0.m[mm][pp] * Ci.m[pp][qq] * FaCont.m[l][j] *
+                         gm0.m[l][qq];
+          Ch.m[i][j] = acc;
+        }
+      lower2(gm0, Ch, CeiCov);
+      inv(CeiCov, CeCont);
+      lower2(gm0, CeCont, CeCov);
+      // ---- ElemElasStress :1458-1487 ----
+      const double detCe = det3(CeCov);
+      const double detG0 = det3(gm0);
+      const double Je = sqrt(detCe / detG0);
+      double Se[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          const double term1 = in.mu * (G.m[i][j] - Ch.m[i][j]);
+          const double term2 = in.K * Ch.m[i][j] * (Je * Je - Je);
+          Se[i][j] = term1 + term2;
+        }
+      // ---- ElemTotStress :1506-1528 ----
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double acc = 0.0;
+          for (int w = 0; w < 3; w++)
+            for (int pp = 0; pp < 3; pp++)
+              for (int z = 0; z < 3; z++)
+                for (int s = 0; s < 3; s++)
+                  acc += 0.5 * FiCov.m[w][pp] * FiCov.m[z][s] * G.m[s][j] *
+                         (Se[pp][z] * G.m[w][i] + Se[pp][i] * G.m[w][z]);
+          Sc[i][j] = acc;
+        }
+      // ---- ElemElsTangMatTens :1548-1585 (alpha, beta; entries formed on demand) ----
+      const double Je2 = Je * Je;
+      const double alpha = in.K * (Je2 - Je) - in.mu;
+      const double beta = 0.5 * in.K * Je * (2.0 * Je - 1.0);
+      auto CCe = [&](int i, int j, int k, int l) {
+        return (2.0 * beta) * Ch.m[i][j] * Ch.m[k][l] -
+               alpha * (Ch.m[i][k] * Ch.m[l][j] + Ch.m[i][l] * Ch.m[k][j]);
+      };
+      // ---- ElemTotTangMatTens :1603-1643, [2][2][2][2] only ----
+      double CC = 0.0;
+      {
+        const int i = 2, j = 2, k = 2, l = 2;
+        for (int gg = 0; gg < 3; gg++)
+          for (int h = 0; h < 3; h++)
+            for (int mm = 0; mm < 3; mm++)
+              for (int n = 0; n < 3; n++)
+                for (int pp = 0; pp < 3; pp++)
+                  for (int s = 0; s < 3; s++)
+                    for (int w = 0; w < 3; w++)
+                      for (int z = 0; z < 3; z++)
+                        CC += 0.25 * FiCov.m[w][pp] * FiCov.m[z][s] * FiCov.m[gg][h] *
+                              FiCov.m[n][mm] * G.m[s][j] * G.m[mm][l] *
+                              (G.m[gg][k] * (CCe(h, n, pp, z) * G.m[w][i] +
+                                             CCe(h, n, pp, i) * G.m[w][z]) +
+                               G.m[gg][n] * (CCe(h, k, pp, z) * G.m[w][i] +
+                                             CCe(h, k, pp, i) * G.m[w][z]));
+      }
+      // ---- ModElemC33 :1669-1707 ----
+      if (!(fabs(CC) > 1e-14)) {
+        skip_mask |= 1u << it;
+        out.status |= kStatCC3333;
+      } else if (!((skip_mask >> it) & 1u)) {
+        double DeltaC33 = -1.0 * Sc[2][2] / CC;
+        const double C33_old = CCov.m[2][2];
+        CCov.m[2][2] = C33_old + DeltaC33;
+        if (det3(CCov) <= 0.0) {
+          double lo = 0.0, hi = 1.0;
+          for (int bk = 0; bk < 16; bk++) {
+            const double mid = 0.5 * (lo + hi);
+            CCov.m[2][2] = C33_old + mid * DeltaC33;
+            if (det3(CCov) > 0.0) lo = mid; else hi = mid;
+          }
+          DeltaC33 *= lo;
+          CCov.m[2][2] = C33_old + DeltaC33;
+        }
+      }
+      // ---- C_inv.Cont = INV(C.Cov) :2101 ----
+      inv(CCov, Ci);
+    }
+    out.Sv[qp][0] = Sc[0][0];
+    out.Sv[qp][1] = Sc[1][1];
+    out.Sv[qp][2] = Sc[0][1];
+    out.c33[qp] = CCov.m[2][2];
+  }
+}
+
+// ElemUpdFint for one patch node a, regular/irregular branch (:1879-1913 / :1934-1983):
+// accumulates over qp in the reference order.  Na0..Nb2 are the node's table entries.
+BF_HD V3<double> node_force_exact(double Na0, double Na1, double Nb0, double Nb1, double Nb2,
+                                  const MidGeom& G, double pref0, double half_h0,
+                                  const ElemOut& o, const double* theta, const double* wq,
+                                  int n_qp, bool irregular) {
+  double F[3] = {0.0, 0.0, 0.0};
+  const double nn[3] = {G.a3.x, G.a3.y, G.a3.z};
+  const double a1[3] = {G.a1.x, G.a1.y, G.a1.z};
+  const double a2[3] = {G.a2.x, G.a2.y, G.a2.z};
+  const V3<double> ng = add(scale(Na0, G.gc1), scale(Na1, G.gc2));
+  const double b1 = dot(ng, G.Aaa), b2 = dot(ng, G.Abb), b3 = dot(ng, G.Aab);
+  for (int qp = 0; qp < n_qp; qp++) {
+    const double w_qp = wq[qp], xi_qp = theta[qp];
+    const double* Sv = o.Sv[qp];
+    for (int c = 0; c < 3; c++) {
+      const double Bm0 = Na0 * a1[c];
+      const double Bm1 = Na1 * a2[c];
+      const double Bm2 = Na0 * a2[c] + Na1 * a1[c];
+      double mem, bend;
+      if (!irregular) {  // :1879-1913, loop form starting from 0.0
+        const double Bs0 = -(Nb0 + b1) * nn[c];
+        const double Bs1 = -(Nb1 + b2) * nn[c];
+        const double Bs2 = -2.0 * (Nb2 + b3) * nn[c];
+        mem = 0.0;
+        bend = 0.0;
+        mem += Bm0 * Sv[0];
+        bend += Bs0 * Sv[0];
+        mem += Bm1 * Sv[1];
+        bend += Bs1 * Sv[1];
+        mem += Bm2 * Sv[2];
+        bend += Bs2 * Sv[2];
+      } else {  // :1959-1983
+        const double Bs0 = -(Nb0 + b1);
+        const double Bs1 = -(Nb1 + b2);
+        const double Bs2 = -2.0 * (Nb2 + b3);
+        mem = Bm0 * Sv[0] + Bm1 * Sv[1] + Bm2 * Sv[2];
+        bend = (Bs0 * nn[c]) * Sv[0] + (Bs1 * nn[c]) * Sv[1] + (Bs2 * nn[c]) * Sv[2];
+      }
+      F[c] += pref0 * w_qp * (mem + bend * (half_h0 * xi_qp));
+    }
+  }
+  return mk(F[0], F[1], F[2]);
+}
+
+// LV apex-cap CST membrane force (:1825-1850) for corner a with shape gradient (Na0, Na1).
+BF_HD V3<double> cap_force_exact(double Na0, double Na1, const MidGeom& G, double pref0,
+                                 const ElemOut& o, const double* wq, int n_qp) {
+  double F[3] = {0.0, 0.0, 0.0};
+  const double a1[3] = {G.a1.x, G.a1.y, G.a1.z};
+  const double a2[3] = {G.a2.x, G.a2.y, G.a2.z};
+  for (int qp = 0; qp < n_qp; qp++) {
+    const double w_qp = wq[qp];
+    const double* Sv = o.Sv[qp];
+    for (int c = 0; c < 3; c++) {
+      const double Bm0 = Na0 * a1[c];
+      const double Bm1 = Na1 * a2[c];
+      const double Bm2 = Na0 * a2[c] + Na1 * a1[c];
+      F[c] += pref0 * w_qp * (Bm0 * Sv[0] + Bm1 * Sv[1] + Bm2 * Sv[2]);
+    }
+  }
+  return mk(F[0], F[1], F[2]);
+}
+
+}  // namespace exact
+}  // namespace bf

@@ -22,121 +22,11 @@
 
 #include "../../include/biofem_hip.h"
 #include "bf_elem.h"
+#include "bf_kernels.h"
 
 using namespace bf;
 
-// ------------------------------------------------------------------ constant tables ----
-// Loop box-spline derivatives at the element centre, verbatim 4-decimal values of
-// src/active_strain.c:69-88.
-__constant__ double c_Na[2][12] = {
-    {-0.0247, -0.0309, 0.0000, -0.4815, -0.1852, 0.0247, 0.4815, 0.0000, -0.0062, 0.0309,
-     0.1852, 0.0062},
-    {-0.0309, -0.0247, -0.1852, -0.4815, 0.0000, -0.0062, 0.0000, 0.4815, 0.0247, 0.0062,
-     0.1852, 0.0309}};
-__constant__ double c_Nab[3][12] = {
-    {0.1111, 0.2222, -0.2222, -0.2222, 0.4444, 0.1111, -0.2222, -0.8889, 0.0000, 0.2222,
-     0.4444, 0.0000},
-    {0.2222, 0.1111, 0.4444, -0.2222, -0.2222, 0.0000, -0.8889, -0.2222, 0.1111, 0.0000,
-     0.4444, 0.2222},
-    {0.1667, 0.1667, -0.1111, 0.2222, -0.1111, -0.0556, -0.4444, -0.4444, -0.0556, 0.0556,
-     0.5556, 0.0556}};
-// IN* rows per valence (filled from bf_loop_tables at context creation): [v][row][slot]
-__constant__ double c_IN[13][5][kMaxNen];
-
-// ------------------------------------------------------------------ kernel params ------
-struct RampParams {
-  double gamma, gamma_T, tau_up, tau_down, apd, pulse_t0, pulse_sigma, t, T_default;
-  int type;
-};
-
-struct KParams {
-  int E, NS, n_qp, iters;
-  int act_mode;    // 0: I - gamma f f (cart_fib_act), 1: manufactured diag, 2: isotropic diag
-  int has_delay;   // per-element activation delay present
-  double theta[kMaxQp], w[kMaxQp];
-  double h0, mu, K;
-  double gamma_t;  // GammaOfTimeDelayed(p, 0) evaluated on the host
-  double man_gamma;
-  RampParams ramp;
-};
-
-struct DevMesh {
-  const int* patch;    // [NS][E]
-  const int* meta;     // kind | val << 8 | nen << 16
-  const int* corner;   // [6][E] cap: nv1..3 then their patch slots (NULL if no cap)
-  const double* ref;   // [15][E] a1 a2 a3 a31 a32 of geom0
-  const double* dA0;
-  const double* fib;   // [3][E]
-  const double* gscale;
-  const double* delay;
-  double* Fe;          // [3*NS][E]
-  double* c33;         // [n_qp][E]
-  int* status;         // [4] first error elem, code, first warn elem, warn count
-};
-
-// GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
-__device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) {
-  if (p.type == 0) return p.gamma;
-  const double t = p.t - delay;
-  if (p.type == 3) {
-    const double tau_up = (p.tau_up > 0.0) ? p.tau_up : 0.02;
-    const double tau_down = (p.tau_down > 0.0) ? p.tau_down : 0.08;
-    const double apd = (p.apd > 0.0) ? p.apd : 0.30;
-    const double t_up = 4.0 * tau_up;
-    const double t_down = t_up + apd;
-    const double phi_up = 1.0 / (1.0 + exp(-(t - t_up) / tau_up));
-    const double phi_down = 1.0 / (1.0 + exp((t - t_down) / tau_down));
-    return p.gamma * phi_up * phi_down;
-  }
-  if (p.type == 4) {
-    const double s = (t - p.pulse_t0) / p.pulse_sigma;
-    return p.gamma * exp(-0.5 * s * s);
-  }
-  const double T = (p.gamma_T > 0.0) ? p.gamma_T : p.T_default;
-  if (t <= 0.0) return 0.0;
-  const double s = (p.type == 1) ? sin(M_PI * t / T) : sin(M_PI * 0.5 * t / T);
-  return p.gamma * s * s;
-}
-
 // ------------------------------------------------------------------ kernels ------------
-__device__ inline V3<double> ld3(const double* __restrict__ x, const double* __restrict__ y,
-                                 const double* __restrict__ z, int n) {
-  return mk(x[n], y[n], z[n]);
-}
-
-// Reference midsurface cache (ElemUpdateGeom0Subdiv over all cells, dmplex_geom.c:859-865),
-// stored as a1, a2, a3, a3_1, a3_2 -- everything ElemUpdateG reads from geom0 (:925-931).
-__global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E, int NS,
-                                                       const double* __restrict__ x,
-                                                       const double* __restrict__ y,
-                                                       const double* __restrict__ z,
-                                                       double* __restrict__ ref) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  const int meta = m.meta[e];
-  const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
-  MidGeom G;
-  auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
-  if (kind == kRegular) {
-    G = mid_geom(12, c_Na[0], c_Na[1], c_Nab[0], c_Nab[1], c_Nab[2], xyz);
-  } else if (kind == kIrregular) {
-    const double(*t)[kMaxNen] = c_IN[val];
-    G = mid_geom(nen, t[0], t[1], t[2], t[3], t[4], xyz);
-  } else {
-    G = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),
-                 ld3(x, y, z, m.corner[(size_t)2 * E + e]));
-  }
-  V3<double> a31, a32;
-  a3_alpha(G.a1, G.a2, G.Aaa, G.Aab, G.Abb, a31, a32);
-  const V3<double> v[5] = {G.a1, G.a2, G.a3, a31, a32};
-#pragma unroll
-  for (int k = 0; k < 5; k++) {
-    ref[(size_t)(3 * k + 0) * E + e] = v[k].x;
-    ref[(size_t)(3 * k + 1) * E + e] = v[k].y;
-    ref[(size_t)(3 * k + 2) * E + e] = v[k].z;
-  }
-}
-
 template <class R>
 __global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
                                                          const double* __restrict__ x,
@@ -384,6 +274,7 @@ int upload_tables(int device) {
       for (int j = 0; j < nen; j++) tab[v][r][j] = buf[(size_t)r * nen + j];
   }
   BF_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_IN), tab, sizeof(tab)));
+  BF_HIP(upload_in_tables_exact(&tab[0][0][0], sizeof(tab)));
   g_tables_device = device;
   return BF_OK;
 }
@@ -623,9 +514,7 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
     dm.patch = c->d_patch;
     dm.meta = c->d_meta;
     dm.corner = c->d_corner;
-    hipLaunchKernelGGL(bf_geom0_kernel, dim3((E + 255) / 256), dim3(256), 0, s, dm, E, NS,
-                       c->d_x, c->d_y, c->d_z, c->d_ref);
-    if ((rc = kernel_launch_check()) != BF_OK) return rc;
+    BF_HIP(launch_geom0(dm, E, c->d_x, c->d_y, c->d_z, c->d_ref, s));
   }
   BF_HIP(hipStreamSynchronize(s));
   c->has_mesh = true;
@@ -685,7 +574,8 @@ int bf_params_set(bf_ctx* c, const bf_params* p) {
 static int launch_eval(bf_ctx* c, const double* x, const double* y, const double* z,
                        double* fint, int accumulate, int precision, hipStream_t s) {
   if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
-  if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED) return BF_ERR_ARG;
+  if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
+    return BF_ERR_ARG;
   KParams kp = c->kp;
   kp.E = c->E;
   kp.NS = c->NS;
@@ -721,6 +611,8 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   if (c->E > 0) {
     const dim3 grid((c->E + 255) / 256), block(256);
     if (precision == BF_PREC_FP64)
+      BF_HIP(launch_element_exact(dm, kp, x, y, z, s));
+    else if (precision == BF_PREC_FP64_FAST)
       hipLaunchKernelGGL(bf_element_kernel<double>, grid, block, 0, s, dm, kp, x, y, z);
     else
       hipLaunchKernelGGL(bf_element_kernel<float>, grid, block, 0, s, dm, kp, x, y, z);

@@ -1,0 +1,96 @@
+// bf_kernels.h -- device-side structures shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bf_elem.h"
+
+namespace bf {
+
+struct RampParams {
+  double gamma, gamma_T, tau_up, tau_down, apd, pulse_t0, pulse_sigma, t, T_default;
+  int type;
+};
+
+struct KParams {
+  int E, NS, n_qp, iters;
+  int act_mode;    // 0: I - gamma f f (cart_fib_act), 1: manufactured diag, 2: isotropic diag
+  int has_delay;   // per-element activation delay present
+  double theta[kMaxQp], w[kMaxQp];
+  double h0, mu, K;
+  double gamma_t;  // GammaOfTimeDelayed(p, 0) evaluated on the host
+  double man_gamma;
+  RampParams ramp;
+};
+
+struct DevMesh {
+  const int* patch;    // [NS][E]
+  const int* meta;     // kind | val << 8 | nen << 16
+  const int* corner;   // [6][E] cap: nv1..3 then their patch slots (NULL if no cap)
+  const double* ref;   // [15][E] a1 a2 a3 a31 a32 of geom0
+  const double* dA0;
+  const double* fib;   // [3][E]
+  const double* gscale;
+  const double* delay;
+  double* Fe;          // [3*NS][E]
+  double* c33;         // [n_qp][E]
+  int* status;         // [4] first error elem, code, first warn elem, warn count
+};
+
+// GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
+__device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) {
+  if (p.type == 0) return p.gamma;
+  const double t = p.t - delay;
+  if (p.type == 3) {
+    const double tau_up = (p.tau_up > 0.0) ? p.tau_up : 0.02;
+    const double tau_down = (p.tau_down > 0.0) ? p.tau_down : 0.08;
+    const double apd = (p.apd > 0.0) ? p.apd : 0.30;
+    const double t_up = 4.0 * tau_up;
+    const double t_down = t_up + apd;
+    const double phi_up = 1.0 / (1.0 + exp(-(t - t_up) / tau_up));
+    const double phi_down = 1.0 / (1.0 + exp((t - t_down) / tau_down));
+    return p.gamma * phi_up * phi_down;
+  }
+  if (p.type == 4) {
+    const double s = (t - p.pulse_t0) / p.pulse_sigma;
+    return p.gamma * exp(-0.5 * s * s);
+  }
+  const double T = (p.gamma_T > 0.0) ? p.gamma_T : p.T_default;
+  if (t <= 0.0) return 0.0;
+  const double s = (p.type == 1) ? sin(M_PI * t / T) : sin(M_PI * 0.5 * t / T);
+  return p.gamma * s * s;
+}
+
+__device__ inline V3<double> ld3(const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, int n) {
+  return mk(x[n], y[n], z[n]);
+}
+
+
+// Loop box-spline derivatives at the element centre, verbatim 4-decimal values of
+// src/active_strain.c:69-88.
+static __constant__ double c_Na[2][12] = {
+    {-0.0247, -0.0309, 0.0000, -0.4815, -0.1852, 0.0247, 0.4815, 0.0000, -0.0062, 0.0309,
+     0.1852, 0.0062},
+    {-0.0309, -0.0247, -0.1852, -0.4815, 0.0000, -0.0062, 0.0000, 0.4815, 0.0247, 0.0062,
+     0.1852, 0.0309}};
+static __constant__ double c_Nab[3][12] = {
+    {0.1111, 0.2222, -0.2222, -0.2222, 0.4444, 0.1111, -0.2222, -0.8889, 0.0000, 0.2222,
+     0.4444, 0.0000},
+    {0.2222, 0.1111, 0.4444, -0.2222, -0.2222, 0.0000, -0.8889, -0.2222, 0.1111, 0.0000,
+     0.4444, 0.2222},
+    {0.1667, 0.1667, -0.1111, 0.2222, -0.1111, -0.0556, -0.4444, -0.4444, -0.0556, 0.0556,
+     0.5556, 0.0556}};
+// IN* rows per valence (filled from bf_loop_tables at context creation): [v][row][slot].
+// Each kernel translation unit holds its own copy (no relocatable device code).
+static __constant__ double c_IN[13][5][kMaxNen];
+
+
+// Launchers of the kernels compiled without floating-point contraction (bf_exact.hip).
+hipError_t upload_in_tables_exact(const double* tab, size_t bytes);
+hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* y,
+                        const double* z, double* ref, hipStream_t s);
+hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
+                                const double* y, const double* z, hipStream_t s);
+
+}  // namespace bf

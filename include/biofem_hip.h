@@ -41,9 +41,13 @@ extern "C" {
 #define BF_ELEM_NONFINITE 7 /* NaN/Inf in an element force */
 
 /* precision modes of the force evaluation */
-#define BF_PREC_FP64 0  /* plain FP64 throughout */
-#define BF_PREC_MIXED 1 /* FP32 per-quadrature-point geometry/constitutive arithmetic,
-                           FP64 coordinates, midsurface sums and nodal accumulation */
+#define BF_PREC_FP64 0      /* plain FP64 in the reference's operation order (no contraction):
+                               reproduces the CPU FInternalAct bit for bit */
+#define BF_PREC_MIXED 1     /* FP32 per-quadrature-point geometry/constitutive arithmetic on the
+                               staged contractions; FP64 coordinates, midsurface sums and nodal
+                               accumulation */
+#define BF_PREC_FP64_FAST 2 /* FP64 on the staged contractions (closed-form CC^{2222}, 3x3
+                               products instead of the 6- and 8-deep index loops) */
 
 #define BF_PATCH_SENTINEL 1000000 /* empty patch slot (src/io.c:306) */
 #define BF_MAX_NEN 18             /* widest patch (valence 12) */

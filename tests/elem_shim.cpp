@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../biofem-ia_amd/csrc/bf_elem.h"
+#include "../biofem-ia_amd/csrc/bf_exact.h"
 
 using namespace bf;
 
@@ -62,6 +63,35 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
     act.f = mk(fib[3 * e], fib[3 * e + 1], fib[3 * e + 2]);
     double N0, N1, N2, M0, M1, M2;
     int st;
+    if (precision == 2) {  // reference operation order (bf_exact.h)
+      exact::ElemIn in;
+      in.cur = Gc;
+      in.a31c = a31c;
+      in.a32c = a32c;
+      in.ref = R0;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          in.Fa_cart[i][j] = 0.0;
+      const double nf[3] = {fib[3 * e], fib[3 * e + 1], fib[3 * e + 2]};
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) in.Fa_cart[i][j] = (i == j ? 1.0 : 0.0) - gam_e[e] * nf[i] * nf[j];
+      in.h0 = h0; in.mu = mu; in.K = K; in.n_qp = n_qp; in.iters = iters; in.theta = theta; in.wq = wq;
+      exact::ElemOut o;
+      exact::integrate(in, o);
+      if (o.status) warn++;
+      if (c33_out) for (int q = 0; q < n_qp; q++) c33_out[(size_t)e * n_qp + q] = o.c33[q];
+      const double pref0 = 0.5 * h0 * dA0[e];
+      for (int a = 0; a < nen; a++) {
+        V3<double> f = exact::node_force_exact(na0[a], na1[a], nb0[a], nb1[a], nb2[a], Gc, pref0,
+                                               0.5 * h0, o, theta, wq, n_qp, kind != 0);
+        const int n = pt[a];
+        if (n == kSentinel || n < 0 || n >= n_v) continue;
+        fint[3 * n] += f.x;
+        fint[3 * n + 1] += f.y;
+        fint[3 * n + 2] += f.z;
+      }
+      continue;
+    }
     if (precision == 0) {
       Resultants<double> r = integrate_element<double>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act,
                                                        h0, mu, K, n_qp, iters, theta, wq);

@@ -1,7 +1,10 @@
 """GPU parity of the HIP force path against the CPU oracle (through the C ABI).
 
-Tolerances: plain FP64 path <= 1e-12 relative (2-norm over the force vector) against the
-oracle's FP64 restatement of FInternalAct; FP32-mixed <= 1e-5.
+Tolerances (2-norm of the force-vector difference over the oracle's norm):
+  plain FP64 (reference operation order, BF_PREC_FP64): bitwise equal expected, asserted
+    <= 1e-15 (the two sides round every operation identically);
+  FP64 on the staged contractions (BF_PREC_FP64_FAST): <= 1e-12 on well-conditioned states;
+  FP32-mixed (BF_PREC_MIXED): <= 1e-5 on well-conditioned states.
 """
 import numpy as np
 import pytest
@@ -13,6 +16,7 @@ from oracle import bf_oracle as O
 
 pytestmark = pytest.mark.gpu
 
+TOL_EXACT = 1e-15
 TOL_FP64 = 1e-12
 TOL_MIXED = 1e-5
 
@@ -34,10 +38,14 @@ def test_membrane_small_fp64(small):
     print("case membrane 32x24 fp64", flush=True)
     ref, st, c33 = O.finternal_act(m, p, x, y, z, want_c33=True)
     got = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
-    assert rel(got, ref) <= TOL_FP64
+    print("plain fp64 rel err", rel(got, ref), "bitwise", np.array_equal(got, ref))
+    assert rel(got, ref) <= TOL_EXACT
     assert np.all(got[3 * m.n_v:] == 0.0)
-    np.testing.assert_allclose(fe.c33(), c33, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(fe.c33(), c33, rtol=0, atol=1e-15)
     assert fe.last_error() == (-1, 0, 0)
+    fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+    assert rel(fast, ref) <= TOL_FP64
+    np.testing.assert_allclose(fe.c33(), c33, rtol=0, atol=1e-12)
 
 
 def test_membrane_small_displaced(small):
@@ -45,6 +53,8 @@ def test_membrane_small_displaced(small):
     x, y, z = configs.smooth_displacement(m)
     ref, _, _ = O.finternal_act(m, p, x, y, z)
     got = fe.finternal_act(x, y, z)
+    assert rel(got, ref) <= TOL_EXACT
+    got = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
     assert rel(got, ref) <= TOL_FP64
     gm = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
     assert rel(gm, ref) <= TOL_MIXED
@@ -59,13 +69,14 @@ def test_membrane_small_mixed(small):
 
 def test_deterministic_and_accumulate(small):
     m, p, (x, y, z), fe = small
-    a = fe.finternal_act(x, y, z)
-    b = fe.finternal_act(x, y, z)
-    assert np.array_equal(a, b)
+    for prec in (L.BF_PREC_FP64, L.BF_PREC_FP64_FAST, L.BF_PREC_MIXED):
+        a = fe.finternal_act(x, y, z, precision=prec)
+        b = fe.finternal_act(x, y, z, precision=prec)
+        assert np.array_equal(a, b)
     base = np.arange(3 * m.n_nodes, dtype=np.float64) * 1e-3
     acc = fe.finternal_act(x, y, z, fint=base.copy(), accumulate=True)
     ref, _, _ = O.finternal_act(m, p, x, y, z, fint=base.copy(), accumulate=True)
-    assert rel(acc, ref) <= TOL_FP64
+    assert rel(acc, ref) <= TOL_EXACT
     assert np.array_equal(acc[3 * m.n_v:], base[3 * m.n_v:])
 
 
@@ -77,3 +88,26 @@ def test_ghost_update(small):
     fe.ghost_update(gx, gy, gz)
     for a, b in ((gx, x), (gy, y), (gz, z)):
         assert np.array_equal(a, b)
+
+
+def test_membrane_large_bitwise():
+    """256 x 192 membrane with the N(0, 1e-3) out-of-plane state: a quarter of its elements
+    drive the reference's C33 Newton into its det(C) > 0 bisection clamp, where rounding
+    differences are amplified.  The plain FP64 path must still reproduce the oracle; the
+    staged paths report their error (asserted loosely: they are not the parity path)."""
+    m, p, (x, y, z) = configs.membrane_large()
+    print("case membrane 256x192 plain fp64", flush=True)
+    ref, st, c33 = O.finternal_act(m, p, x, y, z, omp_threads=16, want_c33=True)
+    fe = ActStrainFE(m, p)
+    try:
+        got = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
+        err = rel(got, ref)
+        print("plain fp64 rel err", err, "bitwise", np.array_equal(got, ref))
+        assert err <= TOL_EXACT
+        np.testing.assert_allclose(fe.c33(), c33, rtol=0, atol=1e-15)
+        fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+        mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
+        print("fast fp64 rel err", rel(fast, ref), "mixed rel err", rel(mixed, ref))
+        assert rel(fast, ref) <= 1e-8
+    finally:
+        fe.close()

@@ -260,3 +260,29 @@ class Cached:
         if self.g0:
             self.L.bfo_geom0_free(self.g0)
             self.g0 = None
+
+
+def element_force(m, p, ec, x, y, z):
+    """bfo_element_force of one element: (Fb[3*nen], S[n_qp,3,3], C33[n_qp], n_warn)."""
+    L = lib()
+    L.bfo_element_force.argtypes = [C.POINTER(Mesh), C.POINTER(Params), C.POINTER(Geom),
+                                    C.POINTER(Geom), C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
+    om = OracleMesh(m)
+    g0, gc = Geom(), Geom()
+    x0, x0p = _d(m.x0)
+    y0, y0p = _d(m.y0)
+    z0, z0p = _d(m.z0)
+    L.bfo_geom_subdiv(C.byref(om.s), ec, x0p, y0p, z0p, C.byref(g0))
+    x, xp = _d(x)
+    y, yp = _d(y)
+    z, zp = _d(z)
+    L.bfo_geom_subdiv(C.byref(om.s), ec, xp, yp, zp, C.byref(gc))
+    Fb = np.zeros(3 * 64)
+    S = np.zeros(9 * 5)
+    c33 = np.zeros(5)
+    w = C.c_int(0)
+    L.bfo_element_force(C.byref(om.s), C.byref(params(p)), C.byref(g0), C.byref(gc), ec,
+                        Fb.ctypes.data_as(_dp), S.ctypes.data_as(_dp), c33.ctypes.data_as(_dp),
+                        C.byref(w))
+    nen = 12 if m.ire[ec] == 0 else int(m.val[ec]) + 6
+    return Fb[: 3 * nen], S[: 9 * p.n_qp].reshape(p.n_qp, 3, 3), c33[: p.n_qp], w.value

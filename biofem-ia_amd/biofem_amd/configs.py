@@ -51,3 +51,23 @@ def smooth_displacement(m: Mesh, amp: float = 0.01):
     z[: m.n_v] += amp * np.sin(np.pi * X) * np.sin(np.pi * Y / 0.75)
     m.apply_ghosts(x, y, z)
     return x, y, z
+
+
+def ellipsoid_shell(levels: int = 6, ti: int = 30, seed: int = 3):
+    """Truncated prolate ellipsoid (semi-axes 1, 1, 2.2; z >= 0.5), 184 320 triangles at
+    levels = 6, valence-5/7 extraordinary vertices (irregular Loop patches), helix -60 -> +60
+    deg apex to rim, active strain 0.12 times a Gaussian pulse in time (ramp type 4,
+    t0 = 0.3, sigma = 0.1, dt = 0.01).  Current state: the reference shell contracted by 2 %
+    towards the axis and 1 % along it, so the force is not the reference-state force."""
+    from .mesh import ellipsoid_dome
+    m = ellipsoid_dome(levels=levels, seed=seed)
+    p = ActParams.from_young(1.0, 0.3, n_qp=3, c33_iters=3, cart_fib_act=1, h0=0.01,
+                             gamma=0.12, ramp_type=4, pulse_t0=0.3, pulse_sigma=0.1, ti=ti,
+                             dt=0.01, tisteps=100)
+    x, y, z = m.x0.copy(), m.y0.copy(), m.z0.copy()
+    n = m.n_v
+    x[:n] *= 0.98
+    y[:n] *= 0.98
+    z[:n] *= 0.99
+    m.apply_ghosts(x, y, z)
+    return m, p, (x, y, z)

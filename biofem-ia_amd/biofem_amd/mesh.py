@@ -236,3 +236,175 @@ def centroids(m: Mesh, x=None, y=None, z=None) -> np.ndarray:
     return np.stack([(x[t[:, 0]] + x[t[:, 1]] + x[t[:, 2]]) / 3.0,
                      (y[t[:, 0]] + y[t[:, 1]] + y[t[:, 2]]) / 3.0,
                      (z[t[:, 0]] + z[t[:, 1]] + z[t[:, 2]]) / 3.0], 1)
+
+
+# ------------------------------------------------------------------ ellipsoid dome ------
+def _polar_fan(v: int, rings: int):
+    """Disk of `rings` rings around a centre of valence v, v triangular lattice sectors;
+    returns 2-D points, CCW triangles and the rim in ring order."""
+    ids = {}
+    pts = [(0.0, 0.0)]
+
+    def corner(f):
+        a = 2 * np.pi * (f % v) / v
+        return np.array([np.cos(a), np.sin(a)])
+
+    def vid(f, i, j):
+        f %= v
+        if i == 0 and j == 0:
+            return 0
+        if i == 0:
+            f, i, j = (f + 1) % v, j, 0
+        key = (f, i, j)
+        if key not in ids:
+            p = i * corner(f) + j * corner(f + 1)
+            ids[key] = len(pts)
+            pts.append((p[0], p[1]))
+        return ids[key]
+
+    tris = []
+    for f in range(v):
+        for i in range(rings):
+            for j in range(rings - i):
+                tris.append((vid(f, i, j), vid(f, i + 1, j), vid(f, i, j + 1)))
+                if i + j + 2 <= rings:
+                    tris.append((vid(f, i + 1, j), vid(f, i + 1, j + 1), vid(f, i, j + 1)))
+    rim = [vid(f, rings - k, k) for f in range(v) for k in range(rings)]
+    return np.array(pts), [list(t) for t in tris], rim
+
+
+def _flip_edges(P, T, rim, n_flips, rng):
+    """Flip interior edges (convex quads only, valences kept in 4..8, rim untouched) to
+    create mixed valence-5/7 extraordinary vertices in the coarse mesh."""
+    rimset = set(rim)
+    done = 0
+    for _ in range(2000):
+        if done >= n_flips:
+            break
+        edges = {}
+        for ti, t in enumerate(T):
+            for k in range(3):
+                a, b = t[k], t[(k + 1) % 3]
+                edges.setdefault((min(a, b), max(a, b)), []).append(ti)
+        val = np.zeros(len(P), int)
+        for t in T:
+            for a in t:
+                val[a] += 1
+        cand = [e for e, ts in edges.items() if len(ts) == 2 and e[0] not in rimset and e[1] not in rimset]
+        a, b = cand[rng.integers(len(cand))]
+        t1, t2 = edges[(a, b)]
+        c = [x for x in T[t1] if x not in (a, b)][0]
+        d = [x for x in T[t2] if x not in (a, b)][0]
+        if c in rimset or d in rimset:
+            continue
+        if min(val[a], val[b]) <= 5 or max(val[c], val[d]) >= 7:
+            continue
+
+        def cross(o, p, q):
+            return (P[p][0] - P[o][0]) * (P[q][1] - P[o][1]) - (P[p][1] - P[o][1]) * (P[q][0] - P[o][0])
+        # convexity of quad a-c-b-d: c and d on opposite sides of ab, a and b of cd
+        if cross(c, d, a) * cross(c, d, b) >= 0 or cross(a, b, c) * cross(a, b, d) >= 0:
+            continue
+        # keep CCW orientation
+        n1 = [a, d, c] if cross(a, d, c) > 0 else [a, c, d]
+        n2 = [b, c, d] if cross(b, c, d) > 0 else [b, d, c]
+        T[t1], T[t2] = n1, n2
+        done += 1
+    return T
+
+
+def _refine(P, T, rim):
+    """One 1->4 midpoint refinement in the parameter plane (topology of Loop subdivision)."""
+    P = list(map(tuple, P))
+    mid = {}
+
+    def m(a, b):
+        k = (min(a, b), max(a, b))
+        if k not in mid:
+            mid[k] = len(P)
+            P.append(tuple(0.5 * (np.array(P[a]) + np.array(P[b]))))
+        return mid[k]
+
+    nt = []
+    for a, b, c in T:
+        ab, bc, ca = m(a, b), m(b, c), m(c, a)
+        nt += [[a, ab, ca], [ab, b, bc], [ca, bc, c], [ab, bc, ca]]
+    nrim = []
+    for i, a in enumerate(rim):
+        nrim += [a, m(a, rim[(i + 1) % len(rim)])]
+    return np.array(P), nt, nrim
+
+
+def ellipsoid_dome(levels: int = 6, a: float = 2.2, b: float = 1.0, z_cut: float = 0.5,
+                   apex_valence: int = 5, coarse_rings: int = 3, n_flips: int = 4, seed: int = 3,
+                   helix_deg=(-60.0, 60.0), stride: int = 18) -> Mesh:
+    """Truncated prolate ellipsoid shell x^2/b^2 + y^2/b^2 + z^2/a^2 = 1, z >= z_cut.
+
+    A coarse polar fan (apex valence `apex_valence`) gets `n_flips` random interior edge
+    flips (default_rng(seed)), creating valence-5/7 vertices, and is refined `levels` times
+    (1 -> 4), so extraordinary vertices are isolated and every element touches at most one.
+    The parameter disk maps to the surface by polar angle theta in [0, theta_rim] (apex at
+    z = a) and azimuth.  Rim nodes form one closed boundary ring with the reference's ghost
+    layer (GlobalGhostInit rule).  Fibres follow the reference's LV construction
+    (src/lv_geometry_unstructured.c:313-367) with the helix angle blended linearly from
+    helix_deg[0] at the apex to helix_deg[1] at the rim.  levels = 6 gives 184 320 triangles.
+    """
+    rng = np.random.default_rng(seed)
+    P2, T, rim = _polar_fan(apex_valence, coarse_rings)
+    T = _flip_edges(P2, T, rim, n_flips, rng)
+    for _ in range(levels):
+        P2, T, rim = _refine(P2, T, rim)
+    T = np.array(T, dtype=np.int32)
+    # map the polygonal disk to the dome
+    ang = np.arctan2(P2[:, 1], P2[:, 0])
+    r = np.hypot(P2[:, 0], P2[:, 1])
+    sector = 2 * np.pi / apex_valence
+    loc = np.mod(ang, sector) - 0.5 * sector
+    r_poly = coarse_rings * np.cos(0.5 * sector) / np.cos(loc)
+    rho = np.clip(r / r_poly, 0.0, 1.0)
+    th_rim = np.arccos(z_cut / a)
+    th = rho * th_rim
+    n_v = len(P2)
+    X = np.stack([b * np.sin(th) * np.cos(ang), b * np.sin(th) * np.sin(ang), a * np.cos(th)], 1)
+    rim = np.array(rim, dtype=np.int32)
+    E = len(T)
+    G = len(rim)
+    lib = L.lib()
+    nv = [np.zeros(E + 2 * G, np.int32) for _ in range(3)]
+    for k in range(3):
+        nv[k][:E] = T[:, k]
+    rules = np.zeros(3 * G, np.int32)
+    L.check("bf_topo_ghost_ring", lib.bf_topo_ghost_ring(
+        n_v, E, 0, 1, L.ip(np.array([G], np.int32)), L.ip(rim), L.ip(nv[0]), L.ip(nv[1]),
+        L.ip(nv[2]), L.ip(rules)))
+    tri = np.stack(nv, 1).astype(np.int32)
+    ire, irv, val = (np.zeros(E, np.int32) for _ in range(3))
+    L.check("bf_topo_irrver", lib.bf_topo_irrver(n_v, E, 0, L.ip(_i32(T[:, 0])), L.ip(_i32(T[:, 1])),
+                                                  L.ip(_i32(T[:, 2])), G, L.ip(rim), L.ip(ire),
+                                                  L.ip(irv), L.ip(val)))
+    n_nodes = n_v + G
+    x0, y0, z0 = (np.zeros(n_nodes) for _ in range(3))
+    x0[:n_v], y0[:n_v], z0[:n_v] = X[:, 0], X[:, 1], X[:, 2]
+    m = Mesh(n_v=n_v, n_nodes=n_nodes, n_elmt=E, x0=x0, y0=y0, z0=z0, tri=tri,
+             patch=np.zeros((E, stride), np.int32), ire=ire, irv=irv, val=val, dA0=np.zeros(E),
+             n_fib=np.zeros((E, 3)), gamma_scale=np.ones(E), ghost_rules=rules.reshape(-1, 3),
+             name=f"ellipsoid_L{levels}")
+    m.apply_ghosts(m.x0, m.y0, m.z0)
+    m.patch = build_patch(n_v, n_nodes, E, tri, ire, irv, val, stride)
+    m.dA0 = area0(tri[:E], m.x0, m.y0, m.z0)
+    # fibres (helix blended apex -> rim)
+    c = centroids(m)
+    e_n = np.stack([c[:, 0] / b**2, c[:, 1] / b**2, c[:, 2] / a**2], 1)
+    e_n /= np.linalg.norm(e_n, axis=1, keepdims=True)
+    down = np.array([0.0, 0.0, -1.0])
+    e_l = down - (e_n @ down)[:, None] * e_n
+    nl = np.linalg.norm(e_l, axis=1)
+    fallback = np.array([1.0, 0.0, 0.0]) - (e_n @ np.array([1.0, 0, 0]))[:, None] * e_n
+    e_l = np.where((nl > 1e-10)[:, None], e_l / np.maximum(nl, 1e-300)[:, None],
+                   fallback / np.linalg.norm(fallback, axis=1, keepdims=True))
+    e_c = np.cross(e_n, e_l)
+    zstar = np.clip((a - c[:, 2]) / (a - z_cut), 0.0, 1.0)
+    alpha = np.deg2rad((1 - zstar) * helix_deg[0] + zstar * helix_deg[1])
+    m.n_fib = np.cos(alpha)[:, None] * e_c + np.sin(alpha)[:, None] * e_l
+    m.extra.update(levels=levels, a=a, b=b, z_cut=z_cut, rim=rim)
+    return m

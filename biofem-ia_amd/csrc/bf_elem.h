@@ -418,6 +418,242 @@ BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a
   return out;
 }
 
+// FP32-mixed constitutive chain, written in difference form so that no FP32 subtraction
+// cancels.  The reference forms G - Ce^-1 and Je^2 - Je directly (ElemElasStress
+// :1477-1478), which in FP32 loses log10(1/strain) digits.  Here, with G = g0^{..} and
+// M = (Ce^-1 - G) g0_{..}:
+//   C^-1 - G      = -C^-1 (C - g0) G                  (C - g0 formed in FP64)
+//   Ce^-1 - G     = T (C^-1 - G) T^T + (T G T^T - G)
+//   T G T^T - G   = sum_r (d_r^2 - 1) psi_r psi_r^T    (Fa_cart = diag(d))
+//                 = -gam (2 - gam f.f) psi psi^T,  psi = G (f . g0_i)   (Fa_cart = I - gam f f)
+//   det(Ce^-1) det(g0) = det(I + M) = 1 + d,  d = tr M + minors2(M) + det M
+//   Je - 1        = -d / (sqrt(1 + d) (1 + sqrt(1 + d)))
+//   Se            = -mu (Ce^-1 - G) + K Je (Je - 1) Ce^-1
+// The metrics, their inverses, C33 and its Newton update stay in FP64; stress, tangent and
+// the 3x3 products run in FP32.  Algebraically identical to integrate_element.
+BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<double>& a2c,
+                                                const V3<double>& a3c, const V3<double>& a31c,
+                                                const V3<double>& a32c, const RefGeom& ref,
+                                                const Activation& act, double h0, double mu_d,
+                                                double K_d, int n_qp, int iters,
+                                                const double* theta, const double* wq) {
+  typedef float R;
+  Resultants<float> out;
+  out.N0 = out.N1 = out.N2 = out.M0 = out.M1 = out.M2 = 0.0;
+  out.status = 0;
+  const R mu = (R)mu_d, K = (R)K_d;
+  const double hhalf = 0.5 * h0;
+  const bool rank1 = act.gam != 0.0;
+  const bool diag_id = (act.d0 == 1.0 && act.d1 == 1.0 && act.d2 == 1.0);
+  const double gam_d = act.gam;
+  const double ff_d = dot(act.f, act.f);
+  const double kap_d = rank1 ? gam_d / (1.0 - gam_d * ff_d) : 0.0;
+  const R qcoef = (R)(-gam_d * (2.0 - gam_d * ff_d));  // T G T^T - G = qcoef psi psi^T
+  const double dd[3] = {act.d0, act.d1, act.d2};
+  unsigned skip_mask = 0u;
+  for (int qp = 0; qp < n_qp; qp++) {
+    const double t3 = hhalf * theta[qp];
+    // ---- metrics in FP64 ----
+    const V3<double> g0 = add(a1c, scale(t3, a31c));
+    const V3<double> g1 = add(a2c, scale(t3, a32c));
+    const V3<double> h0v = add(ref.a1, scale(t3, ref.a31));
+    const V3<double> h1v = add(ref.a2, scale(t3, ref.a32));
+    const V3<double> h2v = ref.a3;
+    const Sym<double> gm = gram(g0, g1, a3c);
+    const Sym<double> gm0 = gram(h0v, h1v, h2v);
+    const Sym<double> Gd = sinv(gm0, (double*)nullptr);
+    const Sym<double> gmI = sinv(gm, (double*)nullptr);
+    // full-metric difference (gm - gm0) for the first iteration's C^-1 - G
+    Sym<double> dg;
+    dg.a00 = gm.a00 - gm0.a00; dg.a11 = gm.a11 - gm0.a11; dg.a22 = gm.a22 - gm0.a22;
+    dg.a01 = gm.a01 - gm0.a01; dg.a02 = gm.a02 - gm0.a02; dg.a12 = gm.a12 - gm0.a12;
+    const M33<R> G = full(Sym<R>{(R)Gd.a00, (R)Gd.a11, (R)Gd.a22, (R)Gd.a01, (R)Gd.a02, (R)Gd.a12});
+    const M33<R> g0m = full(Sym<R>{(R)gm0.a00, (R)gm0.a11, (R)gm0.a22, (R)gm0.a01, (R)gm0.a02, (R)gm0.a12});
+    // ---- activation: T = Fa.Cont g0, H = Fa_inv.Cov G, Q = T G T^T - G ----
+    const double phid[3] = {dot(act.f, h0v), dot(act.f, h1v), dot(act.f, h2v)};
+    M33<R> T, H, Q;
+    {
+      // psi = G phi (FP64 then rounded)
+      double psid[3];
+      for (int i = 0; i < 3; i++)
+        psid[i] = (i == 0 ? Gd.a00 : i == 1 ? Gd.a01 : Gd.a02) * phid[0] +
+                  (i == 0 ? Gd.a01 : i == 1 ? Gd.a11 : Gd.a12) * phid[1] +
+                  (i == 0 ? Gd.a02 : i == 1 ? Gd.a12 : Gd.a22) * phid[2];
+      // xi_r[i] = g0_i[r]: components of e_r on the reference covariant basis
+      const double xi[3][3] = {{h0v.x, h1v.x, h2v.x}, {h0v.y, h1v.y, h2v.y}, {h0v.z, h1v.z, h2v.z}};
+      // FaCov = S^T diag(d) S - gam phi phi^T, FiCov = S^T diag(1/d) S + kap phi phi^T
+      double FaC[3][3], FiC[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double a = 0.0, b = 0.0;
+          if (diag_id) {
+            const double gij = (i == j) ? (i == 0 ? gm0.a00 : i == 1 ? gm0.a11 : gm0.a22)
+                                        : ((i + j == 1) ? gm0.a01 : (i + j == 2) ? gm0.a02 : gm0.a12);
+            a = b = gij;
+          } else {
+            for (int r = 0; r < 3; r++) {
+              a += dd[r] * xi[r][i] * xi[r][j];
+              b += (1.0 / dd[r]) * xi[r][i] * xi[r][j];
+            }
+          }
+          FaC[i][j] = a - gam_d * phid[i] * phid[j];
+          FiC[i][j] = b + kap_d * phid[i] * phid[j];
+        }
+      const double Gf[3][3] = {{Gd.a00, Gd.a01, Gd.a02}, {Gd.a01, Gd.a11, Gd.a12}, {Gd.a02, Gd.a12, Gd.a22}};
+      const double g0f[3][3] = {{gm0.a00, gm0.a01, gm0.a02}, {gm0.a01, gm0.a11, gm0.a12}, {gm0.a02, gm0.a12, gm0.a22}};
+      // T = G FaCov G g0 = G FaCov (FP64: it multiplies the small C^-1 - G only, but its
+      // identity part must be exact for the Q split to hold)
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double t = 0.0, hh = 0.0;
+          for (int k = 0; k < 3; k++) {
+            t += Gf[i][k] * FaC[k][j];
+            hh += FiC[i][k] * Gf[k][j];
+          }
+          T.m[i][j] = (R)t;
+          H.m[i][j] = (R)hh;
+        }
+      (void)g0f;
+      if (diag_id) {
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) Q.m[i][j] = qcoef * (R)psid[i] * (R)psid[j];
+      } else {
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) Q.m[i][j] = 0;
+        for (int r = 0; r < 3; r++) {
+          double pr[3];
+          for (int i = 0; i < 3; i++) pr[i] = Gf[i][0] * xi[r][0] + Gf[i][1] * xi[r][1] + Gf[i][2] * xi[r][2];
+          const R c = (R)((dd[r] - 1.0) * (dd[r] + 1.0));
+          for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) Q.m[i][j] += c * (R)pr[i] * (R)pr[j];
+        }
+      }
+    }
+    const M33<R> H2 = mul(H, H);
+    // ---- C33 iteration ----
+    const double c00 = gm.a00, c01 = gm.a01, c11 = gm.a11;
+    double c33 = 1.0;
+    // E = C^-1 - G for the first iteration (C^-1 = gm^-1 in-plane block, 1 at [2][2])
+    M33<R> E;
+    {
+      const M33<double> gmIf = full(gmI), dgf = full(dg), Gfull = full(Gd);
+      const M33<double> X = mul(mul(gmIf, dgf), Gfull);  // gm^-1 (gm - gm0) gm0^-1
+      E.m[0][0] = (R)(-X.m[0][0]); E.m[0][1] = (R)(-X.m[0][1]); E.m[1][0] = (R)(-X.m[1][0]);
+      E.m[1][1] = (R)(-X.m[1][1]);
+      E.m[0][2] = E.m[2][0] = (R)(-Gd.a02);
+      E.m[1][2] = E.m[2][1] = (R)(-Gd.a12);
+      E.m[2][2] = (R)(1.0 - Gd.a22);
+    }
+    R S00 = 0, S11 = 0, S01 = 0;
+    for (int it = 0; it < iters; it++) {
+      // dC = T E T^T + Q  (Ce^-1 - G)
+      const M33<R> TE = mul(T, E);
+      M33<R> dC;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          dC.m[i][j] = TE.m[i][0] * T.m[j][0] + TE.m[i][1] * T.m[j][1] + TE.m[i][2] * T.m[j][2] + Q.m[i][j];
+      // M = dC g0, d = det(I + M) - 1
+      const M33<R> Mm = mul(dC, g0m);
+      const R tr = Mm.m[0][0] + Mm.m[1][1] + Mm.m[2][2];
+      const R mn2 = (Mm.m[0][0] * Mm.m[1][1] - Mm.m[0][1] * Mm.m[1][0]) +
+                    (Mm.m[0][0] * Mm.m[2][2] - Mm.m[0][2] * Mm.m[2][0]) +
+                    (Mm.m[1][1] * Mm.m[2][2] - Mm.m[1][2] * Mm.m[2][1]);
+      const R dm = Mm.m[0][0] * (Mm.m[1][1] * Mm.m[2][2] - Mm.m[1][2] * Mm.m[2][1]) -
+                   Mm.m[0][1] * (Mm.m[1][0] * Mm.m[2][2] - Mm.m[1][2] * Mm.m[2][0]) +
+                   Mm.m[0][2] * (Mm.m[1][0] * Mm.m[2][1] - Mm.m[1][1] * Mm.m[2][0]);
+      const R dlt = tr + mn2 + dm;
+      const R sq = sqrt(R(1) + dlt);
+      const R Je = R(1) / sq;
+      const R Jm1 = -dlt / (sq * (R(1) + sq));
+      const R JJ = Je * Jm1;
+      // Ch = G + dC;  Se = -mu dC + K JJ Ch
+      M33<R> Ch, Se;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          Ch.m[i][j] = G.m[i][j] + dC.m[i][j];
+          Se.m[i][j] = -mu * dC.m[i][j] + K * JJ * Ch.m[i][j];
+        }
+      const M33<R> SeH = mul(Se, H);
+      S00 = R(0.5) * (H.m[0][0] * SeH.m[0][0] + H.m[1][0] * SeH.m[1][0] + H.m[2][0] * SeH.m[2][0] +
+                      Se.m[0][0] * H2.m[0][0] + Se.m[1][0] * H2.m[1][0] + Se.m[2][0] * H2.m[2][0]);
+      S11 = R(0.5) * (H.m[0][1] * SeH.m[0][1] + H.m[1][1] * SeH.m[1][1] + H.m[2][1] * SeH.m[2][1] +
+                      Se.m[0][1] * H2.m[0][1] + Se.m[1][1] * H2.m[1][1] + Se.m[2][1] * H2.m[2][1]);
+      S01 = R(0.5) * (H.m[0][0] * SeH.m[0][1] + H.m[1][0] * SeH.m[1][1] + H.m[2][0] * SeH.m[2][1] +
+                      Se.m[0][0] * H2.m[0][1] + Se.m[1][0] * H2.m[1][1] + Se.m[2][0] * H2.m[2][1]);
+      const R S22 = R(0.5) * (H.m[0][2] * SeH.m[0][2] + H.m[1][2] * SeH.m[1][2] + H.m[2][2] * SeH.m[2][2] +
+                              Se.m[0][2] * H2.m[0][2] + Se.m[1][2] * H2.m[1][2] + Se.m[2][2] * H2.m[2][2]);
+      // CC^{2222}
+      const R alpha = K * JJ - mu;
+      const R beta = R(0.5) * K * Je * (R(2) * Je - R(1));
+      const R hx = H.m[0][2], hy = H.m[1][2], hz = H.m[2][2];
+      const R Px = H2.m[0][2], Py = H2.m[1][2], Pz = H2.m[2][2];
+      const R rx = Ch.m[0][2], ry = Ch.m[1][2], rz = Ch.m[2][2];
+      const R Chx = Ch.m[0][0] * hx + Ch.m[0][1] * hy + Ch.m[0][2] * hz;
+      const R Chy = Ch.m[1][0] * hx + Ch.m[1][1] * hy + Ch.m[1][2] * hz;
+      const R Chz = Ch.m[2][0] * hx + Ch.m[2][1] * hy + Ch.m[2][2] * hz;
+      const R b = hx * Chx + hy * Chy + hz * Chz;
+      const R BhP = Px * Chx + Py * Chy + Pz * Chz;
+      const R CPx = Ch.m[0][0] * Px + Ch.m[0][1] * Py + Ch.m[0][2] * Pz;
+      const R CPy = Ch.m[1][0] * Px + Ch.m[1][1] * Py + Ch.m[1][2] * Pz;
+      const R CPz = Ch.m[2][0] * Px + Ch.m[2][1] * Py + Ch.m[2][2] * Pz;
+      const R BPP = Px * CPx + Py * CPy + Pz * CPz;
+      const R Pr = Px * rx + Py * ry + Pz * rz;
+      const R hr = hx * rx + hy * ry + hz * rz;
+      const R CC = R(0.25) * (R(2) * (beta - alpha) * b * b + R(4) * beta * b * Pr -
+                              R(4) * alpha * BhP * hr + R(2) * beta * Pr * Pr -
+                              alpha * BPP * Ch.m[2][2] - alpha * Pr * Pr);
+      // ModElemC33 in FP64
+      if (!(fabs(CC) > R(1e-14))) {
+        skip_mask |= 1u << it;
+        out.status |= kStatCC3333;
+      } else if (!((skip_mask >> it) & 1u)) {
+        double dCd = -(double)S22 / (double)CC;
+        const double cold = c33;
+        double trial = cold + dCd;
+        if (c00 * (c11 * trial) - c01 * (c01 * trial) <= 0.0) {
+          double lo = 0, hi = 1;
+          for (int bk = 0; bk < 16; bk++) {
+            const double mid = 0.5 * (lo + hi);
+            const double cm = cold + mid * dCd;
+            if (c00 * (c11 * cm) - c01 * (c01 * cm) > 0.0) lo = mid; else hi = mid;
+          }
+          dCd *= lo;
+          trial = cold + dCd;
+        }
+        c33 = trial;
+      }
+      // E = C^-1 - G = -C^-1 (C - g0) G with the block-diagonal C (FP64 inverse)
+      {
+        const double det = c00 * (c33 * c11) - c01 * (c33 * c01);
+        const double c = 1.0 / det;
+        const double Ci[3][3] = {{c * (c33 * c11), -c * (c33 * c01), 0.0},
+                                 {-c * (c33 * c01), c * (c33 * c00), 0.0},
+                                 {0.0, 0.0, c * (c11 * c00 - c01 * c01)}};
+        const double Dm[3][3] = {{dg.a00, dg.a01, -gm0.a02},
+                                 {dg.a01, dg.a11, -gm0.a12},
+                                 {-gm0.a02, -gm0.a12, c33 - gm0.a22}};
+        const double Gf[3][3] = {{Gd.a00, Gd.a01, Gd.a02}, {Gd.a01, Gd.a11, Gd.a12}, {Gd.a02, Gd.a12, Gd.a22}};
+        double CD[3][3];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) CD[i][j] = Ci[i][0] * Dm[0][j] + Ci[i][1] * Dm[1][j] + Ci[i][2] * Dm[2][j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            E.m[i][j] = (R)(-(CD[i][0] * Gf[0][j] + CD[i][1] * Gf[1][j] + CD[i][2] * Gf[2][j]));
+      }
+    }
+    out.c33[qp] = c33;
+    const double w = wq[qp];
+    const double wt = w * theta[qp];
+    out.N0 += w * (double)S00;
+    out.N1 += w * (double)S11;
+    out.N2 += w * (double)S01;
+    out.M0 += wt * (double)S00;
+    out.M1 += wt * (double)S11;
+    out.M2 += wt * (double)S01;
+  }
+  return out;
+}
+
 // Nodal force of one patch node a (ElemUpdFint :1879-1913 / :1934-1983), summed over qps:
 //   Fb_a = pref0 [ (Na0 N0 + Na1 N2) a1 + (Na1 N1 + Na0 N2) a2
 //                  - (h0/2) (c0 M0 + c1 M1 + 2 c2 M2) a3 ],

@@ -84,8 +84,13 @@ __global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
     act.gam = 0.0;
   }
 
-  const Resultants<R> r = integrate_element<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0,
-                                               kp.mu, kp.K, kp.n_qp, kp.iters, kp.theta, kp.w);
+  Resultants<R> r;
+  if constexpr (sizeof(R) == 4)
+    r = integrate_element_mixed(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K,
+                                kp.n_qp, kp.iters, kp.theta, kp.w);
+  else
+    r = integrate_element<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.n_qp,
+                             kp.iters, kp.theta, kp.w);
   for (int q = 0; q < kp.n_qp; q++) m.c33[(size_t)q * E + e] = r.c33[q];
 
   // ---- nodal forces (ElemUpdFint, active_strain.c:1762-1995) ----

@@ -98,8 +98,11 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
       N0 = r.N0; N1 = r.N1; N2 = r.N2; M0 = r.M0; M1 = r.M1; M2 = r.M2; st = r.status;
       if (c33_out) for (int q = 0; q < n_qp; q++) c33_out[(size_t)e * n_qp + q] = r.c33[q];
     } else {
-      Resultants<float> r = integrate_element<float>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act,
-                                                     h0, mu, K, n_qp, iters, theta, wq);
+      Resultants<float> r = (precision == 3)
+          ? integrate_element<float>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp,
+                                     iters, theta, wq)
+          : integrate_element_mixed(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp,
+                                    iters, theta, wq);
       N0 = r.N0; N1 = r.N1; N2 = r.N2; M0 = r.M0; M1 = r.M1; M2 = r.M2; st = r.status;
       if (c33_out) for (int q = 0; q < n_qp; q++) c33_out[(size_t)e * n_qp + q] = r.c33[q];
     }

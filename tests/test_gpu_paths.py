@@ -1,0 +1,118 @@
+"""GPU coverage beyond the flat membrane: irregular Loop patches (ellipsoid dome), the
+activation variants of ElemActDefGrad, per-element activation delays, the LV apex-cap CST
+branch, UpdateElementThickness and the error codes of the C ABI."""
+import copy
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from biofem_amd import _lib as L
+from biofem_amd import configs
+from biofem_amd.fem import ActStrainFE
+from biofem_amd.mesh import build_patch, flat_membrane
+from oracle import bf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def check_all(m, p, x, y, z, tol_exact=0.0, tol_fast=1e-10, tol_mixed=1e-5, omp=0):
+    ref, st, c33 = O.finternal_act(m, p, x, y, z, want_c33=True, omp_threads=omp)
+    fe = ActStrainFE(m, p)
+    try:
+        got = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
+        e = rel(got, ref)
+        print(f"  plain {e:.3e} bitwise {np.array_equal(got, ref)}", flush=True)
+        assert e <= tol_exact
+        if tol_exact == 0.0:
+            assert np.array_equal(fe.c33(), c33)
+        fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+        mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
+        print(f"  fast {rel(fast, ref):.3e} mixed {rel(mixed, ref):.3e}", flush=True)
+        assert rel(fast, ref) <= tol_fast
+        assert rel(mixed, ref) <= tol_mixed
+        return fe.thickness(), c33
+    finally:
+        fe.close()
+
+
+def test_ellipsoid_small_irregular():
+    m, p, xyz = configs.ellipsoid_shell(levels=3)
+    assert (m.ire == 1).sum() > 0
+    print("case ellipsoid levels=3", m.n_elmt, flush=True)
+    check_all(m, p, *xyz)
+
+
+def test_ellipsoid_full_bitwise():
+    m, p, xyz = configs.ellipsoid_shell(levels=6)
+    print("case ellipsoid levels=6", m.n_elmt, flush=True)
+    check_all(m, p, *xyz, omp=16)
+
+
+@pytest.mark.parametrize("mode", ["isotropic", "manufactured", "ramp3_delay", "ramp1"])
+def test_activation_variants(mode):
+    m, p, _ = configs.membrane_small()
+    x, y, z = configs.smooth_displacement(m, 0.01)
+    tol_exact = 0.0
+    if mode == "isotropic":
+        p.cart_fib_act = 0
+    elif mode == "manufactured":
+        p.cart_fib_act, p.manufactured = 0, 1
+        p.manufactured_gamma0, p.manufactured_T, p.ti, p.dt = 0.1, 1.0, 30, 0.01
+    elif mode == "ramp3_delay":
+        p.ramp_type, p.ti, p.dt = 3, 20, 0.01
+        m.activation_delay = np.linspace(0.0, 0.1, m.n_elmt)
+        tol_exact = 1e-13  # device exp() against glibc exp()
+    else:
+        p.ramp_type, p.ti, p.dt, p.tisteps = 1, 25, 0.01, 100
+    print("case", mode, flush=True)
+    check_all(m, p, x, y, z, tol_exact=tol_exact, tol_fast=1e-12)
+
+
+def test_thickness_matches_reference_formula():
+    m, p, (x, y, z) = configs.membrane_small()
+    th, c33 = check_all(m, p, x, y, z)
+    mean = c33.mean(axis=1)
+    np.testing.assert_allclose(th, p.h0 * np.sqrt(mean), rtol=1e-15)
+
+
+def test_lv_cap_branch():
+    """Elements >= n_elmt_base use the flat-triangle (CST) membrane branch
+    (active_strain.c:448-472, :1807-1854)."""
+    m = flat_membrane(12, 9, 1.0, 0.75, jitter=1e-3, seed=0)
+    base = m.n_elmt - 20
+    m.n_elmt_base = base
+    m.patch = build_patch(m.n_v, m.n_nodes, m.n_elmt, m.tri, m.ire, m.irv, m.val, 16,
+                          n_elmt_base=base)
+    p = configs.membrane_small()[1]
+    x, y, z = configs.smooth_displacement(m, 0.01)
+    print("case lv cap", flush=True)
+    check_all(m, p, x, y, z, tol_fast=1e-12)
+
+
+def test_error_codes():
+    lib = L.lib()
+    m, p, (x, y, z) = configs.membrane_small()
+    ctx = C.c_void_p()
+    L.check("create", lib.bf_ctx_create(0, C.byref(ctx)))
+    try:
+        assert lib.bf_force_eval(ctx, L.dp(x), L.dp(y), L.dp(z), L.dp(np.zeros(3 * m.n_nodes)), 0, 0) == L.BF_ERR_STATE
+        bad = copy.deepcopy(m)
+        bad.patch[5, 4] = L.SENTINEL
+        d = bad.desc()
+        assert lib.bf_mesh_upload(ctx, C.byref(d)) == L.BF_ERR_ELEMENT
+        d = m.desc()
+        L.check("upload", lib.bf_mesh_upload(ctx, C.byref(d)))
+        q = p.to_c()
+        q.n_qp = 6
+        assert lib.bf_params_set(ctx, C.byref(q)) == L.BF_ERR_ARG
+        q = p.to_c()
+        L.check("params", lib.bf_params_set(ctx, C.byref(q)))
+        assert lib.bf_force_eval(ctx, L.dp(x), L.dp(y), L.dp(z), L.dp(np.zeros(3 * m.n_nodes)), 0, 9) == L.BF_ERR_ARG
+        assert lib.bf_ctx_create(99, C.byref(C.c_void_p())) == L.BF_ERR_ARG
+    finally:
+        lib.bf_ctx_destroy(ctx)

@@ -30,6 +30,9 @@ ERR_NAMES = {
 }
 BF_ELEM_CC3333 = 6
 BF_ELEM_NONFINITE = 7
+FLAG_CC3333 = 1
+FLAG_NONFINITE = 2
+FLAG_CLAMP = 4
 BF_PREC_FP64 = 0
 BF_PREC_MIXED = 1
 BF_PREC_FP64_FAST = 2
@@ -76,6 +79,7 @@ SIGNATURES = {
     "bf_ghost_update_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "bf_c33_download": (C.c_int, [_vp, _dp]),
     "bf_update_thickness": (C.c_int, [_vp, _dp]),
+    "bf_elem_flags_download": (C.c_int, [_vp, C.POINTER(C.c_ubyte)]),
     "bf_last_error": (C.c_int, [_vp, _ip, _ip, _ip]),
     "bf_last_timing": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "bf_timing_enable": (C.c_int, [_vp, C.c_int]),

@@ -99,6 +99,12 @@ class ActStrainFE:
         L.check("bf_c33_download", self.lib.bf_c33_download(self.ctx, L.dp(out)))
         return out.reshape(self.mesh.n_elmt, self.params.n_qp)
 
+    def elem_flags(self) -> np.ndarray:
+        out = np.zeros(self.mesh.n_elmt, np.uint8)
+        L.check("bf_elem_flags_download",
+                self.lib.bf_elem_flags_download(self.ctx, out.ctypes.data_as(C.POINTER(C.c_ubyte))))
+        return out
+
     def thickness(self) -> np.ndarray:
         out = np.zeros(self.mesh.n_elmt)
         L.check("bf_update_thickness", self.lib.bf_update_thickness(self.ctx, L.dp(out)))

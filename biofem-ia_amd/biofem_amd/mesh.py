@@ -313,26 +313,40 @@ def _flip_edges(P, T, rim, n_flips, rng):
     return T
 
 
-def _refine(P, T, rim):
-    """One 1->4 midpoint refinement in the parameter plane (topology of Loop subdivision)."""
-    P = list(map(tuple, P))
-    mid = {}
-
-    def m(a, b):
-        k = (min(a, b), max(a, b))
-        if k not in mid:
-            mid[k] = len(P)
-            P.append(tuple(0.5 * (np.array(P[a]) + np.array(P[b]))))
-        return mid[k]
-
-    nt = []
-    for a, b, c in T:
-        ab, bc, ca = m(a, b), m(b, c), m(c, a)
-        nt += [[a, ab, ca], [ab, b, bc], [ca, bc, c], [ab, bc, ca]]
-    nrim = []
-    for i, a in enumerate(rim):
-        nrim += [a, m(a, rim[(i + 1) % len(rim)])]
-    return np.array(P), nt, nrim
+def _loop_refine(P, T, rim):
+    """One Loop subdivision step in 3-D (topology 1 -> 4): interior edge points
+    3/8 (a + b) + 1/8 (c + d), interior vertices (1 - n b) v + b sum(nbrs) with Loop's
+    b(n) = (5/8 - (3/8 + cos(2 pi / n) / 4)^2) / n; the rim follows the cubic B-spline curve
+    rules (midpoints, 1/8 3/4 1/8).  New nodes: old vertices keep their ids, edge points
+    follow in edge order; the four children of a triangle are contiguous."""
+    T = np.asarray(T, dtype=np.int64)
+    n = len(P)
+    sides = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
+    opp = np.concatenate([T[:, 2], T[:, 0], T[:, 1]])
+    key = np.minimum(sides[:, 0], sides[:, 1]) * n + np.maximum(sides[:, 0], sides[:, 1])
+    ukey, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    a, b = ukey // n, ukey % n
+    osum = np.zeros((len(ukey), 3))
+    np.add.at(osum, inv, P[opp])
+    bnd_edge = cnt == 1
+    EP = np.where(bnd_edge[:, None], 0.5 * (P[a] + P[b]), 0.375 * (P[a] + P[b]) + 0.125 * osum)
+    nsum = np.zeros((n, 3))
+    val = np.zeros(n)
+    np.add.at(nsum, a, P[b]); np.add.at(nsum, b, P[a])
+    np.add.at(val, a, 1.0); np.add.at(val, b, 1.0)
+    beta = (0.625 - (0.375 + 0.25 * np.cos(2 * np.pi / np.maximum(val, 1))) ** 2) / np.maximum(val, 1)
+    VP = (1 - val * beta)[:, None] * P + beta[:, None] * nsum
+    rim = np.asarray(rim)
+    prev, nxt = np.roll(rim, 1), np.roll(rim, -1)
+    VP[rim] = 0.75 * P[rim] + 0.125 * (P[prev] + P[nxt])
+    eid = n + inv
+    e01, e12, e20 = eid[: len(T)], eid[len(T): 2 * len(T)], eid[2 * len(T):]
+    nt = np.stack([np.stack([T[:, 0], e01, e20], 1), np.stack([e01, T[:, 1], e12], 1),
+                   np.stack([e20, e12, T[:, 2]], 1), np.stack([e01, e12, e20], 1)], 1).reshape(-1, 3)
+    rk = np.minimum(rim, nxt) * n + np.maximum(rim, nxt)
+    rmid = n + np.searchsorted(ukey, rk)
+    nrim = np.stack([rim, rmid], 1).reshape(-1)
+    return np.vstack([VP, EP]), nt, nrim
 
 
 def ellipsoid_dome(levels: int = 6, a: float = 2.2, b: float = 1.0, z_cut: float = 0.5,
@@ -342,9 +356,10 @@ def ellipsoid_dome(levels: int = 6, a: float = 2.2, b: float = 1.0, z_cut: float
 
     A coarse polar fan (apex valence `apex_valence`) gets `n_flips` random interior edge
     flips (default_rng(seed)), creating valence-5/7 vertices, and is refined `levels` times
-    (1 -> 4), so extraordinary vertices are isolated and every element touches at most one.
-    The parameter disk maps to the surface by polar angle theta in [0, theta_rim] (apex at
-    z = a) and azimuth.  Rim nodes form one closed boundary ring with the reference's ghost
+    (1 -> 4) with Loop subdivision masks in 3-D, so the node positions are smooth,
+    extraordinary vertices are isolated and every element touches at most one.  The refined
+    net is projected onto the ellipsoid with a smooth polar-angle stretch that puts the rim
+    on z = z_cut.  Rim nodes form one closed boundary ring with the reference's ghost
     layer (GlobalGhostInit rule).  Fibres follow the reference's LV construction
     (src/lv_geometry_unstructured.c:313-367) with the helix angle blended linearly from
     helix_deg[0] at the apex to helix_deg[1] at the rim.  levels = 6 gives 184 320 triangles.
@@ -352,20 +367,36 @@ def ellipsoid_dome(levels: int = 6, a: float = 2.2, b: float = 1.0, z_cut: float
     rng = np.random.default_rng(seed)
     P2, T, rim = _polar_fan(apex_valence, coarse_rings)
     T = _flip_edges(P2, T, rim, n_flips, rng)
-    for _ in range(levels):
-        P2, T, rim = _refine(P2, T, rim)
-    T = np.array(T, dtype=np.int32)
-    # map the polygonal disk to the dome
-    ang = np.arctan2(P2[:, 1], P2[:, 0])
-    r = np.hypot(P2[:, 0], P2[:, 1])
-    sector = 2 * np.pi / apex_valence
-    loc = np.mod(ang, sector) - 0.5 * sector
-    r_poly = coarse_rings * np.cos(0.5 * sector) / np.cos(loc)
-    rho = np.clip(r / r_poly, 0.0, 1.0)
     th_rim = np.arccos(z_cut / a)
-    th = rho * th_rim
-    n_v = len(P2)
-    X = np.stack([b * np.sin(th) * np.cos(ang), b * np.sin(th) * np.sin(ang), a * np.cos(th)], 1)
+
+    def on_surface(th, ph):
+        return np.stack([b * np.sin(th) * np.cos(ph), b * np.sin(th) * np.sin(ph), a * np.cos(th)], 1)
+
+    # coarse control net on the surface (any placement: subdivision smooths it)
+    ang = np.arctan2(P2[:, 1], P2[:, 0])
+    P = on_surface(th_rim * np.hypot(P2[:, 0], P2[:, 1]) / coarse_rings, ang)
+    for _ in range(levels):
+        P, T, rim = _loop_refine(P, T, rim)
+    # project onto the ellipsoid along (x/b, y/b, z/a), then stretch the polar angle
+    # smoothly so that the rim lands on z = z_cut: th' = th (1 + (th_rim / th_b(ph) - 1) s),
+    # s = (th / th_b(ph))^2, th_b a Fourier fit of the subdivided rim's polar angle.
+    q = P / np.array([b, b, a])
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    th = np.arccos(np.clip(q[:, 2], -1.0, 1.0))
+    ph = np.arctan2(q[:, 1], q[:, 0])
+    K = 12
+    A = np.concatenate([np.ones((len(rim), 1))] + [f(k * ph[rim])[:, None] for k in range(1, K + 1)
+                                                   for f in (np.cos, np.sin)], 1)
+    coef = np.linalg.lstsq(A, th[rim], rcond=None)[0]
+    Aall = np.concatenate([np.ones((len(ph), 1))] + [f(k * ph)[:, None] for k in range(1, K + 1)
+                                                     for f in (np.cos, np.sin)], 1)
+    thb = Aall @ coef
+    sfac = (th / thb) ** 2
+    th2 = th * (1.0 + (th_rim / thb - 1.0) * sfac)
+    th2[rim] = th_rim
+    X = on_surface(th2, ph)
+    T = np.asarray(T, dtype=np.int32)
+    n_v = len(X)
     rim = np.array(rim, dtype=np.int32)
     E = len(T)
     G = len(rim)

@@ -38,7 +38,9 @@ constexpr int kMaxQp = 5;
 enum : int { kRegular = 0, kIrregular = 1, kCap = 2 };
 
 // per-element status bits (OR-ed into a flag word, first element kept by atomicMin)
-enum : int { kStatCC3333 = 1, kStatNonFinite = 2 };
+enum : int { kStatCC3333 = 1, kStatNonFinite = 2, kStatClamp = 4 };  // kStatClamp: the det(C) > 0
+// bisection of ModElemC33 (active_strain.c:1696-1706) ran: the element is outside the regime
+// where the local C33 iteration is well conditioned
 
 template <class R>
 struct V3 {
@@ -391,6 +393,7 @@ BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a
           }
           dC *= lo;
           trial = cold + dC;
+          out.status |= kStatClamp;
         }
         c33 = trial;
       }
@@ -619,6 +622,7 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
           }
           dCd *= lo;
           trial = cold + dCd;
+          out.status |= kStatClamp;
         }
         c33 = trial;
       }

@@ -228,6 +228,7 @@ BF_HD void integrate(const ElemIn& in, ElemOut& out) {
           }
           DeltaC33 *= lo;
           CCov.m[2][2] = C33_old + DeltaC33;
+          out.status |= kStatClamp;
         }
       }
       // ---- C_inv.Cont = INV(C.Cov) :2101 ----

@@ -139,6 +139,7 @@ __global__ void __launch_bounds__(128) bf_element_kernel_exact(DevMesh m, KParam
     }
   }
   if (!finite) atomicMin(&m.status[0], e);
+  m.flags[e] = (unsigned char)(o.status | (finite ? 0 : kStatNonFinite));
   if (o.status & kStatCC3333) {
     atomicMin(&m.status[2], e);
     atomicAdd(&m.status[3], 1);

@@ -129,6 +129,7 @@ __global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
   if (!finite) {
     atomicMin(&m.status[0], e);
   }
+  m.flags[e] = (unsigned char)(r.status | (finite ? 0 : kStatNonFinite));
   if (r.status & kStatCC3333) {
     atomicMin(&m.status[2], e);
     atomicAdd(&m.status[3], 1);
@@ -221,6 +222,7 @@ struct bf_ctx {
   double* d_delay = nullptr;
   double* d_Fe = nullptr;
   double* d_c33 = nullptr;
+  unsigned char* d_flags = nullptr;
   int* d_ptr = nullptr;
   unsigned* d_ent = nullptr;
   int* d_rules = nullptr;
@@ -262,6 +264,7 @@ void dfree(T*& p) {
 void free_mesh(bf_ctx* c) {
   dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
   dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
+  dfree(c->d_flags);
   dfree(c->d_ptr); dfree(c->d_ent); dfree(c->d_rules); dfree(c->d_x); dfree(c->d_y);
   dfree(c->d_z); dfree(c->d_fint);
   c->has_mesh = false;
@@ -485,6 +488,7 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
   ALLOC(c->d_gscale, En);
   ALLOC(c->d_Fe, (size_t)3 * NS * En);
   ALLOC(c->d_c33, (size_t)kMaxQp * En);
+  ALLOC(c->d_flags, En);
   ALLOC(c->d_ptr, (size_t)m->n_v + 1);
   ALLOC(c->d_ent, (size_t)std::max(nnz, 1));
   ALLOC(c->d_x, Nn);
@@ -597,6 +601,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   dm.Fe = c->d_Fe;
   dm.c33 = c->d_c33;
   dm.status = c->d_status;
+  dm.flags = c->d_flags;
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
   hipEvent_t e0 = c->own_ev[0], e1 = c->own_ev[1], e2 = c->own_ev[2];
   c->ev[0] = e0;
@@ -714,6 +719,15 @@ int bf_c33_download(bf_ctx* c, double* c33) {
   BF_HIP(hipStreamSynchronize(c->stream));
   for (int e = 0; e < E; e++)
     for (int k = 0; k < q; k++) c33[(size_t)e * q + k] = buf[(size_t)k * E + e];
+  return BF_OK;
+}
+
+int bf_elem_flags_download(bf_ctx* c, unsigned char* flags) {
+  if (!c || !flags) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  BF_HIP(hipMemcpyAsync(flags, c->d_flags, (size_t)c->E, hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipStreamSynchronize(c->stream));
   return BF_OK;
 }
 

@@ -35,6 +35,7 @@ struct DevMesh {
   double* Fe;          // [3*NS][E]
   double* c33;         // [n_qp][E]
   int* status;         // [4] first error elem, code, first warn elem, warn count
+  unsigned char* flags;  // [E] per-element status bits of the last evaluation
 };
 
 // GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.

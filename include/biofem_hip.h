@@ -125,6 +125,12 @@ int bf_ghost_update_device(bf_ctx *ctx, double *d_x, double *d_y, double *d_z, v
 /* C33 state of the last evaluation, [n_elmt * n_qp] (ead->C[qp].Cov[2][2]). */
 int bf_c33_download(bf_ctx *ctx, double *c33);
 
+/* Per-element status bits of the last evaluation, [n_elmt]: 1 = CC3333 guard tripped
+ * (BF_ELEM_CC3333), 2 = non-finite force, 4 = the det(C) > 0 bisection of ModElemC33
+ * (active_strain.c:1696-1706) ran, i.e. the local C33 iteration left its well-conditioned
+ * regime.  No reference counterpart: the reference clamps silently. */
+int bf_elem_flags_download(bf_ctx *ctx, unsigned char *flags);
+
 /* UpdateElementThickness (active_strain.c:2198-2224): thickness[ec] = h0 sqrt(mean C33). */
 int bf_update_thickness(bf_ctx *ctx, double *thickness);
 

@@ -4,10 +4,11 @@
 A step is one residual's worth of the hot path on device-resident state: the ghost mirror
 update (GlobalGhost) plus one FInternalAct-equivalent force evaluation (element kernel +
 deterministic node gather) of the 256 x 192 active-strain membrane (98 304 triangles,
-3 through-thickness points, 3 C33 Newton iterations).  The headline value is the plain FP64
-path (reference operation order, bitwise equal to the CPU oracle) in elem-QP force
-evaluations per second; the FP64 path on staged contractions and the FP32-mixed path are
-measured in the same run, each with its error against the oracle.
+3 through-thickness points, 3 C33 Newton iterations).  The headline value is the FP64 path
+in elem-QP force evaluations per second: staged contractions, with the elements whose local
+C33 iteration leaves its well-conditioned range recomputed in the reference's operation
+order.  The reference-order FP64 path (bitwise equal to the CPU oracle) and the FP32-mixed
+path are measured in the same run; every path states its error against the oracle.
 
 Printed: one JSON line (see the keys below).  --gpus N > 1 exits: this build measures one GPU.
 """
@@ -91,7 +92,7 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed work, run 3 device copies of this many MiB (a known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--modes", default="fp64,fp64_fast,mixed")
+    ap.add_argument("--modes", default="fp64,fp64_ref_order,mixed")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -112,17 +113,19 @@ def main():
     E, q = m.n_elmt, p.n_qp
     fe = ActStrainFE(m, p, device=0)
     # one host-API evaluation stages the current coordinates in the context's device buffers
-    f64 = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
-    ffast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+    fexact = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
+    f64 = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+    handed64 = fe.fallback_count()
     fmx = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
+    handedmx = fe.fallback_count()
     fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
     elem_bytes, path_bytes, overhead_bytes = byte_model(m, q)
 
     res = {}
-    modes = [(n, v) for n, v in (("fp64", L.BF_PREC_FP64), ("fp64_fast", L.BF_PREC_FP64_FAST),
+    modes = [(n, v) for n, v in (("fp64", L.BF_PREC_FP64_FAST), ("fp64_ref_order", L.BF_PREC_FP64),
                                  ("mixed", L.BF_PREC_MIXED)) if n in args.modes.split(",")]
     for name, prec in modes:
-        steps = args.steps if name != "fp64" else max(3, args.steps // 10)
+        steps = args.steps if name != "fp64_ref_order" else max(3, args.steps // 10)
         log(f"timing {name}: warmup {args.warmup}, steps {steps}")
         wall, n, t_elem, t_gather = run_gpu(fe, lib, L, prec, steps, args.warmup)
         ms = 1e3 * wall / steps
@@ -162,9 +165,10 @@ def main():
                "omp": {"value": E * q / tomp, "threads": nth, "seconds": tomp}}
         nrm = np.linalg.norm(fref)
         parity = {"fp64_rel_err": float(np.linalg.norm(f64 - fref) / nrm),
-                  "fp64_bitwise_equal": bool(np.array_equal(f64, fref)),
-                  "fp64_fast_rel_err": float(np.linalg.norm(ffast - fref) / nrm),
+                  "fp64_ref_order_rel_err": float(np.linalg.norm(fexact - fref) / nrm),
+                  "fp64_ref_order_bitwise_equal": bool(np.array_equal(fexact, fref)),
                   "mixed_rel_err": float(np.linalg.norm(fmx - fref) / nrm),
+                  "elements_handed_to_ref_order": {"fp64": handed64, "mixed": handedmx},
                   "omp_bitwise_equal": bool(np.array_equal(fomp, fref)),
                   "oracle_status": list(st)}
         log(f"cpu {t1:.2f} s single, {tomp:.2f} s x{nth}; parity {parity}")
@@ -199,8 +203,9 @@ def main():
         "config": {
             "workload": f"flat active-strain membrane {args.nx}x{args.ny} ({E} triangles, "
                         f"{m.n_nodes} nodes incl. ghosts), n_qp={q}, C33 iters={p.c33_iters}, "
-                        "plain FP64 (reference operation order); step = ghost update + force "
-                        "evaluation",
+                        "FP64 (staged contractions; elements outside the well-conditioned C33 "
+                        "range recomputed in the reference's operation order); step = ghost "
+                        "update + force evaluation",
             "elements": E, "nodes": m.n_nodes, "n_qp": q, "c33_iters": p.c33_iters,
             "elem_qp_per_step": E * q,
         },
@@ -212,15 +217,17 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": None,
-            "kernel": "bf_element_kernel_exact",
+            "kernel": "bf_element_kernel<double>",
             "algorithmic_bytes_per_launch": elem_bytes,
             "kernel_ms": r64["kernel_ms_element"],
         },
         "cpu_baseline": cpu,
         "fp32_mixed": dict(res["mixed"], speedup_vs_fp64=res["mixed"]["elem_qp_per_s"] / r64["elem_qp_per_s"],
+                           speedup_vs_ref_order=res["mixed"]["elem_qp_per_s"] / res["fp64_ref_order"]["elem_qp_per_s"],
                            roofline_frac=res["mixed"]["gbps_element_kernel"] / HBM_PEAK_GBPS),
-        "fp64_fast": dict(res["fp64_fast"], speedup_vs_fp64=res["fp64_fast"]["elem_qp_per_s"] / r64["elem_qp_per_s"],
-                          roofline_frac=res["fp64_fast"]["gbps_element_kernel"] / HBM_PEAK_GBPS),
+        "fp64_ref_order": dict(res["fp64_ref_order"],
+                               note="reference operation order, no contraction: bitwise equal to the CPU oracle",
+                               speedup_vs_fp64=res["fp64_ref_order"]["elem_qp_per_s"] / r64["elem_qp_per_s"]),
         "fp64_detail": r64,
         "bytes": {"element_kernel": elem_bytes, "path": path_bytes, "overhead_element_buffer": overhead_bytes},
         "parity": parity,

@@ -81,6 +81,8 @@ SIGNATURES = {
     "bf_update_thickness": (C.c_int, [_vp, _dp]),
     "bf_elem_flags_download": (C.c_int, [_vp, C.POINTER(C.c_ubyte)]),
     "bf_last_error": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "bf_last_fallback_count": (C.c_int, [_vp, _ip]),
+    "bf_set_fallback": (C.c_int, [_vp, C.c_int]),
     "bf_last_timing": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "bf_timing_enable": (C.c_int, [_vp, C.c_int]),
     "bf_timing_read": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp, _dp]),

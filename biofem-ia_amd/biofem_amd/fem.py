@@ -99,6 +99,14 @@ class ActStrainFE:
         L.check("bf_c33_download", self.lib.bf_c33_download(self.ctx, L.dp(out)))
         return out.reshape(self.mesh.n_elmt, self.params.n_qp)
 
+    def fallback_count(self) -> int:
+        n = C.c_int()
+        L.check("bf_last_fallback_count", self.lib.bf_last_fallback_count(self.ctx, C.byref(n)))
+        return n.value
+
+    def set_fallback(self, on: bool) -> None:
+        L.check("bf_set_fallback", self.lib.bf_set_fallback(self.ctx, int(on)))
+
     def elem_flags(self) -> np.ndarray:
         out = np.zeros(self.mesh.n_elmt, np.uint8)
         L.check("bf_elem_flags_download",

@@ -38,7 +38,7 @@ constexpr int kMaxQp = 5;
 enum : int { kRegular = 0, kIrregular = 1, kCap = 2 };
 
 // per-element status bits (OR-ed into a flag word, first element kept by atomicMin)
-enum : int { kStatCC3333 = 1, kStatNonFinite = 2, kStatClamp = 4 };  // kStatClamp: the det(C) > 0
+enum : int { kStatCC3333 = 1, kStatNonFinite = 2, kStatClamp = 4, kStatFallback = 8 };  // kStatClamp: the det(C) > 0
 // bisection of ModElemC33 (active_strain.c:1696-1706) ran: the element is outside the regime
 // where the local C33 iteration is well conditioned
 
@@ -656,6 +656,19 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
     out.M2 += wt * (double)S01;
   }
   return out;
+}
+
+// Elements whose staged result must be replaced by the reference-order one: the local C33
+// iteration left the range where the reference's own rounding stays small (a clamp of
+// ModElemC33's bisection, C33 outside [1/2, 2] at any qp, or a non-finite value).  There
+// the 3 half-Newton steps of the reference amplify any rounding difference by orders of
+// magnitude, so the fast paths hand such elements to the plain kernel.
+constexpr double kC33Lo = 0.5, kC33Hi = 2.0;
+BF_HD bool needs_exact(const double* c33, int n_qp, int status, bool finite) {
+  if (!finite || (status & (kStatClamp | kStatCC3333))) return true;
+  for (int q = 0; q < n_qp; q++)
+    if (!(c33[q] >= kC33Lo && c33[q] <= kC33Hi)) return true;
+  return false;
 }
 
 // Nodal force of one patch node a (ElemUpdFint :1879-1913 / :1934-1983), summed over qps:

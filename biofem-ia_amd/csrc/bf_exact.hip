@@ -3,6 +3,8 @@
 // Build flag: -ffp-contract=off (so a*b+c rounds twice, as in the reference's C).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bf_exact.h"
 #include "bf_kernels.h"
 
@@ -46,13 +48,12 @@ __global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E,
 // ElemCGDefTens + ElemC33Solve + ElemUpdFint of one element per lane, every operation in
 // the order of src/active_strain.c.  Writes the same slot-major element-force buffer as the
 // staged kernel; the node gather then sums in element order (active_strain.c:2238-2260).
-__global__ void __launch_bounds__(128) bf_element_kernel_exact(DevMesh m, KParams kp,
-                                                               const double* __restrict__ x,
-                                                               const double* __restrict__ y,
-                                                               const double* __restrict__ z) {
+__device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& kp,
+                                              const double* __restrict__ x,
+                                              const double* __restrict__ y,
+                                              const double* __restrict__ z, int e,
+                                              unsigned char extra_flags) {
   const int E = kp.E;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
   const int meta = m.meta[e];
   const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
   auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
@@ -139,11 +140,32 @@ __global__ void __launch_bounds__(128) bf_element_kernel_exact(DevMesh m, KParam
     }
   }
   if (!finite) atomicMin(&m.status[0], e);
-  m.flags[e] = (unsigned char)(o.status | (finite ? 0 : kStatNonFinite));
+  m.flags[e] = (unsigned char)(o.status | (finite ? 0 : kStatNonFinite) | extra_flags);
   if (o.status & kStatCC3333) {
     atomicMin(&m.status[2], e);
     atomicAdd(&m.status[3], 1);
   }
+}
+
+
+__global__ void __launch_bounds__(128) bf_element_kernel_exact(DevMesh m, KParams kp,
+                                                               const double* __restrict__ x,
+                                                               const double* __restrict__ y,
+                                                               const double* __restrict__ z) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kp.E) return;
+  exact_element(m, kp, x, y, z, e, 0);
+}
+
+// Persistent drain of the fallback list: every lane strides over [0, count); count is
+// final because the producing kernel completed earlier on the same stream.
+__global__ void __launch_bounds__(128) bf_element_kernel_exact_list(DevMesh m, KParams kp,
+                                                                    const double* __restrict__ x,
+                                                                    const double* __restrict__ y,
+                                                                    const double* __restrict__ z) {
+  const int count = m.status[4];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    exact_element(m, kp, x, y, z, m.fallback[i], kStatFallback);
 }
 
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes) {
@@ -154,6 +176,15 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
                         const double* z, double* ref, hipStream_t s) {
   if (E <= 0) return hipSuccess;
   hipLaunchKernelGGL(bf_geom0_kernel, dim3((E + 255) / 256), dim3(256), 0, s, m, E, x, y, z, ref);
+  return hipGetLastError();
+}
+
+hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
+                                         const double* y, const double* z, hipStream_t s) {
+  if (kp.E <= 0) return hipSuccess;
+  // one lane per listed element when the list fills the machine (2048 waves of 64)
+  const int blocks = std::min((kp.E + 127) / 128, 1024);
+  hipLaunchKernelGGL(bf_element_kernel_exact_list, dim3(blocks), dim3(128), 0, s, m, kp, x, y, z);
   return hipGetLastError();
 }
 

@@ -91,6 +91,13 @@ __global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
   else
     r = integrate_element<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.n_qp,
                              kp.iters, kp.theta, kp.w);
+  const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) && isfinite(r.M0) &&
+                      isfinite(r.M1) && isfinite(r.M2);
+  if (kp.fallback && needs_exact(r.c33, kp.n_qp, r.status, finite)) {
+    // outside the well-conditioned regime: the plain kernel recomputes this element
+    m.fallback[atomicAdd(&m.status[4], 1)] = e;
+    return;
+  }
   for (int q = 0; q < kp.n_qp; q++) m.c33[(size_t)q * E + e] = r.c33[q];
 
   // ---- nodal forces (ElemUpdFint, active_strain.c:1762-1995) ----
@@ -124,8 +131,6 @@ __global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
     }
   }
   // ---- status ----
-  const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) && isfinite(r.M0) &&
-                      isfinite(r.M1) && isfinite(r.M2);
   if (!finite) {
     atomicMin(&m.status[0], e);
   }
@@ -192,6 +197,7 @@ __global__ void bf_status_reset(int* st) {
     st[1] = 0;
     st[2] = INT_MAX;
     st[3] = 0;
+    st[4] = 0;
   }
 }
 
@@ -223,6 +229,7 @@ struct bf_ctx {
   double* d_Fe = nullptr;
   double* d_c33 = nullptr;
   unsigned char* d_flags = nullptr;
+  int* d_fallback = nullptr;
   int* d_ptr = nullptr;
   unsigned* d_ent = nullptr;
   int* d_rules = nullptr;
@@ -264,7 +271,7 @@ void dfree(T*& p) {
 void free_mesh(bf_ctx* c) {
   dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
   dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
-  dfree(c->d_flags);
+  dfree(c->d_flags); dfree(c->d_fallback);
   dfree(c->d_ptr); dfree(c->d_ent); dfree(c->d_rules); dfree(c->d_x); dfree(c->d_y);
   dfree(c->d_z); dfree(c->d_fint);
   c->has_mesh = false;
@@ -331,12 +338,13 @@ int bf_ctx_create(int device, bf_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->own_ev[0]) != hipSuccess || hipEventCreate(&c->own_ev[1]) != hipSuccess ||
       hipEventCreate(&c->own_ev[2]) != hipSuccess ||
-      hipMalloc((void**)&c->d_status, 4 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_status, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipMalloc((void**)&c->d_status, 8 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_status, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     bf_ctx_destroy(c);
     return BF_ERR_DEVICE;
   }
   for (int i = 0; i < 3; i++) c->ev[i] = c->own_ev[i];
+  c->kp.fallback = 1;
   *out = c;
   return BF_OK;
 }
@@ -489,6 +497,7 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
   ALLOC(c->d_Fe, (size_t)3 * NS * En);
   ALLOC(c->d_c33, (size_t)kMaxQp * En);
   ALLOC(c->d_flags, En);
+  ALLOC(c->d_fallback, En);
   ALLOC(c->d_ptr, (size_t)m->n_v + 1);
   ALLOC(c->d_ent, (size_t)std::max(nnz, 1));
   ALLOC(c->d_x, Nn);
@@ -602,6 +611,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   dm.c33 = c->d_c33;
   dm.status = c->d_status;
   dm.flags = c->d_flags;
+  dm.fallback = c->d_fallback;
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
   hipEvent_t e0 = c->own_ev[0], e1 = c->own_ev[1], e2 = c->own_ev[2];
   c->ev[0] = e0;
@@ -626,6 +636,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
       hipLaunchKernelGGL(bf_element_kernel<double>, grid, block, 0, s, dm, kp, x, y, z);
     else
       hipLaunchKernelGGL(bf_element_kernel<float>, grid, block, 0, s, dm, kp, x, y, z);
+    if (precision != BF_PREC_FP64 && kp.fallback) BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, s));
   }
   BF_HIP(hipEventRecord(e1, s));
   if (c->n_nodes > 0)
@@ -642,7 +653,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
 }
 
 static int collect_status(bf_ctx* c, int* elem, int* code, int* n_warn) {
-  BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   const int* st = c->h_status;
   int el = -1, cd = 0;
@@ -752,6 +763,20 @@ int bf_last_error(bf_ctx* c, int* elem, int* code, int* n_warn) {
   if (!c) return BF_ERR_ARG;
   BF_HIP(hipSetDevice(c->device));
   collect_status(c, elem, code, n_warn);
+  return BF_OK;
+}
+
+int bf_last_fallback_count(bf_ctx* c, int* n) {
+  if (!c || !n) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  collect_status(c, nullptr, nullptr, nullptr);
+  *n = c->h_status[4];
+  return BF_OK;
+}
+
+int bf_set_fallback(bf_ctx* c, int on) {
+  if (!c) return BF_ERR_ARG;
+  c->kp.fallback = on ? 1 : 0;
   return BF_OK;
 }
 

@@ -16,6 +16,7 @@ struct KParams {
   int E, NS, n_qp, iters;
   int act_mode;    // 0: I - gamma f f (cart_fib_act), 1: manufactured diag, 2: isotropic diag
   int has_delay;   // per-element activation delay present
+  int fallback;    // staged kernels hand ill-conditioned elements to the plain kernel
   double theta[kMaxQp], w[kMaxQp];
   double h0, mu, K;
   double gamma_t;  // GammaOfTimeDelayed(p, 0) evaluated on the host
@@ -34,8 +35,9 @@ struct DevMesh {
   const double* delay;
   double* Fe;          // [3*NS][E]
   double* c33;         // [n_qp][E]
-  int* status;         // [4] first error elem, code, first warn elem, warn count
+  int* status;         // [8] first error elem, code, first warn elem, warn count, fallback count
   unsigned char* flags;  // [E] per-element status bits of the last evaluation
+  int* fallback;       // [E] elements handed from a staged kernel to the plain kernel
 };
 
 // GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
@@ -93,5 +95,9 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
                         const double* z, double* ref, hipStream_t s);
 hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
                                 const double* y, const double* z, hipStream_t s);
+// Re-evaluates, in the reference's operation order, the elements a staged kernel listed in
+// m.fallback[0 .. m.status[4]) (persistent grid, count read on the device).
+hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
+                                         const double* y, const double* z, hipStream_t s);
 
 }  // namespace bf

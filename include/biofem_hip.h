@@ -48,6 +48,9 @@ extern "C" {
                                accumulation */
 #define BF_PREC_FP64_FAST 2 /* FP64 on the staged contractions (closed-form CC^{2222}, 3x3
                                products instead of the 6- and 8-deep index loops) */
+/* Both staged modes hand every element whose local C33 iteration leaves its
+ * well-conditioned range (bisection clamp, or C33 outside [1/2, 2] at a qp) to the plain
+ * kernel, which recomputes it in the reference's operation order (bf_set_fallback). */
 
 #define BF_PATCH_SENTINEL 1000000 /* empty patch slot (src/io.c:306) */
 #define BF_MAX_NEN 18             /* widest patch (valence 12) */
@@ -137,6 +140,12 @@ int bf_update_thickness(bf_ctx *ctx, double *thickness);
 /* First failure of the last evaluation: element index (-1 if none), BF_ELEM_* code,
  * number of elements that raised the non-fatal CC3333 condition.  Synchronises. */
 int bf_last_error(bf_ctx *ctx, int *elem, int *code, int *n_warn);
+
+/* Number of elements the last staged evaluation recomputed in the reference's operation
+ * order; bf_set_fallback(ctx, 0) disables that hand-over (default on).  No reference
+ * counterpart: the reference has a single arithmetic path. */
+int bf_last_fallback_count(bf_ctx *ctx, int *n);
+int bf_set_fallback(bf_ctx *ctx, int on);
 
 /* Wall-clock duration of the last bf_force_eval* kernels on the device, in ms
  * (HIP events on the context stream): element kernel and node-gather kernel. */

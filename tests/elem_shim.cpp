@@ -4,6 +4,7 @@
 // Test-only: it lets the CPU test suite check the staged constitutive algebra and the
 // FP32-mixed error budget against the oracle without a GPU.  It is never linked into
 // or called by the product library.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -63,7 +64,41 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
     act.f = mk(fib[3 * e], fib[3 * e + 1], fib[3 * e + 2]);
     double N0, N1, N2, M0, M1, M2;
     int st;
-    if (precision == 2) {  // reference operation order (bf_exact.h)
+    bool do_exact = precision == 2;
+    if (precision == 4 || precision == 5) {  // staged (4: FP64, 5: mixed) with exact fallback
+      Resultants<double> rd;
+      Resultants<float> rf;
+      const double* cc;
+      int stt;
+      bool fin;
+      if (precision == 4) {
+        rd = integrate_element<double>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp, iters, theta, wq);
+        cc = rd.c33; stt = rd.status; fin = std::isfinite(rd.N0) && std::isfinite(rd.M0);
+        N0 = rd.N0; N1 = rd.N1; N2 = rd.N2; M0 = rd.M0; M1 = rd.M1; M2 = rd.M2;
+      } else {
+        rf = integrate_element_mixed(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp, iters, theta, wq);
+        cc = rf.c33; stt = rf.status; fin = std::isfinite(rf.N0) && std::isfinite(rf.M0);
+        N0 = rf.N0; N1 = rf.N1; N2 = rf.N2; M0 = rf.M0; M1 = rf.M1; M2 = rf.M2;
+      }
+      if (needs_exact(cc, n_qp, stt, fin)) {
+        do_exact = true;
+        warn++;
+      } else {
+        if (c33_out) for (int q = 0; q < n_qp; q++) c33_out[(size_t)e * n_qp + q] = cc[q];
+        const double pref0 = 0.5 * h0 * dA0[e];
+        for (int a = 0; a < nen; a++) {
+          V3<double> f = node_force(na0[a], na1[a], nb0[a], nb1[a], nb2[a], Gc, pref0, 0.5 * h0,
+                                    N0, N1, N2, M0, M1, M2);
+          const int n = pt[a];
+          if (n == kSentinel || n < 0 || n >= n_v) continue;
+          fint[3 * n] += f.x;
+          fint[3 * n + 1] += f.y;
+          fint[3 * n + 2] += f.z;
+        }
+        continue;
+      }
+    }
+    if (do_exact) {  // reference operation order (bf_exact.h)
       exact::ElemIn in;
       in.cur = Gc;
       in.a31c = a31c;
@@ -78,7 +113,7 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
       in.h0 = h0; in.mu = mu; in.K = K; in.n_qp = n_qp; in.iters = iters; in.theta = theta; in.wq = wq;
       exact::ElemOut o;
       exact::integrate(in, o);
-      if (o.status) warn++;
+      if (precision == 2 && o.status) warn++;
       if (c33_out) for (int q = 0; q < n_qp; q++) c33_out[(size_t)e * n_qp + q] = o.c33[q];
       const double pref0 = 0.5 * h0 * dA0[e];
       for (int a = 0; a < nen; a++) {

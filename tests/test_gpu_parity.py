@@ -92,9 +92,10 @@ def test_ghost_update(small):
 
 def test_membrane_large_bitwise():
     """256 x 192 membrane with the N(0, 1e-3) out-of-plane state: a quarter of its elements
-    drive the reference's C33 Newton into its det(C) > 0 bisection clamp, where rounding
-    differences are amplified.  The plain FP64 path must still reproduce the oracle; the
-    staged paths report their error (asserted loosely: they are not the parity path)."""
+    drive the reference's C33 iteration out of its well-conditioned range (down to its
+    det(C) > 0 bisection clamp), where rounding differences are amplified.  The plain FP64
+    path reproduces the oracle bit for bit; the staged paths hand those elements to it and
+    stay within their tolerances."""
     m, p, (x, y, z) = configs.membrane_large()
     print("case membrane 256x192 plain fp64", flush=True)
     ref, st, c33 = O.finternal_act(m, p, x, y, z, omp_threads=16, want_c33=True)
@@ -107,7 +108,9 @@ def test_membrane_large_bitwise():
         np.testing.assert_allclose(fe.c33(), c33, rtol=0, atol=1e-15)
         fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
         mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
-        print("fast fp64 rel err", rel(fast, ref), "mixed rel err", rel(mixed, ref))
-        assert rel(fast, ref) <= 1e-8
+        print("fast fp64 rel err", rel(fast, ref), "mixed rel err", rel(mixed, ref),
+              "handed over", fe.fallback_count())
+        assert rel(fast, ref) <= TOL_FP64
+        assert rel(mixed, ref) <= TOL_MIXED
     finally:
         fe.close()

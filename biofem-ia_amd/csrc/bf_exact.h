@@ -90,154 +90,256 @@ struct ElemOut {
   int status;
 };
 
-BF_HD void integrate(const ElemIn& in, ElemOut& out) {
+// Pick element i of (a0, a1, a2) for a lane-uniform index: keeps 3x3 operands in
+// registers where a runtime-indexed array would spill to scratch.
+BF_HD double sel3(int i, double a0, double a1, double a2) { return i == 0 ? a0 : (i == 1 ? a1 : a2); }
+BF_HD void row3(const Mat& M, int i, double r[3]) {
+  r[0] = sel3(i, M.m[0][0], M.m[1][0], M.m[2][0]);
+  r[1] = sel3(i, M.m[0][1], M.m[1][1], M.m[2][1]);
+  r[2] = sel3(i, M.m[0][2], M.m[1][2], M.m[2][2]);
+}
+BF_HD void col3(const Mat& M, int j, double c[3]) {
+  c[0] = sel3(j, M.m[0][0], M.m[0][1], M.m[0][2]);
+  c[1] = sel3(j, M.m[1][0], M.m[1][1], M.m[1][2]);
+  c[2] = sel3(j, M.m[2][0], M.m[2][1], M.m[2][2]);
+}
+
+// Returns v unchanged, but the device compiler can no longer see that it is a constant:
+// products that start from it stay inside the loop that calls this instead of being
+// hoisted out of it and kept live (81 hoisted FP64 products per contraction would
+// exceed the register file and spill).
+BF_HD double opaque(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(v));
+#endif
+  return v;
+}
+
+// One through-thickness point: ElemUpdateG, ElemActDefGrad, ElemCGDefTens and the C33
+// loop for qp `qp`.  `skip_mask` carries the CC3333-guard state across qps (bit it set:
+// a qp at or before this one failed the guard in iteration it).
+struct QpOut {
+  double S00, S11, S01, c33;
+  int status;
+};
+
+BF_HD QpOut integrate_qp(const ElemIn& in, int qp, unsigned& skip_mask) {
+  QpOut out;
   out.status = 0;
-  const double hhalf = 0.5 * in.h0;  // :913
+  // Fa_cart and its inverse (:1227-1256, :1291): rebuilt per qp (same values every time)
+  // instead of being held live across the qp loop
   Mat Fc, Finv_c;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) Fc.m[i][j] = in.Fa_cart[i][j];
-  inv(Fc, Finv_c);  // :1291 (same inverse for every qp)
-  unsigned skip_mask = 0u;
-  for (int qp = 0; qp < in.n_qp; qp++) {
-    const double theta3 = hhalf * in.theta[qp];  // :936
-    // ---- ElemUpdateG :940-978 ----
-    V3<double> g[3], g0[3];
-    g[0] = add(in.cur.a1, scale(theta3, in.a31c));
-    g[1] = add(in.cur.a2, scale(theta3, in.a32c));
-    g[2] = in.cur.a3;
-    Mat gm, gmI;
-    metric(g, gm);
-    inv(gm, gmI);
-    g0[0] = add(in.ref.a1, scale(theta3, in.ref.a31));
-    g0[1] = add(in.ref.a2, scale(theta3, in.ref.a32));
-    g0[2] = in.ref.a3;
-    Mat gm0, G;
-    metric(g0, gm0);
-    inv(gm0, G);
-    // ---- ElemActDefGrad :1272-1320 ----
-    Mat S, ST, tmp, FaCov, FiCov, FaCont;
-    for (int j = 0; j < 3; j++) {
-      S.m[0][j] = g0[j].x;
-      S.m[1][j] = g0[j].y;
-      S.m[2][j] = g0[j].z;
+  inv(Fc, Finv_c);
+  const double hhalf = 0.5 * in.h0;  // :913
+  const double theta3 = hhalf * in.theta[qp];  // :936
+  // ---- ElemUpdateG :940-978 ----
+  V3<double> g[3], g0[3];
+  g[0] = add(in.cur.a1, scale(theta3, in.a31c));
+  g[1] = add(in.cur.a2, scale(theta3, in.a32c));
+  g[2] = in.cur.a3;
+  Mat gm, gmI;
+  metric(g, gm);
+  inv(gm, gmI);
+  g0[0] = add(in.ref.a1, scale(theta3, in.ref.a31));
+  g0[1] = add(in.ref.a2, scale(theta3, in.ref.a32));
+  g0[2] = in.ref.a3;
+  Mat gm0, G;
+  metric(g0, gm0);
+  inv(gm0, G);
+  // ---- ElemActDefGrad :1272-1320 ----
+  Mat S, ST, tmp, FaCov, Fi, FaCont;
+  for (int j = 0; j < 3; j++) {
+    S.m[0][j] = g0[j].x;
+    S.m[1][j] = g0[j].y;
+    S.m[2][j] = g0[j].z;
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) ST.m[i][j] = S.m[j][i];
+  matmult(ST, Fc, tmp);
+  matmult(tmp, S, FaCov);
+  matmult(ST, Finv_c, tmp);
+  matmult(tmp, S, Fi);
+  raise2(G, FaCov, FaCont);
+  // ---- ElemCGDefTens :1341-1363 ----
+  Mat CCov, Ci;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      CCov.m[i][j] = gm.m[i][j];
+      Ci.m[i][j] = gmI.m[i][j];
     }
+  for (int a = 0; a < 2; a++) {
+    CCov.m[a][2] = 0.0;
+    CCov.m[2][a] = 0.0;
+    Ci.m[a][2] = 0.0;
+    Ci.m[2][a] = 0.0;
+  }
+  CCov.m[2][2] = 1.0;
+  Ci.m[2][2] = 1.0;
+  double S00 = 0.0, S11 = 0.0, S01 = 0.0;
+#pragma unroll 1
+  for (int it = 0; it < in.iters; it++) {
+    // ---- ElemElasCGDefTens :1389-1430: sum over m, p, q, l per entry ----
+    Mat Ch, CeiCov, CeCont, CeCov;
+#pragma unroll 1
+    for (int i = 0; i < 3; i++) {
+      double Fr[3];
+      row3(FaCont, i, Fr);
+#pragma unroll 1
+      for (int j = 0; j < 3; j++) {
+        double Fcj[3];
+        col3(FaCont, j, Fcj);
+        const double Fo[3] = {opaque(Fr[0]), opaque(Fr[1]), opaque(Fr[2])};
+        double acc = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+          for (int pp = 0; pp < 3; pp++)
+#pragma unroll
+            for (int qq = 0; qq < 3; qq++)
+#pragma unroll
+              for (int l = 0; l < 3; l++)
+                acc += Fo[mm] * gm0.m[mm][pp] * Ci.m[pp][qq] * Fcj[l] * gm0.m[l][qq];
+        if (i == 0) {
+          if (j == 0) Ch.m[0][0] = acc; else if (j == 1) Ch.m[0][1] = acc; else Ch.m[0][2] = acc;
+        } else if (i == 1) {
+          if (j == 0) Ch.m[1][0] = acc; else if (j == 1) Ch.m[1][1] = acc; else Ch.m[1][2] = acc;
+        } else {
+          if (j == 0) Ch.m[2][0] = acc; else if (j == 1) Ch.m[2][1] = acc; else Ch.m[2][2] = acc;
+        }
+      }
+    }
+    lower2(gm0, Ch, CeiCov);
+    inv(CeiCov, CeCont);
+    lower2(gm0, CeCont, CeCov);
+    // ---- ElemElasStress :1458-1487 ----
+    const double detCe = det3(CeCov);
+    const double detG0 = det3(gm0);
+    const double Je = sqrt(detCe / detG0);
+    Mat Se;
+#pragma unroll
     for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) ST.m[i][j] = S.m[j][i];
-    matmult(ST, Fc, tmp);
-    matmult(tmp, S, FaCov);
-    matmult(ST, Finv_c, tmp);
-    matmult(tmp, S, FiCov);
-    raise2(G, FaCov, FaCont);
-    // ---- ElemCGDefTens :1341-1363 ----
-    Mat CCov, Ci;
-    for (int i = 0; i < 2; i++)
-      for (int j = 0; j < 2; j++) {
-        CCov.m[i][j] = gm.m[i][j];
-        Ci.m[i][j] = gmI.m[i][j];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const double term1 = in.mu * (G.m[i][j] - Ch.m[i][j]);
+        const double term2 = in.K * Ch.m[i][j] * (Je * Je - Je);
+        Se.m[i][j] = term1 + term2;
       }
-    for (int a = 0; a < 2; a++) {
-      CCov.m[a][2] = 0.0;
-      CCov.m[2][a] = 0.0;
-      Ci.m[a][2] = 0.0;
-      Ci.m[2][a] = 0.0;
-    }
-    CCov.m[2][2] = 1.0;
-    Ci.m[2][2] = 1.0;
-    double Sc[3][3];
-    for (int it = 0; it < in.iters; it++) {
-      // ---- ElemElasCGDefTens :1389-1430 ----
-      Mat Ch, CeiCov, CeCont, CeCov;
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-          double acc = 0.0;
-          for (int mm = 0; mm < 3; mm++)
-            for (int pp = 0; pp < 3; pp++)
-              for (int qq = 0; qq < 3; qq++)
-                for (int l = 0; l < 3; l++)
-                  acc += FaCont.m[i][mm] * gm0.m[mm][pp] * Ci.m[pp][qq] * FaCont.m[l][j] *
-                         gm0.m[l][qq];
-          Ch.m[i][j] = acc;
-        }
-      lower2(gm0, Ch, CeiCov);
-      inv(CeiCov, CeCont);
-      lower2(gm0, CeCont, CeCov);
-      // ---- ElemElasStress :1458-1487 ----
-      const double detCe = det3(CeCov);
-      const double detG0 = det3(gm0);
-      const double Je = sqrt(detCe / detG0);
-      double Se[3][3];
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-          const double term1 = in.mu * (G.m[i][j] - Ch.m[i][j]);
-          const double term2 = in.K * Ch.m[i][j] * (Je * Je - Je);
-          Se[i][j] = term1 + term2;
-        }
-      // ---- ElemTotStress :1506-1528 ----
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-          double acc = 0.0;
-          for (int w = 0; w < 3; w++)
-            for (int pp = 0; pp < 3; pp++)
-              for (int z = 0; z < 3; z++)
-                for (int s = 0; s < 3; s++)
-                  acc += 0.5 * FiCov.m[w][pp] * FiCov.m[z][s] * G.m[s][j] *
-                         (Se[pp][z] * G.m[w][i] + Se[pp][i] * G.m[w][z]);
-          Sc[i][j] = acc;
-        }
-      // ---- ElemElsTangMatTens :1548-1585 (alpha, beta; entries formed on demand) ----
-      const double Je2 = Je * Je;
-      const double alpha = in.K * (Je2 - Je) - in.mu;
-      const double beta = 0.5 * in.K * Je * (2.0 * Je - 1.0);
-      auto CCe = [&](int i, int j, int k, int l) {
-        return (2.0 * beta) * Ch.m[i][j] * Ch.m[k][l] -
-               alpha * (Ch.m[i][k] * Ch.m[l][j] + Ch.m[i][l] * Ch.m[k][j]);
-      };
-      // ---- ElemTotTangMatTens :1603-1643, [2][2][2][2] only ----
-      double CC = 0.0;
-      {
-        const int i = 2, j = 2, k = 2, l = 2;
-        for (int gg = 0; gg < 3; gg++)
-          for (int h = 0; h < 3; h++)
-            for (int mm = 0; mm < 3; mm++)
-              for (int n = 0; n < 3; n++)
-                for (int pp = 0; pp < 3; pp++)
-                  for (int s = 0; s < 3; s++)
-                    for (int w = 0; w < 3; w++)
-                      for (int z = 0; z < 3; z++)
-                        CC += 0.25 * FiCov.m[w][pp] * FiCov.m[z][s] * FiCov.m[gg][h] *
-                              FiCov.m[n][mm] * G.m[s][j] * G.m[mm][l] *
-                              (G.m[gg][k] * (CCe(h, n, pp, z) * G.m[w][i] +
-                                             CCe(h, n, pp, i) * G.m[w][z]) +
-                               G.m[gg][n] * (CCe(h, k, pp, z) * G.m[w][i] +
-                                             CCe(h, k, pp, i) * G.m[w][z]));
-      }
-      // ---- ModElemC33 :1669-1707 ----
-      if (!(fabs(CC) > 1e-14)) {
-        skip_mask |= 1u << it;
-        out.status |= kStatCC3333;
-      } else if (!((skip_mask >> it) & 1u)) {
-        double DeltaC33 = -1.0 * Sc[2][2] / CC;
-        const double C33_old = CCov.m[2][2];
-        CCov.m[2][2] = C33_old + DeltaC33;
-        if (det3(CCov) <= 0.0) {
-          double lo = 0.0, hi = 1.0;
-          for (int bk = 0; bk < 16; bk++) {
-            const double mid = 0.5 * (lo + hi);
-            CCov.m[2][2] = C33_old + mid * DeltaC33;
-            if (det3(CCov) > 0.0) lo = mid; else hi = mid;
+    // ---- ElemTotStress :1506-1528, the four entries later steps read ----
+    auto tot = [&](int i, int j) {
+      const double half = opaque(0.5);
+      double acc = 0.0;
+#pragma unroll
+      for (int w = 0; w < 3; w++)
+#pragma unroll
+        for (int pp = 0; pp < 3; pp++)
+#pragma unroll
+          for (int z = 0; z < 3; z++)
+#pragma unroll
+            for (int s = 0; s < 3; s++)
+              acc += half * Fi.m[w][pp] * Fi.m[z][s] * G.m[s][j] *
+                     (Se.m[pp][z] * G.m[w][i] + Se.m[pp][i] * G.m[w][z]);
+      return acc;
+    };
+    S00 = tot(0, 0);
+    S11 = tot(1, 1);
+    S01 = tot(0, 1);
+    const double S22 = tot(2, 2);
+    // ---- ElemElsTangMatTens :1548-1585 (alpha, beta; entries formed on demand) ----
+    const double Je2 = Je * Je;
+    const double alpha = in.K * (Je2 - Je) - in.mu;
+    const double beta = 0.5 * in.K * Je * (2.0 * Je - 1.0);
+    const double b2 = 2.0 * beta;
+    // ---- ElemTotTangMatTens :1603-1643, [2][2][2][2]; loop order g,h,m,n,p,s,w,z ----
+    double CC = 0.0;
+#pragma unroll 1
+    for (int gg = 0; gg < 3; gg++) {
+      double Gr[3], Fgr[3];
+      row3(G, gg, Gr);       // G[g][*]
+      row3(Fi, gg, Fgr);     // Fi[g][*]
+      const double Gg2 = Gr[2];
+#pragma unroll 1
+      for (int h = 0; h < 3; h++) {
+        double Chr[3];
+        row3(Ch, h, Chr);    // Ch[h][*]
+        const double Figh = sel3(h, Fgr[0], Fgr[1], Fgr[2]);
+#pragma unroll 1
+        for (int mm = 0; mm < 3; mm++) {
+          const double Gm2 = sel3(mm, G.m[0][2], G.m[1][2], G.m[2][2]);
+          double Fmc[3];
+          col3(Fi, mm, Fmc);  // Fi[*][m]
+#pragma unroll 1
+          for (int n = 0; n < 3; n++) {
+            double Chc[3];
+            col3(Ch, n, Chc);  // Ch[*][n]
+            const double Finm = sel3(n, Fmc[0], Fmc[1], Fmc[2]);
+            const double Ggn = sel3(n, Gr[0], Gr[1], Gr[2]);
+            const double Chhn = sel3(n, Chr[0], Chr[1], Chr[2]);
+            const double quarter = opaque(0.25);
+#pragma unroll
+            for (int pp = 0; pp < 3; pp++) {
+              // CCe(h,n,p,2), CCe(h,2,p,2) (:1578-1581)
+              const double e_np2 = b2 * Chhn * Ch.m[pp][2] - alpha * (Chr[pp] * Chc[2] + Chr[2] * Chc[pp]);
+              const double e_2p2 = b2 * Chr[2] * Ch.m[pp][2] - alpha * (Chr[pp] * Ch.m[2][2] + Chr[2] * Ch.m[pp][2]);
+#pragma unroll
+              for (int s = 0; s < 3; s++)
+#pragma unroll
+                for (int w = 0; w < 3; w++)
+#pragma unroll
+                  for (int z = 0; z < 3; z++) {
+                    const double e_npz = b2 * Chhn * Ch.m[pp][z] - alpha * (Chr[pp] * Chc[z] + Chr[z] * Chc[pp]);
+                    const double e_2pz = b2 * Chr[2] * Ch.m[pp][z] - alpha * (Chr[pp] * Ch.m[z][2] + Chr[z] * Ch.m[pp][2]);
+                    CC += quarter * Fi.m[w][pp] * Fi.m[z][s] * Figh * Finm * G.m[s][2] * Gm2 *
+                          (Gg2 * (e_npz * G.m[w][2] + e_np2 * G.m[w][z]) +
+                           Ggn * (e_2pz * G.m[w][2] + e_2p2 * G.m[w][z]));
+                  }
+            }
           }
-          DeltaC33 *= lo;
-          CCov.m[2][2] = C33_old + DeltaC33;
-          out.status |= kStatClamp;
         }
       }
-      // ---- C_inv.Cont = INV(C.Cov) :2101 ----
-      inv(CCov, Ci);
     }
-    out.Sv[qp][0] = Sc[0][0];
-    out.Sv[qp][1] = Sc[1][1];
-    out.Sv[qp][2] = Sc[0][1];
-    out.c33[qp] = CCov.m[2][2];
+    // ---- ModElemC33 :1669-1707 ----
+    if (!(fabs(CC) > 1e-14)) {
+      skip_mask |= 1u << it;
+      out.status |= kStatCC3333;
+    } else if (!((skip_mask >> it) & 1u)) {
+      double DeltaC33 = -1.0 * S22 / CC;
+      const double C33_old = CCov.m[2][2];
+      CCov.m[2][2] = C33_old + DeltaC33;
+      if (det3(CCov) <= 0.0) {
+        double lo = 0.0, hi = 1.0;
+        for (int bk = 0; bk < 16; bk++) {
+          const double mid = 0.5 * (lo + hi);
+          CCov.m[2][2] = C33_old + mid * DeltaC33;
+          if (det3(CCov) > 0.0) lo = mid; else hi = mid;
+        }
+        DeltaC33 *= lo;
+        CCov.m[2][2] = C33_old + DeltaC33;
+        out.status |= kStatClamp;
+      }
+    }
+    // ---- C_inv.Cont = INV(C.Cov) :2101 ----
+    inv(CCov, Ci);
+  }
+  out.S00 = S00;
+  out.S11 = S11;
+  out.S01 = S01;
+  out.c33 = CCov.m[2][2];
+  return out;
+}
+
+BF_HD void integrate(const ElemIn& in, ElemOut& out) {
+  out.status = 0;
+  unsigned skip_mask = 0u;
+#pragma unroll 1
+  for (int qp = 0; qp < in.n_qp; qp++) {
+    const QpOut q = integrate_qp(in, qp, skip_mask);
+    out.Sv[qp][0] = q.S00;
+    out.Sv[qp][1] = q.S11;
+    out.Sv[qp][2] = q.S01;
+    out.c33[qp] = q.c33;
+    out.status |= q.status;
   }
 }
 

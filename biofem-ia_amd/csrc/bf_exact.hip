@@ -48,16 +48,18 @@ __global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E,
 // ElemCGDefTens + ElemC33Solve + ElemUpdFint of one element per lane, every operation in
 // the order of src/active_strain.c.  Writes the same slot-major element-force buffer as the
 // staged kernel; the node gather then sums in element order (active_strain.c:2238-2260).
+constexpr int kExactBlock = 128;
+
 __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& kp,
                                               const double* __restrict__ x,
                                               const double* __restrict__ y,
                                               const double* __restrict__ z, int e,
-                                              unsigned char extra_flags) {
+                                              unsigned char extra_flags,
+                                              exact::ElemIn& in, exact::ElemOut& o) {
   const int E = kp.E;
   const int meta = m.meta[e];
   const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
   auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
-  exact::ElemIn in;
   const double *na0, *na1, *nb0, *nb1, *nb2;
   if (kind == kRegular) {
     na0 = c_Na[0]; na1 = c_Na[1]; nb0 = c_Nab[0]; nb1 = c_Nab[1]; nb2 = c_Nab[2];
@@ -103,7 +105,6 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
   in.iters = kp.iters;
   in.theta = kp.theta;
   in.wq = kp.w;
-  exact::ElemOut o;
   exact::integrate(in, o);
   for (int q = 0; q < kp.n_qp; q++) m.c33[(size_t)q * E + e] = o.c33[q];
   // ---- ElemUpdFint :1799-1801 ----
@@ -148,24 +149,29 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
 }
 
 
-__global__ void __launch_bounds__(128) bf_element_kernel_exact(DevMesh m, KParams kp,
+__global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact(DevMesh m, KParams kp,
                                                                const double* __restrict__ x,
                                                                const double* __restrict__ y,
                                                                const double* __restrict__ z) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= kp.E) return;
-  exact_element(m, kp, x, y, z, e, 0);
+  exact::ElemIn in;
+  exact::ElemOut o;
+  exact_element(m, kp, x, y, z, e, 0, in, o);
 }
 
 // Persistent drain of the fallback list: every lane strides over [0, count); count is
 // final because the producing kernel completed earlier on the same stream.
-__global__ void __launch_bounds__(128) bf_element_kernel_exact_list(DevMesh m, KParams kp,
+__global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact_list(DevMesh m, KParams kp,
                                                                     const double* __restrict__ x,
                                                                     const double* __restrict__ y,
                                                                     const double* __restrict__ z) {
   const int count = m.status[4];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-    exact_element(m, kp, x, y, z, m.fallback[i], kStatFallback);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    exact::ElemIn in;
+    exact::ElemOut o;
+    exact_element(m, kp, x, y, z, m.fallback[i], kStatFallback, in, o);
+  }
 }
 
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes) {
@@ -184,15 +190,15 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
   if (kp.E <= 0) return hipSuccess;
   // one lane per listed element when the list fills the machine (2048 waves of 64)
   const int blocks = std::min((kp.E + 127) / 128, 1024);
-  hipLaunchKernelGGL(bf_element_kernel_exact_list, dim3(blocks), dim3(128), 0, s, m, kp, x, y, z);
+  hipLaunchKernelGGL(bf_element_kernel_exact_list, dim3(blocks), dim3(kExactBlock), 0, s, m, kp, x, y, z);
   return hipGetLastError();
 }
 
 hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
                                 const double* y, const double* z, hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
-  hipLaunchKernelGGL(bf_element_kernel_exact, dim3((kp.E + 127) / 128), dim3(128), 0, s, m, kp,
-                     x, y, z);
+  hipLaunchKernelGGL(bf_element_kernel_exact, dim3((kp.E + kExactBlock - 1) / kExactBlock),
+                     dim3(kExactBlock), 0, s, m, kp, x, y, z);
   return hipGetLastError();
 }
 

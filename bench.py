@@ -71,13 +71,15 @@ def run_gpu(fe, lib, L, prec, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     n = C.c_longlong()
-    te, tg = C.c_double(), C.c_double()
-    L.check("timing_read", lib.bf_timing_read(ctx, C.byref(n), C.byref(te), C.byref(tg)))
+    te, th, tg = C.c_double(), C.c_double(), C.c_double()
+    L.check("timing_read", lib.bf_timing_read_split(ctx, C.byref(n), C.byref(te), C.byref(th),
+                                                    C.byref(tg)))
     L.check("timing", lib.bf_timing_enable(ctx, 0))
     e, code, warn = fe.last_error()
     if code == L.BF_ELEM_NONFINITE:
         raise RuntimeError(f"non-finite element force at element {e}")
-    return wall, n.value, te.value / max(n.value, 1), tg.value / max(n.value, 1)
+    k = max(n.value, 1)
+    return wall, n.value, te.value / k, th.value / k, tg.value / k
 
 
 def main():
@@ -127,19 +129,21 @@ def main():
     for name, prec in modes:
         steps = args.steps if name != "fp64_ref_order" else max(3, args.steps // 10)
         log(f"timing {name}: warmup {args.warmup}, steps {steps}")
-        wall, n, t_elem, t_gather = run_gpu(fe, lib, L, prec, steps, args.warmup)
+        wall, n, t_elem, t_hand, t_gather = run_gpu(fe, lib, L, prec, steps, args.warmup)
         ms = 1e3 * wall / steps
         res[name] = dict(
             steps=steps,
             ms_per_step=ms,
             elem_qp_per_s=E * q * steps / wall,
             kernel_ms_element=t_elem,
+            kernel_ms_handover=t_hand,
             kernel_ms_gather=t_gather,
             timed_launches=n,
             gbps_element_kernel=elem_bytes / (t_elem * 1e-3) / 1e9,
-            gbps_path=path_bytes / ((t_elem + t_gather) * 1e-3) / 1e9,
+            gbps_path=path_bytes / ((t_elem + t_hand + t_gather) * 1e-3) / 1e9,
         )
-        log(f"{name}: {ms:.4f} ms/step, element kernel {t_elem:.4f} ms, gather {t_gather:.4f} ms")
+        log(f"{name}: {ms:.4f} ms/step, element kernel {t_elem:.4f} ms, hand-over "
+            f"{t_hand:.4f} ms, gather {t_gather:.4f} ms")
 
     cpu = None
     parity = {}

@@ -86,6 +86,7 @@ SIGNATURES = {
     "bf_last_timing": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "bf_timing_enable": (C.c_int, [_vp, C.c_int]),
     "bf_timing_read": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp, _dp]),
+    "bf_timing_read_split": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp, _dp, _dp]),
     "bf_device_buffers": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "bf_sync": (C.c_int, [_vp]),
     "bf_topo_area0": (C.c_int, [C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),

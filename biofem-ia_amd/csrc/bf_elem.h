@@ -38,7 +38,8 @@ constexpr int kMaxQp = 5;
 enum : int { kRegular = 0, kIrregular = 1, kCap = 2 };
 
 // per-element status bits (OR-ed into a flag word, first element kept by atomicMin)
-enum : int { kStatCC3333 = 1, kStatNonFinite = 2, kStatClamp = 4, kStatFallback = 8 };  // kStatClamp: the det(C) > 0
+enum : int { kStatCC3333 = 1, kStatNonFinite = 2, kStatClamp = 4, kStatFallback = 8,
+             kStatIllCond = 16, kStatRange = 32 };  // kStatRange: C33 left [kC33Lo, kC33Hi]  // kStatClamp: the det(C) > 0
 // bisection of ModElemC33 (active_strain.c:1696-1706) ran: the element is outside the regime
 // where the local C33 iteration is well conditioned
 
@@ -133,6 +134,64 @@ BF_HD M33<R> mul(const M33<R>& A, const M33<R>& B) {
     for (int j = 0; j < 3; j++) C.m[i][j] = A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j] + A.m[i][2] * B.m[2][j];
   return C;
 }
+// |A| |B| with element-wise absolute values
+template <class R>
+BF_HD M33<R> abs_mul(const M33<R>& A, const M33<R>& B) {
+  M33<R> C;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      C.m[i][j] = fabs(A.m[i][0]) * fabs(B.m[0][j]) + fabs(A.m[i][1]) * fabs(B.m[1][j]) +
+                  fabs(A.m[i][2]) * fabs(B.m[2][j]);
+  return C;
+}
+
+// Rounding noise of one ModElemC33 update as the reference computes it.  The reference sums
+// CC^{2222} over 6561 terms (ElemTotTangMatTens :1603-1643) and S^{22} over 81
+// (ElemTotStress :1506-1528); recursive summation of n products of k factors errs by at most
+// (n + k) eps times the sum of the terms' magnitudes.  Those magnitude sums are the staged
+// closed forms evaluated on element-wise absolute values (Ha = |Fa_inv.Cov| |gm0.Cont|,
+// H2a = Ha Ha, |Ce_inv.Cont|, |Se|, |alpha|, |beta|), an upper bound because the staged
+// derivation only regroups terms.  Returns the bound on the resulting change of C33,
+// (|dC33| rel.err(CC) + err(S22) / |CC|).  Elements where it exceeds the fast paths'
+// budget are ill-conditioned: the reference's own C33 is noise-dominated there.
+template <class R>
+BF_HD R c33_update_noise(const M33<R>& Ha, const M33<R>& H2a, const M33<R>& Ch,
+                         const M33<R>& Se, R alpha, R beta, R S22, R CC) {
+  const R eps = R(1.1102230246251565e-16);  // 2^-53, the reference's FP64
+  const R hx = Ha.m[0][2], hy = Ha.m[1][2], hz = Ha.m[2][2];
+  const R Px = H2a.m[0][2], Py = H2a.m[1][2], Pz = H2a.m[2][2];
+  M33<R> Ca;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) Ca.m[i][j] = fabs(Ch.m[i][j]);
+  const R rx = Ca.m[0][2], ry = Ca.m[1][2], rz = Ca.m[2][2];
+  const R Chx = Ca.m[0][0] * hx + Ca.m[0][1] * hy + Ca.m[0][2] * hz;
+  const R Chy = Ca.m[1][0] * hx + Ca.m[1][1] * hy + Ca.m[1][2] * hz;
+  const R Chz = Ca.m[2][0] * hx + Ca.m[2][1] * hy + Ca.m[2][2] * hz;
+  const R b = hx * Chx + hy * Chy + hz * Chz;
+  const R BhP = Px * Chx + Py * Chy + Pz * Chz;
+  const R BPP = Px * (Ca.m[0][0] * Px + Ca.m[0][1] * Py + Ca.m[0][2] * Pz) +
+                Py * (Ca.m[1][0] * Px + Ca.m[1][1] * Py + Ca.m[1][2] * Pz) +
+                Pz * (Ca.m[2][0] * Px + Ca.m[2][1] * Py + Ca.m[2][2] * Pz);
+  const R Pr = Px * rx + Py * ry + Pz * rz;
+  const R hr = hx * rx + hy * ry + hz * rz;
+  const R aa = fabs(alpha), ab = fabs(beta);
+  const R CCabs = R(0.25) * (R(2) * (ab + aa) * b * b + R(4) * ab * b * Pr + R(4) * aa * BhP * hr +
+                             R(2) * ab * Pr * Pr + aa * BPP * Ca.m[2][2] + aa * Pr * Pr);
+  // S^22 = 1/2 (H^T Se H + Se H2)_22 on magnitudes
+  R SHz[3];
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+    SHz[p] = fabs(Se.m[p][0]) * hx + fabs(Se.m[p][1]) * hy + fabs(Se.m[p][2]) * hz;
+  const R S22abs = R(0.5) * (hx * SHz[0] + hy * SHz[1] + hz * SHz[2] + fabs(Se.m[0][2]) * Px +
+                             fabs(Se.m[1][2]) * Py + fabs(Se.m[2][2]) * Pz);
+  const R aCC = fabs(CC);
+  return (fabs(S22) * (R(6569) * eps * CCabs) / aCC + R(89) * eps * S22abs) / aCC;
+}
+
 // Gram matrix of three vectors g_i . g_j (ComputeMetricTensor, active_strain.c:769-779)
 template <class R>
 BF_HD Sym<R> gram(V3<R> g0, V3<R> g1, V3<R> g2) {
@@ -227,21 +286,37 @@ struct Resultants {
   double N0, N1, N2, M0, M1, M2;
   double c33[kMaxQp];
   int status;
+  float noise;  // largest c33_update_noise of the element's updates
 };
 
-// Staged constitutive chain for one element (ElemUpdateG, ElemActDefGrad, ElemCGDefTens,
-// ElemC33Solve; active_strain.c:896-982, 1196-1378, 1381-1721, 2048-2138).
-//   cur: current a1,a2,a3,a3_1,a3_2   ref: reference ones   R: per-qp arithmetic type.
+// Bound on the reference's rounding-driven change of C33 in one update above which the
+// staged paths hand the element to the reference-order kernel.
+constexpr double kC33NoiseMax = 1e-8;
+
+// Result of one through-thickness point of a staged chain: the total stress entries
+// ElemUpdFint reads, the final C33 and the status bits.
 template <class R>
-BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a2c,
-                                      const V3<double>& a3c, const V3<double>& a31c,
-                                      const V3<double>& a32c, const RefGeom& ref,
-                                      const Activation& act, double h0, double mu_d,
-                                      double K_d, int n_qp, int iters, const double* theta,
-                                      const double* wq) {
-  Resultants<R> out;
-  out.N0 = out.N1 = out.N2 = out.M0 = out.M1 = out.M2 = 0.0;
+struct StagedQp {
+  R S00, S11, S01;
+  double c33;
+  int status;
+  float noise;
+};
+
+// Staged constitutive chain for one through-thickness point (ElemUpdateG, ElemActDefGrad,
+// ElemCGDefTens, ElemC33Solve; active_strain.c:896-982, 1196-1378, 1381-1721, 2048-2138).
+//   cur: current a1,a2,a3,a3_1,a3_2   ref: reference ones   R: per-qp arithmetic type.
+// skip(fail, it) decides whether the CC3333 guard of iteration it keeps this qp's C33 (see
+// integrate_element for the reference's cross-qp rule).
+template <class R, class Skip>
+BF_HD StagedQp<R> staged_qp(const V3<double>& a1c, const V3<double>& a2c,
+                            const V3<double>& a3c, const V3<double>& a31c,
+                            const V3<double>& a32c, const RefGeom& ref, const Activation& act,
+                            double h0, double mu_d, double K_d, int iters, double theta_q,
+                            Skip&& skip) {
+  StagedQp<R> out;
   out.status = 0;
+  out.noise = 0.0f;
   const R mu = (R)mu_d, K = (R)K_d;
   const double hhalf = 0.5 * h0;
   // activation in the Cartesian frame, its inverse:  Fa^-1 = diag(1/d) + kap f f^T
@@ -252,13 +327,8 @@ BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a
   const R kap = rank1 ? gam / (R(1) - gam * ff) : R(0);
   const R d0 = (R)act.d0, d1 = (R)act.d1, d2 = (R)act.d2;
   const bool diag_id = (act.d0 == 1.0 && act.d1 == 1.0 && act.d2 == 1.0);
-  // bit `it` set once some qp failed the CC3333 guard at iteration `it`: the reference's
-  // ModElemC33 returns at the first failing qp (:1679), so that qp and every later qp keep
-  // their C33 in that iteration (iteration-outer loop order, :2058/:1660).
-  unsigned skip_mask = 0u;
-
-  for (int qp = 0; qp < n_qp; qp++) {
-    const double t3d = hhalf * theta[qp];
+  {
+    const double t3d = hhalf * theta_q;
     const R t3 = (R)t3d;
     // ---- current metric (ElemUpdateG :940-957) ----
     const V3<R> g0 = add(cvt<R>(a1c), scale(t3, cvt<R>(a31c)));
@@ -308,6 +378,8 @@ BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a
     // H = Fa_inv.Cov G  (mixed), H2 = H H
     const M33<R> H = mul(full(FiCov), Gm);
     const M33<R> H2 = mul(H, H);
+    const M33<R> Ha = abs_mul(full(FiCov), Gm);
+    const M33<R> H2a = abs_mul(Ha, Ha);
     // ---- ElemCGDefTens (:1341-1363): C.Cov in-plane block from gm, C33 = 1;
     //      C_inv.Cont in-plane block from gm.Cont, 1 at [2][2]
     const R c00 = gm.a00, c01 = gm.a01, c11 = gm.a11;
@@ -375,11 +447,15 @@ BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a
       const R CC = R(0.25) * (R(2) * (beta - alpha) * b * b + R(4) * beta * b * Pr -
                               R(4) * alpha * BhP * hr + R(2) * beta * Pr * Pr -
                               alpha * BPP * Ch.a22 - alpha * Pr * Pr);
+      {
+        const R nz = c33_update_noise(Ha, H2a, full(Ch), SeF, alpha, beta, S22, CC);
+        if (!(nz <= R(kC33NoiseMax))) out.status |= kStatIllCond;
+        out.noise = fmaxf(out.noise, (float)nz);
+      }
       // ---- ModElemC33 (:1669-1707) ----
-      if (!(fabs(CC) > R(1e-14))) {
-        skip_mask |= 1u << it;  // error ignored by the caller (:2086)
-        out.status |= kStatCC3333;
-      } else if (!((skip_mask >> it) & 1u)) {
+      const bool fail = !(fabs(CC) > R(1e-14));  // error ignored by the caller (:2086)
+      if (fail) out.status |= kStatCC3333;
+      if (!skip(fail, it)) {
         R dC = -S22 / CC;
         const R cold = c33;
         R trial = cold + dC;
@@ -407,16 +483,59 @@ BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a
         ci22 = c * (c11 * c00 - c01 * c01);
       }
     }
-    out.c33[qp] = (double)c33;
-    // ---- through-thickness resultants for ElemUpdFint (:1859-1914) ----
-    const double w = wq[qp];
-    const double wt = w * theta[qp];
-    out.N0 += w * (double)S00;
-    out.N1 += w * (double)S11;
-    out.N2 += w * (double)S01;
-    out.M0 += wt * (double)S00;
-    out.M1 += wt * (double)S11;
-    out.M2 += wt * (double)S01;
+    out.c33 = (double)c33;
+    out.S00 = S00;
+    out.S11 = S11;
+    out.S01 = S01;
+  }
+  return out;
+}
+
+// Guard rule of ModElemC33 across qps: the reference returns at the first qp that fails the
+// CC3333 guard (:1679), so that qp and every later qp keep their C33 in that iteration
+// (iteration-outer loop order, :2058/:1660).  Bit `it` of the mask is set once some qp
+// failed in iteration `it`.
+struct SeqGuard {
+  unsigned mask = 0u;
+  BF_HD bool operator()(bool fail, int it) {
+    if (fail) mask |= 1u << it;
+    return ((mask >> it) & 1u) != 0u;
+  }
+};
+
+// Through-thickness resultants for ElemUpdFint (:1859-1914), qp contributions added in qp
+// order: N += w_q S_q, M += w_q theta_q S_q.
+template <class R, class Res>
+BF_HD void add_resultants(Res& out, const StagedQp<R>& s, double w, double theta_q) {
+  const double wt = w * theta_q;
+  out.N0 += w * (double)s.S00;
+  out.N1 += w * (double)s.S11;
+  out.N2 += w * (double)s.S01;
+  out.M0 += wt * (double)s.S00;
+  out.M1 += wt * (double)s.S11;
+  out.M2 += wt * (double)s.S01;
+}
+
+// Staged chain of a whole element: every qp in order, resultants summed in qp order.
+template <class R>
+BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a2c,
+                                      const V3<double>& a3c, const V3<double>& a31c,
+                                      const V3<double>& a32c, const RefGeom& ref,
+                                      const Activation& act, double h0, double mu_d,
+                                      double K_d, int n_qp, int iters, const double* theta,
+                                      const double* wq) {
+  Resultants<R> out;
+  out.N0 = out.N1 = out.N2 = out.M0 = out.M1 = out.M2 = 0.0;
+  out.status = 0;
+  out.noise = 0.0f;
+  SeqGuard guard;
+  for (int qp = 0; qp < n_qp; qp++) {
+    const StagedQp<R> s =
+        staged_qp<R>(a1c, a2c, a3c, a31c, a32c, ref, act, h0, mu_d, K_d, iters, theta[qp], guard);
+    out.c33[qp] = s.c33;
+    out.status |= s.status;
+    out.noise = fmaxf(out.noise, s.noise);
+    add_resultants(out, s, wq[qp], theta[qp]);
   }
   return out;
 }
@@ -434,16 +553,16 @@ BF_HD Resultants<R> integrate_element(const V3<double>& a1c, const V3<double>& a
 //   Se            = -mu (Ce^-1 - G) + K Je (Je - 1) Ce^-1
 // The metrics, their inverses, C33 and its Newton update stay in FP64; stress, tangent and
 // the 3x3 products run in FP32.  Algebraically identical to integrate_element.
-BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<double>& a2c,
-                                                const V3<double>& a3c, const V3<double>& a31c,
-                                                const V3<double>& a32c, const RefGeom& ref,
-                                                const Activation& act, double h0, double mu_d,
-                                                double K_d, int n_qp, int iters,
-                                                const double* theta, const double* wq) {
+template <class Skip>
+BF_HD StagedQp<float> staged_qp_mixed(const V3<double>& a1c, const V3<double>& a2c,
+                                      const V3<double>& a3c, const V3<double>& a31c,
+                                      const V3<double>& a32c, const RefGeom& ref,
+                                      const Activation& act, double h0, double mu_d,
+                                      double K_d, int iters, double theta_q, Skip&& skip) {
   typedef float R;
-  Resultants<float> out;
-  out.N0 = out.N1 = out.N2 = out.M0 = out.M1 = out.M2 = 0.0;
+  StagedQp<float> out;
   out.status = 0;
+  out.noise = 0.0f;
   const R mu = (R)mu_d, K = (R)K_d;
   const double hhalf = 0.5 * h0;
   const bool rank1 = act.gam != 0.0;
@@ -453,9 +572,8 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
   const double kap_d = rank1 ? gam_d / (1.0 - gam_d * ff_d) : 0.0;
   const R qcoef = (R)(-gam_d * (2.0 - gam_d * ff_d));  // T G T^T - G = qcoef psi psi^T
   const double dd[3] = {act.d0, act.d1, act.d2};
-  unsigned skip_mask = 0u;
-  for (int qp = 0; qp < n_qp; qp++) {
-    const double t3 = hhalf * theta[qp];
+  {
+    const double t3 = hhalf * theta_q;
     // ---- metrics in FP64 ----
     const V3<double> g0 = add(a1c, scale(t3, a31c));
     const V3<double> g1 = add(a2c, scale(t3, a32c));
@@ -474,7 +592,7 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
     const M33<R> g0m = full(Sym<R>{(R)gm0.a00, (R)gm0.a11, (R)gm0.a22, (R)gm0.a01, (R)gm0.a02, (R)gm0.a12});
     // ---- activation: T = Fa.Cont g0, H = Fa_inv.Cov G, Q = T G T^T - G ----
     const double phid[3] = {dot(act.f, h0v), dot(act.f, h1v), dot(act.f, h2v)};
-    M33<R> T, H, Q;
+    M33<R> T, H, Q, Ha;
     {
       // psi = G phi (FP64 then rounded)
       double psid[3];
@@ -508,13 +626,15 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
       // identity part must be exact for the Q split to hold)
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) {
-          double t = 0.0, hh = 0.0;
+          double t = 0.0, hh = 0.0, ha = 0.0;
           for (int k = 0; k < 3; k++) {
             t += Gf[i][k] * FaC[k][j];
             hh += FiC[i][k] * Gf[k][j];
+            ha += fabs(FiC[i][k]) * fabs(Gf[k][j]);
           }
           T.m[i][j] = (R)t;
           H.m[i][j] = (R)hh;
+          Ha.m[i][j] = (R)ha;
         }
       (void)g0f;
       if (diag_id) {
@@ -533,6 +653,7 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
       }
     }
     const M33<R> H2 = mul(H, H);
+    const M33<R> H2a = abs_mul(Ha, Ha);
     // ---- C33 iteration ----
     const double c00 = gm.a00, c01 = gm.a01, c11 = gm.a11;
     double c33 = 1.0;
@@ -605,11 +726,15 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
       const R CC = R(0.25) * (R(2) * (beta - alpha) * b * b + R(4) * beta * b * Pr -
                               R(4) * alpha * BhP * hr + R(2) * beta * Pr * Pr -
                               alpha * BPP * Ch.m[2][2] - alpha * Pr * Pr);
+      {
+        const R nz = c33_update_noise(Ha, H2a, Ch, Se, alpha, beta, S22, CC);
+        if (!(nz <= R(kC33NoiseMax))) out.status |= kStatIllCond;
+        out.noise = fmaxf(out.noise, (float)nz);
+      }
       // ModElemC33 in FP64
-      if (!(fabs(CC) > R(1e-14))) {
-        skip_mask |= 1u << it;
-        out.status |= kStatCC3333;
-      } else if (!((skip_mask >> it) & 1u)) {
+      const bool fail = !(fabs(CC) > R(1e-14));
+      if (fail) out.status |= kStatCC3333;
+      if (!skip(fail, it)) {
         double dCd = -(double)S22 / (double)CC;
         const double cold = c33;
         double trial = cold + dCd;
@@ -645,30 +770,50 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
             E.m[i][j] = (R)(-(CD[i][0] * Gf[0][j] + CD[i][1] * Gf[1][j] + CD[i][2] * Gf[2][j]));
       }
     }
-    out.c33[qp] = c33;
-    const double w = wq[qp];
-    const double wt = w * theta[qp];
-    out.N0 += w * (double)S00;
-    out.N1 += w * (double)S11;
-    out.N2 += w * (double)S01;
-    out.M0 += wt * (double)S00;
-    out.M1 += wt * (double)S11;
-    out.M2 += wt * (double)S01;
+    out.c33 = c33;
+    out.S00 = S00;
+    out.S11 = S11;
+    out.S01 = S01;
+  }
+  return out;
+}
+
+BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<double>& a2c,
+                                                const V3<double>& a3c, const V3<double>& a31c,
+                                                const V3<double>& a32c, const RefGeom& ref,
+                                                const Activation& act, double h0, double mu_d,
+                                                double K_d, int n_qp, int iters,
+                                                const double* theta, const double* wq) {
+  Resultants<float> out;
+  out.N0 = out.N1 = out.N2 = out.M0 = out.M1 = out.M2 = 0.0;
+  out.status = 0;
+  out.noise = 0.0f;
+  SeqGuard guard;
+  for (int qp = 0; qp < n_qp; qp++) {
+    const StagedQp<float> s = staged_qp_mixed(a1c, a2c, a3c, a31c, a32c, ref, act, h0, mu_d, K_d,
+                                              iters, theta[qp], guard);
+    out.c33[qp] = s.c33;
+    out.status |= s.status;
+    out.noise = fmaxf(out.noise, s.noise);
+    add_resultants(out, s, wq[qp], theta[qp]);
   }
   return out;
 }
 
 // Elements whose staged result must be replaced by the reference-order one: the local C33
 // iteration left the range where the reference's own rounding stays small (a clamp of
-// ModElemC33's bisection, C33 outside [1/2, 2] at any qp, or a non-finite value).  There
-// the 3 half-Newton steps of the reference amplify any rounding difference by orders of
-// magnitude, so the fast paths hand such elements to the plain kernel.
+// ModElemC33's bisection, C33 outside [1/2, 2] at any qp, an update whose reference
+// rounding bound c33_update_noise exceeds kC33NoiseMax, the CC3333 guard, or a non-finite
+// value).  There the 3 half-Newton steps of the reference amplify any rounding difference
+// by orders of magnitude, so the fast paths hand such elements to the plain kernel.
 constexpr double kC33Lo = 0.5, kC33Hi = 2.0;
+BF_HD int c33_range_bit(double c33) { return (c33 >= kC33Lo && c33 <= kC33Hi) ? 0 : kStatRange; }
+BF_HD bool needs_exact_bits(int status, bool finite) {
+  return !finite || (status & (kStatClamp | kStatCC3333 | kStatIllCond | kStatRange)) != 0;
+}
 BF_HD bool needs_exact(const double* c33, int n_qp, int status, bool finite) {
-  if (!finite || (status & (kStatClamp | kStatCC3333))) return true;
-  for (int q = 0; q < n_qp; q++)
-    if (!(c33[q] >= kC33Lo && c33[q] <= kC33Hi)) return true;
-  return false;
+  for (int q = 0; q < n_qp; q++) status |= c33_range_bit(c33[q]);
+  return needs_exact_bits(status, finite);
 }
 
 // Nodal force of one patch node a (ElemUpdFint :1879-1913 / :1934-1983), summed over qps:

@@ -116,14 +116,17 @@ BF_HD double opaque(double v) {
 }
 
 // One through-thickness point: ElemUpdateG, ElemActDefGrad, ElemCGDefTens and the C33
-// loop for qp `qp`.  `skip_mask` carries the CC3333-guard state across qps (bit it set:
-// a qp at or before this one failed the guard in iteration it).
+// loop for qp `qp`.  ModElemC33 returns at the first qp whose |CC3333| fails the guard
+// (:1674-1681), so in iteration `it` this qp keeps its C33 if it or any earlier qp failed:
+// skip(fail, it) receives this qp's guard result and answers that question (sequentially
+// from a bit mask, or across the lanes that hold the element's other qps).
 struct QpOut {
   double S00, S11, S01, c33;
   int status;
 };
 
-BF_HD QpOut integrate_qp(const ElemIn& in, int qp, unsigned& skip_mask) {
+template <class Skip>
+BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
   QpOut out;
   out.status = 0;
   // Fa_cart and its inverse (:1227-1256, :1291): rebuilt per qp (same values every time)
@@ -300,10 +303,9 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, unsigned& skip_mask) {
       }
     }
     // ---- ModElemC33 :1669-1707 ----
-    if (!(fabs(CC) > 1e-14)) {
-      skip_mask |= 1u << it;
-      out.status |= kStatCC3333;
-    } else if (!((skip_mask >> it) & 1u)) {
+    const bool fail = !(fabs(CC) > 1e-14);
+    if (fail) out.status |= kStatCC3333;
+    if (!skip(fail, it)) {
       double DeltaC33 = -1.0 * S22 / CC;
       const double C33_old = CCov.m[2][2];
       CCov.m[2][2] = C33_old + DeltaC33;
@@ -332,9 +334,13 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, unsigned& skip_mask) {
 BF_HD void integrate(const ElemIn& in, ElemOut& out) {
   out.status = 0;
   unsigned skip_mask = 0u;
+  auto skip = [&](bool fail, int it) {
+    if (fail) skip_mask |= 1u << it;
+    return ((skip_mask >> it) & 1u) != 0u;
+  };
 #pragma unroll 1
   for (int qp = 0; qp < in.n_qp; qp++) {
-    const QpOut q = integrate_qp(in, qp, skip_mask);
+    const QpOut q = integrate_qp(in, qp, skip);
     out.Sv[qp][0] = q.S00;
     out.Sv[qp][1] = q.S11;
     out.Sv[qp][2] = q.S01;
@@ -347,7 +353,7 @@ BF_HD void integrate(const ElemIn& in, ElemOut& out) {
 // accumulates over qp in the reference order.  Na0..Nb2 are the node's table entries.
 BF_HD V3<double> node_force_exact(double Na0, double Na1, double Nb0, double Nb1, double Nb2,
                                   const MidGeom& G, double pref0, double half_h0,
-                                  const ElemOut& o, const double* theta, const double* wq,
+                                  const double (*Sq)[3], const double* theta, const double* wq,
                                   int n_qp, bool irregular) {
   double F[3] = {0.0, 0.0, 0.0};
   const double nn[3] = {G.a3.x, G.a3.y, G.a3.z};
@@ -357,7 +363,7 @@ BF_HD V3<double> node_force_exact(double Na0, double Na1, double Nb0, double Nb1
   const double b1 = dot(ng, G.Aaa), b2 = dot(ng, G.Abb), b3 = dot(ng, G.Aab);
   for (int qp = 0; qp < n_qp; qp++) {
     const double w_qp = wq[qp], xi_qp = theta[qp];
-    const double* Sv = o.Sv[qp];
+    const double* Sv = Sq[qp];
     for (int c = 0; c < 3; c++) {
       const double Bm0 = Na0 * a1[c];
       const double Bm1 = Na1 * a2[c];
@@ -390,13 +396,13 @@ BF_HD V3<double> node_force_exact(double Na0, double Na1, double Nb0, double Nb1
 
 // LV apex-cap CST membrane force (:1825-1850) for corner a with shape gradient (Na0, Na1).
 BF_HD V3<double> cap_force_exact(double Na0, double Na1, const MidGeom& G, double pref0,
-                                 const ElemOut& o, const double* wq, int n_qp) {
+                                 const double (*Sq)[3], const double* wq, int n_qp) {
   double F[3] = {0.0, 0.0, 0.0};
   const double a1[3] = {G.a1.x, G.a1.y, G.a1.z};
   const double a2[3] = {G.a2.x, G.a2.y, G.a2.z};
   for (int qp = 0; qp < n_qp; qp++) {
     const double w_qp = wq[qp];
-    const double* Sv = o.Sv[qp];
+    const double* Sv = Sq[qp];
     for (int c = 0; c < 3; c++) {
       const double Bm0 = Na0 * a1[c];
       const double Bm1 = Na1 * a2[c];

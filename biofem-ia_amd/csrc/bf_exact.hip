@@ -45,21 +45,37 @@ __global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E,
 
 
 // Plain FP64 element kernel: ElemUpdateGeomSubdiv + ElemUpdateG + ElemActDefGrad +
-// ElemCGDefTens + ElemC33Solve + ElemUpdFint of one element per lane, every operation in
-// the order of src/active_strain.c.  Writes the same slot-major element-force buffer as the
-// staged kernel; the node gather then sums in element order (active_strain.c:2238-2260).
+// ElemCGDefTens + ElemC33Solve + ElemUpdFint, every operation in the order of
+// src/active_strain.c.  An element occupies a group of P adjacent lanes of one wave (P = the
+// largest n_qp of the launch; floor(64 / P) groups per wave), lane q of the group running
+// through-thickness point q: each lane forms the element geometry, runs its qp's C33 loop,
+// then the group exchanges the qp stresses by cross-lane reads and each lane writes the
+// forces of every P-th patch node, summing over qp in the reference's order.  The
+// CC3333 guard, which in the reference stops the qp loop of an iteration at the first
+// failing qp, becomes a wave ballot per iteration.  Writes the same slot-major element-force
+// buffer as the staged kernels; the node gather then sums in element order
+// (active_strain.c:2238-2260).
 constexpr int kExactBlock = 128;
+
+struct QpGroup {
+  int e;      // element of this lane's group, -1 when the lane has no work
+  int q;      // through-thickness point of this lane
+  int base;   // wave lane of the group's q = 0
+};
 
 __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& kp,
                                               const double* __restrict__ x,
                                               const double* __restrict__ y,
-                                              const double* __restrict__ z, int e,
-                                              unsigned char extra_flags,
-                                              exact::ElemIn& in, exact::ElemOut& o) {
+                                              const double* __restrict__ z, const QpGroup& grp,
+                                              unsigned char extra_flags) {
   const int E = kp.E;
+  const int e = grp.e, q = grp.q;
+  const int n_qp = kp.n_qp;
+  if (q >= n_qp) return;  // lanes beyond this element's qps (mixed n_qp in one launch)
   const int meta = m.meta[e];
   const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
   auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
+  exact::ElemIn in;
   const double *na0, *na1, *nb0, *nb1, *nb2;
   if (kind == kRegular) {
     na0 = c_Na[0]; na1 = c_Na[1]; nb0 = c_Nab[0]; nb1 = c_Nab[1]; nb2 = c_Nab[2];
@@ -101,29 +117,47 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
   in.h0 = kp.h0;
   in.mu = kp.mu;
   in.K = kp.K;
-  in.n_qp = kp.n_qp;
+  in.n_qp = n_qp;
   in.iters = kp.iters;
   in.theta = kp.theta;
   in.wq = kp.w;
-  exact::integrate(in, o);
-  for (int q = 0; q < kp.n_qp; q++) m.c33[(size_t)q * E + e] = o.c33[q];
-  // ---- ElemUpdFint :1799-1801 ----
+  // ---- this lane's qp; the guard of iteration it skips qp q if any qp <= q failed ----
+  const unsigned long long upto = (2ull << q) - 1ull;
+  const exact::QpOut r = exact::integrate_qp(in, q, [&](bool fail, int) {
+    return ((__ballot(fail) >> grp.base) & upto) != 0ull;
+  });
+  m.c33[(size_t)q * E + e] = r.c33;
+  // ---- the group's qp stresses, in qp order ----
+  double Sv[kMaxQp][3];
+  int status = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxQp; j++) {
+    if (j < n_qp) {
+      Sv[j][0] = __shfl(r.S00, grp.base + j);
+      Sv[j][1] = __shfl(r.S11, grp.base + j);
+      Sv[j][2] = __shfl(r.S01, grp.base + j);
+      status |= __shfl(r.status, grp.base + j);
+    } else {
+      Sv[j][0] = Sv[j][1] = Sv[j][2] = 0.0;
+    }
+  }
+  // ---- ElemUpdFint :1799-1801, nodes q, q + n_qp, ... of the patch ----
   const double A0 = m.dA0[e];
   const double half_h0 = 0.5 * kp.h0;
   const double pref0 = 0.5 * kp.h0 * A0;
   double* Fe = m.Fe;
   bool finite = true;
   if (kind != kCap) {
-    for (int a = 0; a < nen; a++) {
+    for (int a = q; a < nen; a += n_qp) {
       const V3<double> f = exact::node_force_exact(na0[a], na1[a], nb0[a], nb1[a], nb2[a], in.cur,
-                                                   pref0, half_h0, o, kp.theta, kp.w, kp.n_qp,
+                                                   pref0, half_h0, Sv, kp.theta, kp.w, n_qp,
                                                    kind == kIrregular);
       finite = finite && isfinite(f.x) && isfinite(f.y) && isfinite(f.z);
       Fe[(size_t)(3 * a + 0) * E + e] = f.x;
       Fe[(size_t)(3 * a + 1) * E + e] = f.y;
       Fe[(size_t)(3 * a + 2) * E + e] = f.z;
     }
-  } else {
+  } else if (q == 0) {
     for (int a = 0; a < nen; a++) {
       Fe[(size_t)(3 * a + 0) * E + e] = 0.0;
       Fe[(size_t)(3 * a + 1) * E + e] = 0.0;
@@ -133,44 +167,58 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
     for (int a = 0; a < 3; a++) {
       const int s = m.corner[(size_t)(3 + a) * E + e];
       if (s < 0) continue;
-      const V3<double> f = exact::cap_force_exact(dN[a][0], dN[a][1], in.cur, pref0, o, kp.w, kp.n_qp);
+      const V3<double> f = exact::cap_force_exact(dN[a][0], dN[a][1], in.cur, pref0, Sv, kp.w, n_qp);
       finite = finite && isfinite(f.x) && isfinite(f.y) && isfinite(f.z);
       Fe[(size_t)(3 * s + 0) * E + e] = f.x;
       Fe[(size_t)(3 * s + 1) * E + e] = f.y;
       Fe[(size_t)(3 * s + 2) * E + e] = f.z;
     }
   }
-  if (!finite) atomicMin(&m.status[0], e);
-  m.flags[e] = (unsigned char)(o.status | (finite ? 0 : kStatNonFinite) | extra_flags);
-  if (o.status & kStatCC3333) {
+  // ---- element status from the group (non-finite from any lane's nodes) ----
+  int bad = finite ? 0 : 1;
+#pragma unroll
+  for (int j = 0; j < kMaxQp; j++)
+    if (j < n_qp) bad |= __shfl(bad, grp.base + j);
+  if (q != 0) return;
+  if (bad) atomicMin(&m.status[0], e);
+  m.flags[e] = (unsigned char)(status | (bad ? kStatNonFinite : 0) | extra_flags);
+  if (status & kStatCC3333) {
     atomicMin(&m.status[2], e);
     atomicAdd(&m.status[3], 1);
   }
 }
 
-
 __global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact(DevMesh m, KParams kp,
                                                                const double* __restrict__ x,
                                                                const double* __restrict__ y,
                                                                const double* __restrict__ z) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= kp.E) return;
-  exact::ElemIn in;
-  exact::ElemOut o;
-  exact_element(m, kp, x, y, z, e, 0, in, o);
+  const int P = kp.n_qp, per_wave = 64 / P;
+  QpGroup g;
+  int slot;
+  lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
+  const int wave = blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6);
+  g.e = wave * per_wave + slot;
+  if (slot >= per_wave || g.e >= kp.E) return;
+  exact_element(m, kp, x, y, z, g, 0);
 }
 
-// Persistent drain of the fallback list: every lane strides over [0, count); count is
-// final because the producing kernel completed earlier on the same stream.
+// Persistent drain of the hand-over list: groups stride over [0, count); count is final
+// because the producing kernel completed earlier on the same stream.
 __global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact_list(DevMesh m, KParams kp,
                                                                     const double* __restrict__ x,
                                                                     const double* __restrict__ y,
                                                                     const double* __restrict__ z) {
   const int count = m.status[4];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    exact::ElemIn in;
-    exact::ElemOut o;
-    exact_element(m, kp, x, y, z, m.fallback[i], kStatFallback, in, o);
+  const int P = kp.n_qp, per_wave = 64 / P;
+  QpGroup g;
+  int slot;
+  lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
+  if (slot >= per_wave) return;
+  const int waves = gridDim.x * (kExactBlock / 64);
+  for (int i = (blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6)) * per_wave + slot; i < count;
+       i += waves * per_wave) {
+    g.e = m.fallback[i];
+    exact_element(m, kp, x, y, z, g, kStatFallback);
   }
 }
 
@@ -188,8 +236,9 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
-  // one lane per listed element when the list fills the machine (2048 waves of 64)
-  const int blocks = std::min((kp.E + 127) / 128, 1024);
+  // one lane group per listed element when the list fills the machine
+  const int per_block = (kExactBlock / 64) * (64 / kp.n_qp);
+  const int blocks = std::min((kp.E + per_block - 1) / per_block, 4096);
   hipLaunchKernelGGL(bf_element_kernel_exact_list, dim3(blocks), dim3(kExactBlock), 0, s, m, kp, x, y, z);
   return hipGetLastError();
 }
@@ -197,7 +246,8 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
 hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
                                 const double* y, const double* z, hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
-  hipLaunchKernelGGL(bf_element_kernel_exact, dim3((kp.E + kExactBlock - 1) / kExactBlock),
+  const int per_block = (kExactBlock / 64) * (64 / kp.n_qp);
+  hipLaunchKernelGGL(bf_element_kernel_exact, dim3((kp.E + per_block - 1) / per_block),
                      dim3(kExactBlock), 0, s, m, kp, x, y, z);
   return hipGetLastError();
 }

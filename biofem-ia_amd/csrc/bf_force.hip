@@ -27,14 +27,21 @@
 using namespace bf;
 
 // ------------------------------------------------------------------ kernels ------------
-template <class R>
-__global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
-                                                         const double* __restrict__ x,
-                                                         const double* __restrict__ y,
-                                                         const double* __restrict__ z) {
+// Staged element body for lane q of the P-lane group that holds element e (P >= n_qp):
+// every lane of the group forms the element's current midsurface geometry (FP64), lane q
+// runs through-thickness point q of the staged constitutive chain in R, the group sums the
+// qp resultants in qp order by cross-lane reads, and each lane writes the nodal forces of
+// patch nodes q, q + n_qp, ...  Elements outside the well-conditioned regime are pushed to
+// the sink for the reference-order kernel and write nothing here.
+template <class R, class Sink>
+__device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& kp,
+                                               const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ z, int e, int q,
+                                               int base, const Sink& sink) {
   const int E = kp.E;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
+  const int n_qp = kp.n_qp;
+  if (q >= n_qp) return;
   const int meta = m.meta[e];
   const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
   auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
@@ -84,35 +91,53 @@ __global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
     act.gam = 0.0;
   }
 
-  Resultants<R> r;
+  // ---- this lane's qp.  A CC3333 guard failure hands the element over, so the
+  //      reference's cross-qp guard rule never reaches a staged result: no coupling. ----
+  auto own = [](bool fail, int) { return fail; };
+  StagedQp<R> s;
   if constexpr (sizeof(R) == 4)
-    r = integrate_element_mixed(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K,
-                                kp.n_qp, kp.iters, kp.theta, kp.w);
+    s = staged_qp_mixed(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters,
+                        kp.theta[q], own);
   else
-    r = integrate_element<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.n_qp,
-                             kp.iters, kp.theta, kp.w);
+    s = staged_qp<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters,
+                     kp.theta[q], own);
+  // ---- group: resultants in qp order, status bits ----
+  Resultants<R> r;
+  r.N0 = r.N1 = r.N2 = r.M0 = r.M1 = r.M2 = 0.0;
+  int st = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxQp; j++) {
+    if (j < n_qp) {
+      StagedQp<R> o;
+      o.S00 = __shfl(s.S00, base + j);
+      o.S11 = __shfl(s.S11, base + j);
+      o.S01 = __shfl(s.S01, base + j);
+      add_resultants(r, o, kp.w[j], kp.theta[j]);
+      st |= __shfl(s.status | c33_range_bit(s.c33), base + j);
+    }
+  }
   const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) && isfinite(r.M0) &&
                       isfinite(r.M1) && isfinite(r.M2);
-  if (kp.fallback && needs_exact(r.c33, kp.n_qp, r.status, finite)) {
-    // outside the well-conditioned regime: the plain kernel recomputes this element
-    m.fallback[atomicAdd(&m.status[4], 1)] = e;
+  if (kp.fallback && needs_exact_bits(st, finite)) {
+    // outside the well-conditioned regime: the reference-order kernel recomputes it
+    if (q == 0) sink.push(e);
     return;
   }
-  for (int q = 0; q < kp.n_qp; q++) m.c33[(size_t)q * E + e] = r.c33[q];
+  m.c33[(size_t)q * E + e] = s.c33;
 
   // ---- nodal forces (ElemUpdFint, active_strain.c:1762-1995) ----
   const double pref0 = 0.5 * kp.h0 * m.dA0[e];
   const double hh = 0.5 * kp.h0;
   double* Fe = m.Fe;
   if (kind != kCap) {
-    for (int a = 0; a < nen; a++) {
+    for (int a = q; a < nen; a += n_qp) {
       const V3<double> f = node_force(na0[a], na1[a], nb0[a], nb1[a], nb2[a], G, pref0, hh, r.N0,
                                       r.N1, r.N2, r.M0, r.M1, r.M2);
       Fe[(size_t)(3 * a + 0) * E + e] = f.x;
       Fe[(size_t)(3 * a + 1) * E + e] = f.y;
       Fe[(size_t)(3 * a + 2) * E + e] = f.z;
     }
-  } else {
+  } else if (q == 0) {
     // CST membrane into the patch slots holding nv1..nv3 (:1807-1851); other slots zero
     for (int a = 0; a < nen; a++) {
       Fe[(size_t)(3 * a + 0) * E + e] = 0.0;
@@ -121,24 +146,39 @@ __global__ void __launch_bounds__(256) bf_element_kernel(DevMesh m, KParams kp,
     }
     const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
     for (int a = 0; a < 3; a++) {
-      const int s = m.corner[(size_t)(3 + a) * E + e];
-      if (s < 0) continue;
+      const int sl = m.corner[(size_t)(3 + a) * E + e];
+      if (sl < 0) continue;
       const V3<double> f = node_force(dN[a][0], dN[a][1], 0.0, 0.0, 0.0, G, pref0, hh, r.N0,
                                       r.N1, r.N2, 0.0, 0.0, 0.0);
-      Fe[(size_t)(3 * s + 0) * E + e] += f.x;
-      Fe[(size_t)(3 * s + 1) * E + e] += f.y;
-      Fe[(size_t)(3 * s + 2) * E + e] += f.z;
+      Fe[(size_t)(3 * sl + 0) * E + e] += f.x;
+      Fe[(size_t)(3 * sl + 1) * E + e] += f.y;
+      Fe[(size_t)(3 * sl + 2) * E + e] += f.z;
     }
   }
-  // ---- status ----
-  if (!finite) {
-    atomicMin(&m.status[0], e);
-  }
-  m.flags[e] = (unsigned char)(r.status | (finite ? 0 : kStatNonFinite));
-  if (r.status & kStatCC3333) {
+  // ---- status (group leader) ----
+  if (q != 0) return;
+  if (!finite) atomicMin(&m.status[0], e);
+  m.flags[e] = (unsigned char)((st & (kStatCC3333 | kStatClamp)) | (finite ? 0 : kStatNonFinite));
+  if (st & kStatCC3333) {
     atomicMin(&m.status[2], e);
     atomicAdd(&m.status[3], 1);
   }
+}
+
+constexpr int kStagedBlock = 256;
+
+template <class R>
+__global__ void __launch_bounds__(kStagedBlock) bf_element_kernel(DevMesh m, KParams kp,
+                                                                  const double* __restrict__ x,
+                                                                  const double* __restrict__ y,
+                                                                  const double* __restrict__ z) {
+  const int P = kp.n_qp, per_wave = 64 / P;
+  int slot, q, base;
+  lane_group(P, threadIdx.x & 63, slot, q, base);
+  const int wave = blockIdx.x * (kStagedBlock / 64) + (threadIdx.x >> 6);
+  const int e = wave * per_wave + slot;
+  if (slot >= per_wave || e >= kp.E) return;
+  staged_element<R>(m, kp, x, y, z, e, q, base, CtxSink{m.fallback, &m.status[4]});
 }
 
 // Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + sum over the
@@ -205,16 +245,18 @@ __global__ void bf_status_reset(int* st) {
 struct bf_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // per evaluation: before the element kernel, after it, after the hand-over kernel,
+  // after the gather
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool timing = false;
-  hipEvent_t own_ev[3] = {nullptr, nullptr, nullptr};
-  // timing ring: 3 events per evaluation, harvested when full or on bf_timing_read
+  hipEvent_t own_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // timing ring: 4 events per evaluation, harvested when full or on bf_timing_read
   static constexpr int kRing = 64;
   std::vector<hipEvent_t> ring;
   int ring_n = 0;
   bool ring_on = false;
   long long t_count = 0;
-  double t_elem = 0.0, t_gather = 0.0;
+  double t_elem = 0.0, t_handover = 0.0, t_gather = 0.0;
   // mesh
   int n_v = 0, n_nodes = 0, E = 0, NS = 12, n_elmt_base = 0, nnz = 0, n_ghost = 0;
   bool has_mesh = false, has_params = false, has_delay = false, has_cap = false;
@@ -296,11 +338,14 @@ int upload_tables(int device) {
 
 int harvest_ring(bf_ctx* c) {
   for (int i = 0; i < c->ring_n; i++) {
-    BF_HIP(hipEventSynchronize(c->ring[3 * i + 2]));
-    float a = 0.f, b = 0.f;
-    BF_HIP(hipEventElapsedTime(&a, c->ring[3 * i], c->ring[3 * i + 1]));
-    BF_HIP(hipEventElapsedTime(&b, c->ring[3 * i + 1], c->ring[3 * i + 2]));
+    hipEvent_t* r = &c->ring[4 * i];
+    BF_HIP(hipEventSynchronize(r[3]));
+    float a = 0.f, h = 0.f, b = 0.f;
+    BF_HIP(hipEventElapsedTime(&a, r[0], r[1]));
+    BF_HIP(hipEventElapsedTime(&h, r[1], r[2]));
+    BF_HIP(hipEventElapsedTime(&b, r[2], r[3]));
     c->t_elem += a;
+    c->t_handover += h;
     c->t_gather += b;
     c->t_count++;
   }
@@ -337,13 +382,13 @@ int bf_ctx_create(int device, bf_ctx** out) {
   if (rc != BF_OK) { delete c; return rc; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->own_ev[0]) != hipSuccess || hipEventCreate(&c->own_ev[1]) != hipSuccess ||
-      hipEventCreate(&c->own_ev[2]) != hipSuccess ||
+      hipEventCreate(&c->own_ev[2]) != hipSuccess || hipEventCreate(&c->own_ev[3]) != hipSuccess ||
       hipMalloc((void**)&c->d_status, 8 * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_status, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     bf_ctx_destroy(c);
     return BF_ERR_DEVICE;
   }
-  for (int i = 0; i < 3; i++) c->ev[i] = c->own_ev[i];
+  for (int i = 0; i < 4; i++) c->ev[i] = c->own_ev[i];
   c->kp.fallback = 1;
   *out = c;
   return BF_OK;
@@ -613,41 +658,36 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   dm.flags = c->d_flags;
   dm.fallback = c->d_fallback;
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
-  hipEvent_t e0 = c->own_ev[0], e1 = c->own_ev[1], e2 = c->own_ev[2];
-  c->ev[0] = e0;
-  c->ev[1] = e1;
-  c->ev[2] = e2;
+  hipEvent_t* ev = c->own_ev;
   if (c->ring_on) {
     if (c->ring_n == bf_ctx::kRing) {
       int rc = harvest_ring(c);
       if (rc != BF_OK) return rc;
     }
-    e0 = c->ring[3 * c->ring_n];
-    e1 = c->ring[3 * c->ring_n + 1];
-    e2 = c->ring[3 * c->ring_n + 2];
+    ev = &c->ring[4 * c->ring_n];
     c->ring_n++;
   }
+  for (int i = 0; i < 4; i++) c->ev[i] = ev[i];  // bf_last_timing reads the latest set
+  hipEvent_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
   BF_HIP(hipEventRecord(e0, s));
   if (c->E > 0) {
-    const dim3 grid((c->E + 255) / 256), block(256);
+    const int per_block = (kStagedBlock / 64) * (64 / kp.n_qp);
+    const dim3 grid((c->E + per_block - 1) / per_block), block(kStagedBlock);
     if (precision == BF_PREC_FP64)
       BF_HIP(launch_element_exact(dm, kp, x, y, z, s));
     else if (precision == BF_PREC_FP64_FAST)
       hipLaunchKernelGGL(bf_element_kernel<double>, grid, block, 0, s, dm, kp, x, y, z);
     else
       hipLaunchKernelGGL(bf_element_kernel<float>, grid, block, 0, s, dm, kp, x, y, z);
-    if (precision != BF_PREC_FP64 && kp.fallback) BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, s));
   }
   BF_HIP(hipEventRecord(e1, s));
+  if (c->E > 0 && precision != BF_PREC_FP64 && kp.fallback)
+    BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, s));
+  BF_HIP(hipEventRecord(e2, s));
   if (c->n_nodes > 0)
     hipLaunchKernelGGL(bf_gather_kernel, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s,
                        c->d_ptr, c->d_ent, c->d_Fe, c->E, c->n_v, c->n_nodes, accumulate, fint);
-  BF_HIP(hipEventRecord(e2, s));
-  if (c->ring_on) {
-    c->ev[0] = e0;  // bf_last_timing reads the latest triple
-    c->ev[1] = e1;
-    c->ev[2] = e2;
-  }
+  BF_HIP(hipEventRecord(e3, s));
   c->timing = true;
   return kernel_launch_check();
 }
@@ -783,10 +823,10 @@ int bf_set_fallback(bf_ctx* c, int on) {
 int bf_last_timing(bf_ctx* c, float* ms_element, float* ms_gather) {
   if (!c) return BF_ERR_ARG;
   if (!c->timing) return BF_ERR_STATE;
-  BF_HIP(hipEventSynchronize(c->ev[2]));
+  BF_HIP(hipEventSynchronize(c->ev[3]));
   float a = 0.f, b = 0.f;
-  BF_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-  BF_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+  BF_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[2]));
+  BF_HIP(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
   if (ms_element) *ms_element = a;
   if (ms_gather) *ms_gather = b;
   return BF_OK;
@@ -796,7 +836,7 @@ int bf_timing_enable(bf_ctx* c, int on) {
   if (!c) return BF_ERR_ARG;
   BF_HIP(hipSetDevice(c->device));
   if (on && c->ring.empty()) {
-    c->ring.assign(3 * bf_ctx::kRing, nullptr);
+    c->ring.assign(4 * bf_ctx::kRing, nullptr);
     for (auto& e : c->ring) BF_HIP(hipEventCreate(&e));
   }
   if (!on && c->ring_on) {
@@ -806,7 +846,7 @@ int bf_timing_enable(bf_ctx* c, int on) {
   c->ring_on = on != 0;
   c->ring_n = 0;
   c->t_count = 0;
-  c->t_elem = c->t_gather = 0.0;
+  c->t_elem = c->t_handover = c->t_gather = 0.0;
   return BF_OK;
 }
 
@@ -816,7 +856,20 @@ int bf_timing_read(bf_ctx* c, long long* count, double* ms_element, double* ms_g
   int rc = harvest_ring(c);
   if (rc != BF_OK) return rc;
   if (count) *count = c->t_count;
+  if (ms_element) *ms_element = c->t_elem + c->t_handover;
+  if (ms_gather) *ms_gather = c->t_gather;
+  return BF_OK;
+}
+
+int bf_timing_read_split(bf_ctx* c, long long* count, double* ms_element,
+                         double* ms_handover, double* ms_gather) {
+  if (!c) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  int rc = harvest_ring(c);
+  if (rc != BF_OK) return rc;
+  if (count) *count = c->t_count;
   if (ms_element) *ms_element = c->t_elem;
+  if (ms_handover) *ms_handover = c->t_handover;
   if (ms_gather) *ms_gather = c->t_gather;
   return BF_OK;
 }

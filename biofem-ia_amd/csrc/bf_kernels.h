@@ -64,6 +64,22 @@ __device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) 
   return p.gamma * s * s;
 }
 
+// Lane -> (group slot in the wave, q, first lane of the group) for groups of P adjacent
+// lanes (one element per group, one through-thickness point per lane; floor(64 / P)
+// groups per wave, the remaining lanes idle).
+__device__ __forceinline__ void lane_group(int P, int lane, int& slot, int& q, int& base) {
+  slot = lane / P;
+  q = lane - slot * P;
+  base = slot * P;
+}
+
+// Where a staged kernel lists the elements it hands to the reference-order kernel.
+struct CtxSink {
+  int* list;
+  int* count;
+  __device__ void push(int e) const { list[atomicAdd(count, 1)] = e; }
+};
+
 __device__ inline V3<double> ld3(const double* __restrict__ x, const double* __restrict__ y,
                                  const double* __restrict__ z, int n) {
   return mk(x[n], y[n], z[n]);

@@ -153,9 +153,15 @@ int bf_last_timing(bf_ctx *ctx, float *ms_element, float *ms_gather);
 
 /* Kernel timing over many evaluations: bf_timing_enable(ctx, 1) resets the totals and
  * records HIP events around the element and gather kernels of every later evaluation;
- * bf_timing_read() synchronises and returns the count and total milliseconds. */
+ * bf_timing_read() synchronises and returns the count and total milliseconds
+ * (ms_element includes the hand-over kernel of the staged precisions). */
 int bf_timing_enable(bf_ctx *ctx, int on);
 int bf_timing_read(bf_ctx *ctx, long long *count, double *ms_element, double *ms_gather);
+/* Same totals with the element phase split: the element kernel alone and the hand-over
+ * kernel (reference-order recomputation of flagged elements; 0 for BF_PREC_FP64).
+ * No reference counterpart (instrumentation). */
+int bf_timing_read_split(bf_ctx *ctx, long long *count, double *ms_element,
+                         double *ms_handover, double *ms_gather);
 
 /* Device pointers owned by the context, for callers that keep state resident:
  * coordinates and force buffers of n_nodes (3 * n_nodes for forces). */

@@ -118,7 +118,7 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
       const double pref0 = 0.5 * h0 * dA0[e];
       for (int a = 0; a < nen; a++) {
         V3<double> f = exact::node_force_exact(na0[a], na1[a], nb0[a], nb1[a], nb2[a], Gc, pref0,
-                                               0.5 * h0, o, theta, wq, n_qp, kind != 0);
+                                               0.5 * h0, o.Sv, theta, wq, n_qp, kind != 0);
         const int n = pt[a];
         if (n == kSentinel || n < 0 || n >= n_v) continue;
         fint[3 * n] += f.x;

@@ -134,3 +134,62 @@ def test_error_codes():
         assert lib.bf_ctx_create(99, C.byref(C.c_void_p())) == L.BF_ERR_ARG
     finally:
         lib.bf_ctx_destroy(ctx)
+
+
+def _noisy_reference(nx, amp, seed):
+    """Membrane whose REFERENCE surface carries N(0, amp) z-noise, evaluated at x = X: the
+    unnormalised a3,alpha of the noisy reference drives some elements through the
+    CC3333 guard of ModElemC33 (active_strain.c:1674-1681) or to non-finite forces."""
+    from biofem_amd import mesh as MM
+    m, p, _ = configs.membrane_large(nx, nx * 3 // 4)
+    rng = np.random.default_rng(seed)
+    m.z0 = m.z0.copy()
+    m.z0[: m.n_v] += rng.normal(0.0, amp, m.n_v)
+    m.apply_ghosts(m.x0, m.y0, m.z0)
+    m.dA0 = MM.area0(m.tri[: m.n_elmt], m.x0, m.y0, m.z0)
+    return m, p, (m.x0.copy(), m.y0.copy(), m.z0.copy())
+
+
+def test_cc3333_guard_case():
+    """One element trips the guard: qps after the failing one keep their C33 in that
+    iteration, exactly as the reference's early return does."""
+    m, p, (x, y, z) = _noisy_reference(128, 3e-3, 2)
+    print("case guard 128x96 amp 3e-3 seed 2", flush=True)
+    ref, st, c33 = O.finternal_act(m, p, x, y, z, want_c33=True)
+    assert st[2] == 1 and np.isfinite(ref).all()
+    fe = ActStrainFE(m, p)
+    try:
+        got = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
+        assert np.array_equal(got, ref)
+        assert np.array_equal(fe.c33(), c33)
+        e, code, warn = fe.last_error()
+        assert warn == st[2]
+        assert fe.elem_flags()[st[1]] & 1
+        fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+        mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
+        assert rel(fast, ref) <= 1e-12 and rel(mixed, ref) <= 1e-5
+    finally:
+        fe.close()
+
+
+def test_nonfinite_case_reported():
+    """Elements whose force is not finite: the plain path reproduces the same values
+    (NaN in the same places), returns BF_ERR_ELEMENT and reports BF_ELEM_NONFINITE with the
+    first such element."""
+    m, p, (x, y, z) = _noisy_reference(128, 3e-3, 0)
+    print("case non-finite 128x96 amp 3e-3 seed 0", flush=True)
+    ref, st, _ = O.finternal_act(m, p, x, y, z, want_c33=True)
+    assert not np.isfinite(ref).all()
+    fe = ActStrainFE(m, p)
+    try:
+        got = np.zeros(3 * m.n_nodes)
+        rc = fe.lib.bf_force_eval(fe.ctx, L.dp(x), L.dp(y), L.dp(z), L.dp(got), 0, L.BF_PREC_FP64)
+        assert rc == L.BF_ERR_ELEMENT  # forces are still written (PETSC_ERR_FP class)
+        assert np.array_equal(got, ref, equal_nan=True)
+        e, code, warn = fe.last_error()
+        assert code == L.BF_ELEM_NONFINITE and e >= 0
+        # the library counts elements, the oracle counts guard trips (element x iteration)
+        assert 0 < warn <= st[2]
+        assert int((fe.elem_flags() & L.FLAG_CC3333).astype(bool).sum()) == warn
+    finally:
+        fe.close()

@@ -227,11 +227,13 @@ def main():
         },
         "cpu_baseline": cpu,
         "fp32_mixed": dict(res["mixed"], speedup_vs_fp64=res["mixed"]["elem_qp_per_s"] / r64["elem_qp_per_s"],
-                           speedup_vs_ref_order=res["mixed"]["elem_qp_per_s"] / res["fp64_ref_order"]["elem_qp_per_s"],
-                           roofline_frac=res["mixed"]["gbps_element_kernel"] / HBM_PEAK_GBPS),
+                           speedup_vs_ref_order=(res["mixed"]["elem_qp_per_s"] / res["fp64_ref_order"]["elem_qp_per_s"]
+                                                 if "fp64_ref_order" in res else None),
+                           roofline_frac=res["mixed"]["gbps_element_kernel"] / HBM_PEAK_GBPS) if "mixed" in res else None,
         "fp64_ref_order": dict(res["fp64_ref_order"],
                                note="reference operation order, no contraction: bitwise equal to the CPU oracle",
-                               speedup_vs_fp64=res["fp64_ref_order"]["elem_qp_per_s"] / r64["elem_qp_per_s"]),
+                               speedup_vs_fp64=res["fp64_ref_order"]["elem_qp_per_s"] / r64["elem_qp_per_s"])
+                          if "fp64_ref_order" in res else None,
         "fp64_detail": r64,
         "bytes": {"element_kernel": elem_bytes, "path": path_bytes, "overhead_element_buffer": overhead_bytes},
         "parity": parity,

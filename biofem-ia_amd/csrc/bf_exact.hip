@@ -236,9 +236,10 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
-  // one lane group per listed element when the list fills the machine
+  // persistent: one wave per SIMD of the 256-CU part at the kernel's occupancy of one
+  // (the count is known on the device only; extra waves of a larger grid would only exit)
   const int per_block = (kExactBlock / 64) * (64 / kp.n_qp);
-  const int blocks = std::min((kp.E + per_block - 1) / per_block, 4096);
+  const int blocks = std::min((kp.E + per_block - 1) / per_block, 512);
   hipLaunchKernelGGL(bf_element_kernel_exact_list, dim3(blocks), dim3(kExactBlock), 0, s, m, kp, x, y, z);
   return hipGetLastError();
 }

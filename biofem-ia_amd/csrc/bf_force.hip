@@ -38,24 +38,38 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
                                                const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z, int e, int q,
-                                               int base, const Sink& sink) {
+                                               int base, const Sink& sink, int wave) {
   const int E = kp.E;
   const int n_qp = kp.n_qp;
   if (q >= n_qp) return;
+  // Every per-element load that does not depend on another one is issued here, before the
+  // first branch, so that they share one memory round trip: kind, the first 12 patch slots,
+  // the geom0 cache, activation data and the area.  The coordinate gathers form the second.
   const int meta = m.meta[e];
+  int pidx[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) pidx[i] = m.patch[(size_t)i * E + e];
+  const double* rf = m.ref;
+  double rv[15];
+#pragma unroll
+  for (int k = 0; k < 15; k++) rv[k] = rf[(size_t)k * E + e];
+  const double f0 = m.fib[e], f1 = m.fib[(size_t)E + e], f2 = m.fib[(size_t)2 * E + e];
+  const double gs = m.gscale ? m.gscale[e] : 1.0;
+  const double dly = kp.has_delay ? m.delay[e] : 0.0;
+  const double A0 = m.dA0[e];
   const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
-  auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
 
   // ---- current midsurface geometry (ElemUpdateGeomSubdiv, active_strain.c:727) ----
   MidGeom G;
   const double *na0, *na1, *nb0, *nb1, *nb2;
   if (kind == kRegular) {
     na0 = c_Na[0]; na1 = c_Na[1]; nb0 = c_Nab[0]; nb1 = c_Nab[1]; nb2 = c_Nab[2];
-    G = mid_geom(12, na0, na1, nb0, nb1, nb2, xyz);
+    G = mid_geom_regular([&](int i) { return ld3(x, y, z, pidx[i]); });
   } else if (kind == kIrregular) {
     const double(*t)[kMaxNen] = c_IN[val];
     na0 = t[0]; na1 = t[1]; nb0 = t[2]; nb1 = t[3]; nb2 = t[4];
-    G = mid_geom(nen, na0, na1, nb0, nb1, nb2, xyz);
+    G = mid_geom(nen, na0, na1, nb0, nb1, nb2,
+                 [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); });
   } else {
     na0 = na1 = nb0 = nb1 = nb2 = nullptr;
     G = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),
@@ -66,18 +80,17 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
 
   // ---- reference cache ----
   RefGeom rg;
-  const double* rf = m.ref;
-  rg.a1 = mk(rf[(size_t)0 * E + e], rf[(size_t)1 * E + e], rf[(size_t)2 * E + e]);
-  rg.a2 = mk(rf[(size_t)3 * E + e], rf[(size_t)4 * E + e], rf[(size_t)5 * E + e]);
-  rg.a3 = mk(rf[(size_t)6 * E + e], rf[(size_t)7 * E + e], rf[(size_t)8 * E + e]);
-  rg.a31 = mk(rf[(size_t)9 * E + e], rf[(size_t)10 * E + e], rf[(size_t)11 * E + e]);
-  rg.a32 = mk(rf[(size_t)12 * E + e], rf[(size_t)13 * E + e], rf[(size_t)14 * E + e]);
+  rg.a1 = mk(rv[0], rv[1], rv[2]);
+  rg.a2 = mk(rv[3], rv[4], rv[5]);
+  rg.a3 = mk(rv[6], rv[7], rv[8]);
+  rg.a31 = mk(rv[9], rv[10], rv[11]);
+  rg.a32 = mk(rv[12], rv[13], rv[14]);
 
   // ---- activation (ElemActDefGrad, active_strain.c:1208-1257) ----
-  double g = kp.has_delay ? ramp_gamma(kp.ramp, m.delay[e]) : kp.gamma_t;
-  g *= m.gscale ? m.gscale[e] : 1.0;
+  double g = kp.has_delay ? ramp_gamma(kp.ramp, dly) : kp.gamma_t;
+  g *= gs;
   Activation act;
-  act.f = mk(m.fib[e], m.fib[(size_t)E + e], m.fib[(size_t)2 * E + e]);
+  act.f = mk(f0, f1, f2);
   if (kp.act_mode == 0) {
     act.d0 = act.d1 = act.d2 = 1.0;
     act.gam = g;
@@ -94,13 +107,18 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
   // ---- this lane's qp.  A CC3333 guard failure hands the element over, so the
   //      reference's cross-qp guard rule never reaches a staged result: no coupling. ----
   auto own = [](bool fail, int) { return fail; };
+  // theta of this lane's qp by selects over the wave-uniform table (a lane-indexed read of
+  // the kernel arguments would be a vector memory load in the middle of the chain)
+  double theta_q = kp.theta[0];
+#pragma unroll
+  for (int j = 1; j < kMaxQp; j++) theta_q = (q == j) ? kp.theta[j] : theta_q;
   StagedQp<R> s;
   if constexpr (sizeof(R) == 4)
     s = staged_qp_mixed(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters,
-                        kp.theta[q], own);
+                        theta_q, own, kp.fallback != 0);
   else
     s = staged_qp<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters,
-                     kp.theta[q], own);
+                     theta_q, own, kp.fallback != 0);
   // ---- group: resultants in qp order, status bits ----
   Resultants<R> r;
   r.N0 = r.N1 = r.N2 = r.M0 = r.M1 = r.M2 = 0.0;
@@ -113,20 +131,19 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
       o.S11 = __shfl(s.S11, base + j);
       o.S01 = __shfl(s.S01, base + j);
       add_resultants(r, o, kp.w[j], kp.theta[j]);
-      st |= __shfl(s.status | c33_range_bit(s.c33), base + j);
+      st |= __shfl(s.status | c33_range_bit(s.c33, sizeof(R) == 4), base + j);
     }
   }
   const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) && isfinite(r.M0) &&
                       isfinite(r.M1) && isfinite(r.M2);
-  if (kp.fallback && needs_exact_bits(st, finite)) {
-    // outside the well-conditioned regime: the reference-order kernel recomputes it
-    if (q == 0) sink.push(e);
-    return;
-  }
+  // outside the well-conditioned regime: the reference-order kernel recomputes it
+  const bool handed = kp.fallback && needs_exact_bits(st, finite);
+  sink.push(handed && q == 0, e, wave);
+  if (handed) return;
   m.c33[(size_t)q * E + e] = s.c33;
 
   // ---- nodal forces (ElemUpdFint, active_strain.c:1762-1995) ----
-  const double pref0 = 0.5 * kp.h0 * m.dA0[e];
+  const double pref0 = 0.5 * kp.h0 * A0;
   const double hh = 0.5 * kp.h0;
   double* Fe = m.Fe;
   if (kind != kCap) {
@@ -178,7 +195,40 @@ __global__ void __launch_bounds__(kStagedBlock) bf_element_kernel(DevMesh m, KPa
   const int wave = blockIdx.x * (kStagedBlock / 64) + (threadIdx.x >> 6);
   const int e = wave * per_wave + slot;
   if (slot >= per_wave || e >= kp.E) return;
-  staged_element<R>(m, kp, x, y, z, e, q, base, CtxSink{m.fallback, &m.status[4]});
+  staged_element<R>(m, kp, x, y, z, e, q, base, WaveSink{m.wlist, m.wcount, per_wave}, wave);
+}
+
+// Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
+// m.fallback and stores the total in m.status[4].  One block: a chunk of waves per thread,
+// a block-wide scan of the chunk sums, then each thread copies its chunk.
+constexpr int kCompactBlock = 1024;
+__global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(const int* __restrict__ wcount,
+                                                                   const int* __restrict__ wlist,
+                                                                   int n_waves, int per_wave,
+                                                                   int* __restrict__ out,
+                                                                   int* __restrict__ status) {
+  __shared__ int s_sum[kCompactBlock];
+  const int t = threadIdx.x;
+  const int chunk = (n_waves + kCompactBlock - 1) / kCompactBlock;
+  const int w0 = t * chunk, w1 = min(n_waves, w0 + chunk);
+  int sum = 0;
+  for (int w = w0; w < w1; w++) sum += wcount[w];
+  s_sum[t] = sum;
+  __syncthreads();
+  // inclusive Hillis-Steele scan over the block
+  for (int d = 1; d < kCompactBlock; d <<= 1) {
+    const int v = (t >= d) ? s_sum[t - d] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  int pos = s_sum[t] - sum;
+  for (int w = w0; w < w1; w++) {
+    const int c = wcount[w];
+    for (int k = 0; k < c; k++) out[pos + k] = wlist[(size_t)w * per_wave + k];
+    pos += c;
+  }
+  if (t == kCompactBlock - 1) status[4] = s_sum[t];
 }
 
 // Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + sum over the
@@ -205,13 +255,29 @@ __global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ 
     ay = fint[3 * (size_t)n + 1];
     az = fint[3 * (size_t)n + 2];
   }
+  // entries are read kGatherChunk at a time (their loads overlap), then added in order
+  constexpr int kGatherChunk = 8;
   const int b = ptr[n], en = ptr[n + 1];
-  for (int j = b; j < en; j++) {
-    const unsigned v = ent[j];
-    const int e = (int)(v >> 5), s = (int)(v & 31u);
-    ax += Fe[(size_t)(3 * s + 0) * E + e];
-    ay += Fe[(size_t)(3 * s + 1) * E + e];
-    az += Fe[(size_t)(3 * s + 2) * E + e];
+  for (int j0 = b; j0 < en; j0 += kGatherChunk) {
+    double vx[kGatherChunk], vy[kGatherChunk], vz[kGatherChunk];
+#pragma unroll
+    for (int k = 0; k < kGatherChunk; k++) {
+      if (j0 + k < en) {
+        const unsigned v = ent[j0 + k];
+        const int e = (int)(v >> 5), s = (int)(v & 31u);
+        vx[k] = Fe[(size_t)(3 * s + 0) * E + e];
+        vy[k] = Fe[(size_t)(3 * s + 1) * E + e];
+        vz[k] = Fe[(size_t)(3 * s + 2) * E + e];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kGatherChunk; k++) {
+      if (j0 + k < en) {
+        ax += vx[k];
+        ay += vy[k];
+        az += vz[k];
+      }
+    }
   }
   fint[3 * (size_t)n + 0] = ax;
   fint[3 * (size_t)n + 1] = ay;
@@ -272,6 +338,8 @@ struct bf_ctx {
   double* d_c33 = nullptr;
   unsigned char* d_flags = nullptr;
   int* d_fallback = nullptr;
+  int* d_wlist = nullptr;
+  int* d_wcount = nullptr;
   int* d_ptr = nullptr;
   unsigned* d_ent = nullptr;
   int* d_rules = nullptr;
@@ -313,7 +381,7 @@ void dfree(T*& p) {
 void free_mesh(bf_ctx* c) {
   dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
   dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
-  dfree(c->d_flags); dfree(c->d_fallback);
+  dfree(c->d_flags); dfree(c->d_fallback); dfree(c->d_wlist); dfree(c->d_wcount);
   dfree(c->d_ptr); dfree(c->d_ent); dfree(c->d_rules); dfree(c->d_x); dfree(c->d_y);
   dfree(c->d_z); dfree(c->d_fint);
   c->has_mesh = false;
@@ -543,6 +611,8 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
   ALLOC(c->d_c33, (size_t)kMaxQp * En);
   ALLOC(c->d_flags, En);
   ALLOC(c->d_fallback, En);
+  ALLOC(c->d_wlist, En + 64);
+  ALLOC(c->d_wcount, En);
   ALLOC(c->d_ptr, (size_t)m->n_v + 1);
   ALLOC(c->d_ent, (size_t)std::max(nnz, 1));
   ALLOC(c->d_x, Nn);
@@ -657,6 +727,8 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   dm.status = c->d_status;
   dm.flags = c->d_flags;
   dm.fallback = c->d_fallback;
+  dm.wlist = c->d_wlist;
+  dm.wcount = c->d_wcount;
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
   hipEvent_t* ev = c->own_ev;
   if (c->ring_on) {
@@ -681,8 +753,13 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
       hipLaunchKernelGGL(bf_element_kernel<float>, grid, block, 0, s, dm, kp, x, y, z);
   }
   BF_HIP(hipEventRecord(e1, s));
-  if (c->E > 0 && precision != BF_PREC_FP64 && kp.fallback)
+  if (c->E > 0 && precision != BF_PREC_FP64 && kp.fallback) {
+    const int per_wave = 64 / kp.n_qp;
+    const int n_waves = (c->E + per_wave - 1) / per_wave;
+    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, c->d_wcount,
+                       c->d_wlist, n_waves, per_wave, c->d_fallback, c->d_status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, s));
+  }
   BF_HIP(hipEventRecord(e2, s));
   if (c->n_nodes > 0)
     hipLaunchKernelGGL(bf_gather_kernel, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s,

@@ -38,6 +38,8 @@ struct DevMesh {
   int* status;         // [8] first error elem, code, first warn elem, warn count, fallback count
   unsigned char* flags;  // [E] per-element status bits of the last evaluation
   int* fallback;       // [E] elements handed from a staged kernel to the plain kernel
+  int* wlist;          // [E + 64] per-wave segments of the staged kernel's hand-over list
+  int* wcount;         // [E] per-wave hand-over counts
 };
 
 // GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
@@ -73,11 +75,23 @@ __device__ __forceinline__ void lane_group(int P, int lane, int& slot, int& q, i
   base = slot * P;
 }
 
-// Where a staged kernel lists the elements it hands to the reference-order kernel.
-struct CtxSink {
-  int* list;
-  int* count;
-  __device__ void push(int e) const { list[atomicAdd(count, 1)] = e; }
+// Where a staged kernel lists the elements it hands to the reference-order kernel: every
+// wave owns a segment of `per_wave` entries and writes its count, with no atomics (device-
+// scope atomics on one counter serialise across the chip and cost more than the kernel's
+// arithmetic).  A compaction pass concatenates the segments.  push() is called by every
+// active lane of a wave that holds at least one element; lanes with `take` set append e.
+struct WaveSink {
+  int* list;    // [n_waves * per_wave]
+  int* count;   // [n_waves]
+  int per_wave;
+  __device__ void push(bool take, int e, int wave) const {
+    const unsigned long long act = __ballot(true);
+    const unsigned long long mask = __ballot(take);
+    const int lane = (int)(threadIdx.x & 63u);
+    const int leader = __ffsll((long long)act) - 1;
+    if (lane == leader) count[wave] = __popcll(mask);
+    if (take) list[(size_t)wave * per_wave + __popcll(mask & ((1ull << lane) - 1ull))] = e;
+  }
 };
 
 __device__ inline V3<double> ld3(const double* __restrict__ x, const double* __restrict__ y,

@@ -82,6 +82,71 @@ def run_gpu(fe, lib, L, prec, steps, warmup):
     return wall, n.value, te.value / k, th.value / k, tg.value / k
 
 
+def run_batched(args, L, log):
+    """64 flat membranes of uneven size in one persistent launch per phase (config of the
+    batched workload): tuned launch shape, hipGraph-captured steps, FP32-mixed against the
+    plain FP64 path (reference operation order, bitwise equal to the CPU oracle)."""
+    import torch  # noqa: F401
+    from biofem_amd import configs
+    from biofem_amd.batch import Batch, upload_coords
+    from biofem_amd.fem import ActStrainFE
+    t0 = time.perf_counter()
+    cases = configs.batch_membranes(n=args.batch_n, seed=2)
+    fes = []
+    for m, p, (x, y, z) in cases:
+        fe = ActStrainFE(m, p)
+        upload_coords(fe, x, y, z)
+        fes.append(fe)
+    b = Batch(fes)
+    E = sum(m.n_elmt for m, _, _ in cases)
+    q = cases[0][1].n_qp
+    elem_bytes = sum(byte_model(m, q)[0] for m, _, _ in cases)
+    path_bytes = sum(byte_model(m, q)[1] for m, _, _ in cases)
+    log(f"batched: {len(cases)} membranes, {E} elements, setup {time.perf_counter() - t0:.1f} s")
+    tuned = b.tune(L.BF_PREC_MIXED, reps=3)
+    log(f"batched: tuned launch {tuned}, {b.info()}")
+    res = {}
+    forces = {}
+    for name, prec in (("mixed", L.BF_PREC_MIXED), ("fp64", L.BF_PREC_FP64_FAST),
+                       ("fp64_ref_order", L.BF_PREC_FP64)):
+        steps = args.steps if name != "fp64_ref_order" else max(3, args.steps // 10)
+        for _ in range(args.warmup):
+            b.eval(prec, ghosts=True, graph=True)
+        b.sync()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            b.eval(prec, ghosts=True, graph=True)
+        b.sync()
+        wall = time.perf_counter() - t1
+        handed = b.fallback_count() if prec != L.BF_PREC_FP64 else 0
+        forces[name] = [b.forces(i) for i in range(len(fes))]
+        res[name] = dict(steps=steps, ms_per_step=1e3 * wall / steps, elem_qp_per_s=E * q * steps / wall,
+                         gbps_step=path_bytes * steps / wall / 1e9, handed_to_ref_order=handed)
+        log(f"batched {name}: {res[name]['ms_per_step']:.4f} ms/step ({handed} handed over)")
+    ref = forces["fp64_ref_order"]
+
+    def worst(key):
+        return max(float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300))
+                   for a, r in zip(forces[key], ref))
+    out = dict(
+        workload=f"{len(cases)} flat active-strain membranes, nx log-uniform [16, 256], ny = 0.75 nx "
+                 f"(seed 2), {E} triangles, per-membrane E U[0.5,2], h0 U[0.005,0.02], gamma U[0.05,0.2], "
+                 "smooth current state; step = GlobalGhost + force evaluation of every membrane, "
+                 "hipGraph replay",
+        elements=E, membranes=len(cases), tuned_launch=tuned, algorithmic_bytes_step=path_bytes,
+        element_phase_bytes=elem_bytes,
+        mixed=res["mixed"], fp64=res["fp64"], fp64_ref_order=res["fp64_ref_order"],
+        mixed_speedup_vs_ref_order=res["mixed"]["elem_qp_per_s"] / res["fp64_ref_order"]["elem_qp_per_s"],
+        mixed_speedup_vs_fp64=res["mixed"]["elem_qp_per_s"] / res["fp64"]["elem_qp_per_s"],
+        mixed_step_roofline_frac=res["mixed"]["gbps_step"] / HBM_PEAK_GBPS,
+        max_rel_err_vs_ref_order={"mixed": worst("mixed"), "fp64": worst("fp64")},
+    )
+    b.close()
+    for fe in fes:
+        fe.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +160,8 @@ def main():
                     help="after the timed work, run 3 device copies of this many MiB (a known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--modes", default="fp64,fp64_ref_order,mixed")
+    ap.add_argument("--batch", type=int, default=1, help="also run the batched 64-membrane workload")
+    ap.add_argument("--batch-n", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -185,6 +252,7 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"calibration copy: 3 x {args.copy_mib} MiB")
+    batched = run_batched(args, L, log) if args.batch else None
     if "fp64" not in res:
         print(json.dumps({k: v for k, v in res.items()}), flush=True)
         fe.close()
@@ -237,6 +305,7 @@ def main():
         "fp64_detail": r64,
         "bytes": {"element_kernel": elem_bytes, "path": path_bytes, "overhead_element_buffer": overhead_bytes},
         "parity": parity,
+        "batched": batched,
     }
     fe.close()
     print(json.dumps(out), flush=True)

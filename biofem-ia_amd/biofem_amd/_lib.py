@@ -87,6 +87,20 @@ SIGNATURES = {
     "bf_timing_enable": (C.c_int, [_vp, C.c_int]),
     "bf_timing_read": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp, _dp]),
     "bf_timing_read_split": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp, _dp, _dp]),
+    "bf_coords_upload": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "bf_fint_download": (C.c_int, [_vp, _dp]),
+    "bf_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "bf_batch_destroy": (C.c_int, [_vp]),
+    "bf_batch_set_cases": (C.c_int, [_vp, C.POINTER(_vp), C.c_int]),
+    "bf_batch_refresh": (C.c_int, [_vp]),
+    "bf_batch_eval": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "bf_batch_sync": (C.c_int, [_vp]),
+    "bf_batch_last_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "bf_batch_last_fallback_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "bf_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bf_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bf_batch_get_launch": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bf_batch_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "bf_device_buffers": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "bf_sync": (C.c_int, [_vp]),
     "bf_topo_area0": (C.c_int, [C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),

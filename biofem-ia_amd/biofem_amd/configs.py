@@ -71,3 +71,29 @@ def ellipsoid_shell(levels: int = 6, ti: int = 30, seed: int = 3):
     z[:n] *= 0.99
     m.apply_ghosts(x, y, z)
     return m, p, (x, y, z)
+
+
+def batch_membranes(n: int = 64, seed: int = 2, nx_lo: int = 16, nx_hi: int = 256,
+                    amp: float = 0.01):
+    """n independent flat 1.0 x 0.75 membranes for one batched launch: nx log-uniform in
+    [nx_lo, nx_hi], ny = round(0.75 nx); per membrane Young's modulus U[0.5, 2] (nu = 0.3),
+    thickness U[0.005, 0.02], active strain U[0.05, 0.2] along a fibre at U[-30, 30] deg per
+    element; current state = reference + the smooth displacement field of amplitude amp.
+    default_rng(seed) draws nx, E, h0, gamma for all membranes first, then per membrane its
+    fibre angles; mesh jitter uses seed 1000 + i."""
+    rng = np.random.default_rng(seed)
+    nxs = np.rint(np.exp(rng.uniform(np.log(nx_lo), np.log(nx_hi), n))).astype(int)
+    E_mod = rng.uniform(0.5, 2.0, n)
+    h0 = rng.uniform(0.005, 0.02, n)
+    gam = rng.uniform(0.05, 0.2, n)
+    out = []
+    for i in range(n):
+        nx = int(nxs[i])
+        ny = max(2, int(round(0.75 * nx)))
+        m = flat_membrane(nx, ny, 1.0, 0.75, jitter=1e-3, seed=1000 + i)
+        ang = np.deg2rad(rng.uniform(-30.0, 30.0, m.n_elmt))
+        m.n_fib = np.stack([np.cos(ang), np.sin(ang), np.zeros_like(ang)], 1)
+        p = ActParams.from_young(float(E_mod[i]), 0.3, n_qp=3, c33_iters=3, cart_fib_act=1,
+                                 h0=float(h0[i]), gamma=float(gam[i]), ramp_type=0)
+        out.append((m, p, smooth_displacement(m, amp)))
+    return out

@@ -222,6 +222,62 @@ __global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact_list(DevM
   }
 }
 
+// Plain FP64 element phase of a batch over the element tiles (one case per tile).
+__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_kernel(BatchDev b) {
+  const int P = b.P, per_wave = 64 / P;
+  QpGroup g;
+  int slot;
+  lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int n_waves = gridDim.x * (blockDim.x >> 6);
+  const int stride = b.persistent ? n_waves : b.n_etiles;
+  for (int t = wave; t < b.n_etiles; t += stride) {
+    const Tile tl = b.etiles[t];
+    const int c = __builtin_amdgcn_readfirstlane(tl.c);
+    const int e0 = __builtin_amdgcn_readfirstlane(tl.b);
+    const int n = __builtin_amdgcn_readfirstlane(tl.n);
+    const CaseDev& cd = b.cases[c];
+    for (int k0 = 0; k0 < n; k0 += per_wave) {
+      const int i = k0 + slot;
+      if (slot < per_wave && i < n) {
+        g.e = e0 + i;
+        exact_element(cd.m, cd.kp, cd.x, cd.y, cd.z, g, 0);
+      }
+    }
+  }
+}
+
+// Hand-over drain of a batch: element groups stride over b.hlist[0 .. *b.htotal).
+__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_list_kernel(BatchDev b) {
+  const int count = *b.htotal;
+  const int P = b.P, per_wave = 64 / P;
+  QpGroup g;
+  int slot;
+  lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
+  if (slot >= per_wave) return;
+  const int waves = gridDim.x * (kExactBlock / 64);
+  for (int i = (blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6)) * per_wave + slot; i < count;
+       i += waves * per_wave) {
+    const int2 v = b.hlist[i];
+    const CaseDev& cd = b.cases[v.x];
+    g.e = v.y;
+    exact_element(cd.m, cd.kp, cd.x, cd.y, cd.z, g, kStatFallback);
+  }
+}
+
+hipError_t launch_batch_exact(const BatchDev& b, int blocks, hipStream_t s) {
+  if (b.n_etiles <= 0) return hipSuccess;
+  if (blocks <= 0) blocks = (b.n_etiles + (kExactBlock / 64) - 1) / (kExactBlock / 64);
+  hipLaunchKernelGGL(bf_batch_exact_kernel, dim3(blocks), dim3(kExactBlock), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_exact_list(const BatchDev& b, int blocks, hipStream_t s) {
+  if (b.n_etiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bf_batch_exact_list_kernel, dim3(blocks), dim3(kExactBlock), 0, s, b);
+  return hipGetLastError();
+}
+
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_IN), tab, bytes);
 }

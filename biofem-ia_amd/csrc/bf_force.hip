@@ -38,7 +38,7 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
                                                const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z, int e, int q,
-                                               int base, const Sink& sink, int wave) {
+                                               int base, const Sink& sink) {
   const int E = kp.E;
   const int n_qp = kp.n_qp;
   if (q >= n_qp) return;
@@ -138,7 +138,7 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
                       isfinite(r.M1) && isfinite(r.M2);
   // outside the well-conditioned regime: the reference-order kernel recomputes it
   const bool handed = kp.fallback && needs_exact_bits(st, finite);
-  sink.push(handed && q == 0, e, wave);
+  sink.push(handed && q == 0, e);
   if (handed) return;
   m.c33[(size_t)q * E + e] = s.c33;
 
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(kStagedBlock) bf_element_kernel(DevMesh m, KPa
   const int wave = blockIdx.x * (kStagedBlock / 64) + (threadIdx.x >> 6);
   const int e = wave * per_wave + slot;
   if (slot >= per_wave || e >= kp.E) return;
-  staged_element<R>(m, kp, x, y, z, e, q, base, WaveSink{m.wlist, m.wcount, per_wave}, wave);
+  staged_element<R>(m, kp, x, y, z, e, q, base, WaveSink{m.wlist, m.wcount, per_wave, wave});
 }
 
 // Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
@@ -234,13 +234,10 @@ __global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(const int* __
 // Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + sum over the
 // node's (element, slot) entries in ascending element order.  Ghost rows (n >= n_v) are
 // zeroed when not accumulating, as after the caller's VecSet(Fint, 0).
-__global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ ptr,
-                                                        const unsigned* __restrict__ ent,
-                                                        const double* __restrict__ Fe, int E,
-                                                        int n_v, int n_nodes, int accumulate,
-                                                        double* __restrict__ fint) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= n_nodes) return;
+__device__ __forceinline__ void gather_node(const int* __restrict__ ptr,
+                                            const unsigned* __restrict__ ent,
+                                            const double* __restrict__ Fe, int E, int n_v,
+                                            int accumulate, double* __restrict__ fint, int n) {
   if (n >= n_v) {
     if (!accumulate) {
       fint[3 * (size_t)n + 0] = 0.0;
@@ -282,6 +279,106 @@ __global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ 
   fint[3 * (size_t)n + 0] = ax;
   fint[3 * (size_t)n + 1] = ay;
   fint[3 * (size_t)n + 2] = az;
+}
+
+__global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ ptr,
+                                                        const unsigned* __restrict__ ent,
+                                                        const double* __restrict__ Fe, int E,
+                                                        int n_v, int n_nodes, int accumulate,
+                                                        double* __restrict__ fint) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  gather_node(ptr, ent, Fe, E, n_v, accumulate, fint, n);
+}
+
+// ------------------------------------------------------------------ batch kernels ------
+// Prologue of a batched step: per-case status reset, then (ghosts != 0) GlobalGhost over
+// the ghost tiles, one block per tile stride.
+__global__ void __launch_bounds__(256) bf_batch_prologue(BatchDev b, int ghosts) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < b.n_cases; c += gridDim.x * blockDim.x) {
+    int* st = b.cases[c].m.status;
+    st[0] = INT_MAX;
+    st[1] = 0;
+    st[2] = INT_MAX;
+    st[3] = 0;
+    st[4] = 0;
+  }
+  if (!ghosts) return;
+  for (int t = blockIdx.x; t < b.n_gtiles; t += gridDim.x) {
+    const Tile tl = b.gtiles[t];
+    const CaseDev& cd = b.cases[tl.c];
+    for (int i = threadIdx.x; i < tl.n; i += blockDim.x) {
+      const int g = tl.b + i;
+      const int rb = cd.rules[3 * g], ra = cd.rules[3 * g + 1], rc = cd.rules[3 * g + 2];
+      cd.x[cd.n_v + g] = cd.x[rb] + cd.x[ra] - cd.x[rc];
+      cd.y[cd.n_v + g] = cd.y[rb] + cd.y[ra] - cd.y[rc];
+      cd.z[cd.n_v + g] = cd.z[rb] + cd.z[ra] - cd.z[rc];
+    }
+  }
+}
+
+// Staged element phase of a batch: each wave takes element tiles from the device work
+// list (persistent: a fixed stride over the list; otherwise one tile per wave) and runs
+// them in passes of floor(64 / P) element groups.  Every tile belongs to one case, so the
+// case's mesh and options are wave-uniform.
+template <class R>
+__global__ void __launch_bounds__(kStagedBlock) bf_batch_element_kernel(BatchDev b) {
+  const int P = b.P, per_wave = 64 / P;
+  int slot, q, base;
+  lane_group(P, threadIdx.x & 63, slot, q, base);
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int n_waves = gridDim.x * (blockDim.x >> 6);
+  const int stride = b.persistent ? n_waves : b.n_etiles;
+  for (int t = wave; t < b.n_etiles; t += stride) {
+    const Tile tl = b.etiles[t];
+    const int c = __builtin_amdgcn_readfirstlane(tl.c);
+    const int e0 = __builtin_amdgcn_readfirstlane(tl.b);
+    const int n = __builtin_amdgcn_readfirstlane(tl.n);
+    const CaseDev& cd = b.cases[c];
+    int pushed = 0;
+    for (int k0 = 0; k0 < n; k0 += per_wave) {
+      const int i = k0 + slot;
+      if (slot < per_wave && i < n)
+        staged_element<R>(cd.m, cd.kp, cd.x, cd.y, cd.z, e0 + i, q, base,
+                          TileSink{b.tlist + (size_t)t * b.tile_cap, &pushed});
+    }
+    if ((threadIdx.x & 63) == 0) b.tcount[t] = pushed;
+  }
+}
+
+// Concatenates the tiles' hand-over segments into (case, element) pairs, *htotal = count.
+__global__ void __launch_bounds__(kCompactBlock) bf_batch_compact_kernel(BatchDev b) {
+  __shared__ int s_sum[kCompactBlock];
+  const int t = threadIdx.x;
+  const int chunk = (b.n_etiles + kCompactBlock - 1) / kCompactBlock;
+  const int w0 = t * chunk, w1 = min(b.n_etiles, w0 + chunk);
+  int sum = 0;
+  for (int w = w0; w < w1; w++) sum += b.tcount[w];
+  s_sum[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < kCompactBlock; d <<= 1) {
+    const int v = (t >= d) ? s_sum[t - d] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  int pos = s_sum[t] - sum;
+  for (int w = w0; w < w1; w++) {
+    const int cnt = b.tcount[w];
+    const int c = b.etiles[w].c;
+    for (int k = 0; k < cnt; k++) b.hlist[pos + k] = make_int2(c, b.tlist[(size_t)w * b.tile_cap + k]);
+    pos += cnt;
+  }
+  if (t == kCompactBlock - 1) *b.htotal = s_sum[t];
+}
+
+// Node gather of a batch: one block per node tile.
+__global__ void __launch_bounds__(256) bf_batch_gather_kernel(BatchDev b) {
+  const Tile tl = b.ntiles[blockIdx.x];
+  const CaseDev& cd = b.cases[tl.c];
+  const int i = threadIdx.x;
+  if (i >= tl.n) return;
+  gather_node(cd.ptr, cd.ent, cd.m.Fe, cd.kp.E, cd.n_v, 0, cd.fint, tl.b + i);
 }
 
 // GlobalGhost (src/bending.c:1763-1814): ghost n_v+g = x[b] + x[a] - x[c] per rule (b,a,c).
@@ -704,15 +801,8 @@ int bf_params_set(bf_ctx* c, const bf_params* p) {
   return BF_OK;
 }
 
-static int launch_eval(bf_ctx* c, const double* x, const double* y, const double* z,
-                       double* fint, int accumulate, int precision, hipStream_t s) {
-  if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
-  if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
-    return BF_ERR_ARG;
-  KParams kp = c->kp;
-  kp.E = c->E;
-  kp.NS = c->NS;
-  kp.has_delay = c->has_delay ? 1 : 0;
+// The kernels' view of a context: its device arrays and its options.
+static DevMesh dev_mesh(const bf_ctx* c) {
   DevMesh dm{};
   dm.patch = c->d_patch;
   dm.meta = c->d_meta;
@@ -729,6 +819,23 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   dm.fallback = c->d_fallback;
   dm.wlist = c->d_wlist;
   dm.wcount = c->d_wcount;
+  return dm;
+}
+static KParams dev_params(const bf_ctx* c) {
+  KParams kp = c->kp;
+  kp.E = c->E;
+  kp.NS = c->NS;
+  kp.has_delay = c->has_delay ? 1 : 0;
+  return kp;
+}
+
+static int launch_eval(bf_ctx* c, const double* x, const double* y, const double* z,
+                       double* fint, int accumulate, int precision, hipStream_t s) {
+  if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
+  if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
+    return BF_ERR_ARG;
+  const KParams kp = dev_params(c);
+  const DevMesh dm = dev_mesh(c);
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
   hipEvent_t* ev = c->own_ev;
   if (c->ring_on) {
@@ -965,6 +1072,384 @@ int bf_sync(bf_ctx* c) {
   if (!c) return BF_ERR_ARG;
   BF_HIP(hipSetDevice(c->device));
   BF_HIP(hipStreamSynchronize(c->stream));
+  return BF_OK;
+}
+
+int bf_coords_upload(bf_ctx* c, const double* x, const double* y, const double* z) {
+  if (!c || !x || !y || !z) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  const size_t N = (size_t)c->n_nodes;
+  BF_HIP(hipMemcpyAsync(c->d_x, x, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  BF_HIP(hipMemcpyAsync(c->d_y, y, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  BF_HIP(hipMemcpyAsync(c->d_z, z, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  return BF_OK;
+}
+
+int bf_fint_download(bf_ctx* c, double* fint) {
+  if (!c || !fint) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  BF_HIP(hipMemcpyAsync(fint, c->d_fint, sizeof(double) * 3 * (size_t)c->n_nodes,
+                        hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  return BF_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches --------------
+struct bf_batch {
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<bf_ctx*> cases;
+  int P = 1;
+  // launch shape of the element phase
+  int passes = 4;       // element groups per work item: tile_cap = passes * floor(64 / P)
+  int block = 256;      // threads per block of the staged kernel
+  int blocks = 0;       // 0: persistent default (waves for every SIMD at the kernel's occupancy)
+  int persistent = 1;
+  // device state
+  CaseDev* d_cases = nullptr;
+  Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr;
+  int *d_tlist = nullptr, *d_tcount = nullptr, *d_htotal = nullptr;
+  int2* d_hlist = nullptr;
+  int n_etiles = 0, n_ntiles = 0, n_gtiles = 0, tile_cap = 0;
+  long long E_total = 0;
+  int* h_total = nullptr;  // pinned
+  // captured step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int g_prec = -1, g_ghosts = -1;
+};
+
+namespace {
+
+void batch_free_lists(bf_batch* b) {
+  dfree(b->d_cases); dfree(b->d_etiles); dfree(b->d_ntiles); dfree(b->d_gtiles);
+  dfree(b->d_tlist); dfree(b->d_tcount); dfree(b->d_hlist); dfree(b->d_htotal);
+  b->n_etiles = b->n_ntiles = b->n_gtiles = 0;
+}
+
+void batch_drop_graph(bf_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->g_prec = b->g_ghosts = -1;
+}
+
+// Rebuilds the case table and the element / node / ghost work lists.
+int batch_build(bf_batch* b) {
+  batch_drop_graph(b);
+  batch_free_lists(b);
+  const int nc = (int)b->cases.size();
+  if (nc == 0) return BF_OK;
+  int P = 1;
+  for (bf_ctx* c : b->cases) P = std::max(P, c->kp.n_qp);
+  b->P = P;
+  b->tile_cap = b->passes * (64 / P);
+  std::vector<CaseDev> cases((size_t)nc);
+  std::vector<Tile> et, nt, gt;
+  long long Et = 0;
+  for (int i = 0; i < nc; i++) {
+    bf_ctx* c = b->cases[i];
+    CaseDev& cd = cases[i];
+    cd.m = dev_mesh(c);
+    cd.kp = dev_params(c);
+    cd.x = c->d_x;
+    cd.y = c->d_y;
+    cd.z = c->d_z;
+    cd.fint = c->d_fint;
+    cd.ptr = c->d_ptr;
+    cd.ent = c->d_ent;
+    cd.rules = c->d_rules;
+    cd.n_v = c->n_v;
+    cd.n_nodes = c->n_nodes;
+    cd.n_ghost = c->n_ghost;
+    cd.pad = 0;
+    for (int e0 = 0; e0 < c->E; e0 += b->tile_cap) et.push_back(Tile{i, e0, std::min(b->tile_cap, c->E - e0), 0});
+    for (int n0 = 0; n0 < c->n_nodes; n0 += 256) nt.push_back(Tile{i, n0, std::min(256, c->n_nodes - n0), 0});
+    for (int g0 = 0; g0 < c->n_ghost; g0 += 256) gt.push_back(Tile{i, g0, std::min(256, c->n_ghost - g0), 0});
+    Et += c->E;
+  }
+  b->E_total = Et;
+  b->n_etiles = (int)et.size();
+  b->n_ntiles = (int)nt.size();
+  b->n_gtiles = (int)gt.size();
+  int rc;
+#define BALLOC(p, n) \
+  if ((rc = dalloc(&(p), (n))) != BF_OK) { batch_free_lists(b); return rc; }
+  BALLOC(b->d_cases, (size_t)nc);
+  BALLOC(b->d_etiles, et.size());
+  BALLOC(b->d_ntiles, nt.size());
+  BALLOC(b->d_gtiles, std::max<size_t>(gt.size(), 1));
+  BALLOC(b->d_tlist, (size_t)b->n_etiles * b->tile_cap);
+  BALLOC(b->d_tcount, (size_t)b->n_etiles);
+  BALLOC(b->d_hlist, (size_t)std::max<long long>(Et, 1));
+  BALLOC(b->d_htotal, 1);
+#undef BALLOC
+  hipStream_t s = b->stream;
+  BF_HIP(hipMemcpyAsync(b->d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(b->d_etiles, et.data(), sizeof(Tile) * et.size(), hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(b->d_ntiles, nt.data(), sizeof(Tile) * nt.size(), hipMemcpyHostToDevice, s));
+  if (!gt.empty())
+    BF_HIP(hipMemcpyAsync(b->d_gtiles, gt.data(), sizeof(Tile) * gt.size(), hipMemcpyHostToDevice, s));
+  BF_HIP(hipStreamSynchronize(s));
+  return BF_OK;
+}
+
+BatchDev batch_dev(const bf_batch* b) {
+  BatchDev d{};
+  d.cases = b->d_cases;
+  d.n_cases = (int)b->cases.size();
+  d.P = b->P;
+  d.etiles = b->d_etiles;
+  d.n_etiles = b->n_etiles;
+  d.tile_cap = b->tile_cap;
+  d.persistent = b->persistent;
+  d.ntiles = b->d_ntiles;
+  d.n_ntiles = b->n_ntiles;
+  d.gtiles = b->d_gtiles;
+  d.n_gtiles = b->n_gtiles;
+  d.tlist = b->d_tlist;
+  d.tcount = b->d_tcount;
+  d.hlist = b->d_hlist;
+  d.htotal = b->d_htotal;
+  return d;
+}
+
+// The kernels of one batched step, enqueued on the batch stream (captured or eager).
+int batch_enqueue(bf_batch* b, int precision, int ghosts) {
+  hipStream_t s = b->stream;
+  const BatchDev d = batch_dev(b);
+  const int pblocks = std::max(1, std::min(b->n_cu * 2, std::max(b->n_gtiles, (d.n_cases + 255) / 256)));
+  hipLaunchKernelGGL(bf_batch_prologue, dim3(pblocks), dim3(256), 0, s, d, ghosts);
+  const int wpb = b->block / 64;
+  if (precision == BF_PREC_FP64) {
+    // reference order: occupancy of one wave per SIMD, 2 waves per block
+    const int blocks = b->persistent ? (b->blocks > 0 ? b->blocks : b->n_cu * 2) : 0;
+    BF_HIP(launch_batch_exact(d, blocks, s));
+  } else {
+    // the staged kernels hold two waves per SIMD: 8 per CU
+    int blocks = b->blocks > 0 ? b->blocks : (b->n_cu * 8 + wpb - 1) / wpb;
+    if (!b->persistent) blocks = (b->n_etiles + wpb - 1) / wpb;
+    if (precision == BF_PREC_FP64_FAST)
+      hipLaunchKernelGGL(bf_batch_element_kernel<double>, dim3(blocks), dim3(b->block), 0, s, d);
+    else
+      hipLaunchKernelGGL(bf_batch_element_kernel<float>, dim3(blocks), dim3(b->block), 0, s, d);
+    hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d);
+    BF_HIP(launch_batch_exact_list(d, b->n_cu * 2, s));
+  }
+  hipLaunchKernelGGL(bf_batch_gather_kernel, dim3(b->n_ntiles), dim3(256), 0, s, d);
+  return kernel_launch_check();
+}
+
+}  // namespace
+
+extern "C" {
+
+int bf_batch_create(int device, bf_batch** out) {
+  if (!out) return BF_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  BF_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(device));
+  bf_batch* b = new (std::nothrow) bf_batch();
+  if (!b) return BF_ERR_NOMEM;
+  b->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    b->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_total, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    bf_batch_destroy(b);
+    return BF_ERR_DEVICE;
+  }
+  *out = b;
+  return BF_OK;
+}
+
+int bf_batch_destroy(bf_batch* b) {
+  if (!b) return BF_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  batch_free_lists(b);
+  if (b->h_total) (void)hipHostFree(b->h_total);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return BF_OK;
+}
+
+int bf_batch_set_cases(bf_batch* b, bf_ctx* const* ctxs, int n) {
+  if (!b || n < 0 || (n > 0 && !ctxs)) return BF_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    bf_ctx* c = ctxs[i];
+    if (!c || c->device != b->device) return BF_ERR_ARG;
+    if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
+  }
+  BF_HIP(hipSetDevice(b->device));
+  BF_HIP(hipStreamSynchronize(b->stream));
+  b->cases.assign(ctxs, ctxs + n);
+  return batch_build(b);
+}
+
+int bf_batch_refresh(bf_batch* b) {
+  if (!b) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(b->device));
+  BF_HIP(hipStreamSynchronize(b->stream));
+  return batch_build(b);
+}
+
+int bf_batch_set_launch(bf_batch* b, int passes, int block, int blocks, int persistent) {
+  if (!b || passes < 1 || passes > 64 || (block != 64 && block != 128 && block != 256) || blocks < 0)
+    return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(b->device));
+  BF_HIP(hipStreamSynchronize(b->stream));
+  b->passes = passes;
+  b->block = block;
+  b->blocks = blocks;
+  b->persistent = persistent ? 1 : 0;
+  return batch_build(b);
+}
+
+int bf_batch_get_launch(bf_batch* b, int* passes, int* block, int* blocks, int* persistent) {
+  if (!b) return BF_ERR_ARG;
+  if (passes) *passes = b->passes;
+  if (block) *block = b->block;
+  if (blocks) *blocks = b->blocks;
+  if (persistent) *persistent = b->persistent;
+  return BF_OK;
+}
+
+int bf_batch_eval(bf_batch* b, int precision, int ghosts, int use_graph) {
+  if (!b) return BF_ERR_ARG;
+  if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
+    return BF_ERR_ARG;
+  if (b->cases.empty()) return BF_OK;
+  BF_HIP(hipSetDevice(b->device));
+  BF_HIP(hipEventRecord(b->ev0, b->stream));
+  if (use_graph) {
+    if (!b->exec || b->g_prec != precision || b->g_ghosts != (ghosts ? 1 : 0)) {
+      batch_drop_graph(b);
+      BF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = batch_enqueue(b, precision, ghosts);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+      if (rc != BF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      BF_HIP(ec);
+      b->graph = g;
+      BF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->g_prec = precision;
+      b->g_ghosts = ghosts ? 1 : 0;
+    }
+    BF_HIP(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    const int rc = batch_enqueue(b, precision, ghosts);
+    if (rc != BF_OK) return rc;
+  }
+  BF_HIP(hipEventRecord(b->ev1, b->stream));
+  return BF_OK;
+}
+
+int bf_batch_sync(bf_batch* b) {
+  if (!b) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(b->device));
+  BF_HIP(hipStreamSynchronize(b->stream));
+  return BF_OK;
+}
+
+int bf_batch_last_ms(bf_batch* b, float* ms) {
+  if (!b || !ms) return BF_ERR_ARG;
+  BF_HIP(hipEventSynchronize(b->ev1));
+  BF_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
+  return BF_OK;
+}
+
+int bf_batch_last_fallback_count(bf_batch* b, int* n) {
+  if (!b || !n) return BF_ERR_ARG;
+  *n = 0;
+  if (b->cases.empty() || !b->d_htotal) return BF_OK;
+  BF_HIP(hipSetDevice(b->device));
+  BF_HIP(hipMemcpyAsync(b->h_total, b->d_htotal, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BF_HIP(hipStreamSynchronize(b->stream));
+  *n = *b->h_total;
+  return BF_OK;
+}
+
+int bf_batch_info(bf_batch* b, int* n_cases, long long* n_elmt, int* n_tiles, int* tile_cap) {
+  if (!b) return BF_ERR_ARG;
+  if (n_cases) *n_cases = (int)b->cases.size();
+  if (n_elmt) *n_elmt = b->E_total;
+  if (n_tiles) *n_tiles = b->n_etiles;
+  if (tile_cap) *tile_cap = b->tile_cap;
+  return BF_OK;
+}
+
+int bf_batch_tune(bf_batch* b, int precision, int reps, int* passes, int* block, int* blocks,
+                  int* persistent, float* best_ms) {
+  if (!b || reps < 1) return BF_ERR_ARG;
+  if (b->cases.empty()) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(b->device));
+  struct Cand { int passes, block, blocks, persistent; };
+  std::vector<Cand> cands;
+  const int pass_opts[] = {1, 2, 4, 8, 16};
+  const int block_opts[] = {64, 128, 256};
+  for (int ps : pass_opts)
+    for (int bl : block_opts) {
+      const int wpb = bl / 64;
+      cands.push_back({ps, bl, 0, 0});  // one tile per wave
+      for (int wps : {1, 2})            // persistent: waves per SIMD
+        cands.push_back({ps, bl, (b->n_cu * 4 * wps + wpb - 1) / wpb, 1});
+    }
+  float best = 1e30f;
+  Cand bc{b->passes, b->block, b->blocks, b->persistent};
+  for (const Cand& c : cands) {
+    b->passes = c.passes;
+    b->block = c.block;
+    b->blocks = c.blocks;
+    b->persistent = c.persistent;
+    int rc = batch_build(b);
+    if (rc != BF_OK) return rc;
+    rc = batch_enqueue(b, precision, 0);  // warm-up
+    if (rc != BF_OK) return rc;
+    BF_HIP(hipEventRecord(b->ev0, b->stream));
+    for (int r = 0; r < reps; r++) {
+      rc = batch_enqueue(b, precision, 0);
+      if (rc != BF_OK) return rc;
+    }
+    BF_HIP(hipEventRecord(b->ev1, b->stream));
+    BF_HIP(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    BF_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    ms /= (float)reps;
+    if (ms < best) {
+      best = ms;
+      bc = c;
+    }
+  }
+  b->passes = bc.passes;
+  b->block = bc.block;
+  b->blocks = bc.blocks;
+  b->persistent = bc.persistent;
+  const int rc = batch_build(b);
+  if (rc != BF_OK) return rc;
+  if (passes) *passes = bc.passes;
+  if (block) *block = bc.block;
+  if (blocks) *blocks = bc.blocks;
+  if (persistent) *persistent = bc.persistent;
+  if (best_ms) *best_ms = best;
   return BF_OK;
 }
 

@@ -84,7 +84,8 @@ struct WaveSink {
   int* list;    // [n_waves * per_wave]
   int* count;   // [n_waves]
   int per_wave;
-  __device__ void push(bool take, int e, int wave) const {
+  int wave;
+  __device__ void push(bool take, int e) const {
     const unsigned long long act = __ballot(true);
     const unsigned long long mask = __ballot(take);
     const int lane = (int)(threadIdx.x & 63u);
@@ -92,6 +93,55 @@ struct WaveSink {
     if (lane == leader) count[wave] = __popcll(mask);
     if (take) list[(size_t)wave * per_wave + __popcll(mask & ((1ull << lane) - 1ull))] = e;
   }
+};
+
+// The batched form: a wave works through one tile of a case in passes of floor(64 / P)
+// elements and appends to the tile's segment; `pushed` (wave-uniform) counts what earlier
+// passes appended.  Every lane active in a pass was active in all earlier passes of the
+// tile, so its copy of `pushed` is current.
+struct TileSink {
+  int* list;    // tile segment
+  int* pushed;
+  __device__ void push(bool take, int e) const {
+    const unsigned long long mask = __ballot(take);
+    const int lane = (int)(threadIdx.x & 63u);
+    if (take) list[*pushed + __popcll(mask & ((1ull << lane) - 1ull))] = e;
+    *pushed += __popcll(mask);
+  }
+};
+
+// ---- batches: many meshes (cases) evaluated by the same launches ----
+// One case as the batch kernels see it: its mesh, options and resident state.
+struct CaseDev {
+  DevMesh m;
+  KParams kp;
+  double *x, *y, *z;      // resident coordinates (the ghost phase writes ghost rows)
+  double* fint;           // resident interleaved forces [3 * n_nodes]
+  const int* ptr;         // node -> (element, slot) CSR
+  const unsigned* ent;
+  const int* rules;       // ghost rules [3 * n_ghost]
+  int n_v, n_nodes, n_ghost, pad;
+};
+// A contiguous range of one case's elements, nodes or ghosts.
+struct Tile {
+  int c, b, n, pad;
+};
+struct BatchDev {
+  const CaseDev* cases;
+  int n_cases;
+  int P;                  // lanes per element group (largest n_qp of the batch)
+  const Tile* etiles;     // element tiles of tile_cap elements
+  int n_etiles;
+  int tile_cap;
+  int persistent;         // 1: waves stride over the tiles; 0: one tile per wave
+  const Tile* ntiles;     // node tiles (gather)
+  int n_ntiles;
+  const Tile* gtiles;     // ghost tiles
+  int n_gtiles;
+  int* tlist;             // [n_etiles * tile_cap] hand-over segments
+  int* tcount;            // [n_etiles]
+  int2* hlist;            // compacted hand-over list: (case, element)
+  int* htotal;
 };
 
 __device__ inline V3<double> ld3(const double* __restrict__ x, const double* __restrict__ y,
@@ -129,5 +179,9 @@ hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const doubl
 // m.fallback[0 .. m.status[4]) (persistent grid, count read on the device).
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, hipStream_t s);
+// Batch forms: the reference-order kernel over the element tiles (plain FP64 batches) and
+// over the compacted hand-over list b.hlist[0 .. *b.htotal).
+hipError_t launch_batch_exact(const BatchDev& b, int blocks, hipStream_t s);
+hipError_t launch_batch_exact_list(const BatchDev& b, int blocks, hipStream_t s);
 
 }  // namespace bf

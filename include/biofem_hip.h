@@ -189,6 +189,51 @@ int bf_loop_tables(int v, double *out);
 
 const char *bf_version(void);
 
+/* Host copies to and from a context's resident buffers (the arrays bf_device_buffers
+ * returns): coordinates [n_nodes] each, forces [3 * n_nodes].  Synchronous.  No reference
+ * counterpart (the reference's arrays live in host memory). */
+int bf_coords_upload(bf_ctx *ctx, const double *x, const double *y, const double *z);
+int bf_fint_download(bf_ctx *ctx, double *fint);
+
+/* ---- batches: many meshes in the same launches ----
+ * A batch evaluates the force of every case (a context with a mesh and params) in one
+ * step of five launches -- status reset + GlobalGhost, the element phase (a persistent
+ * kernel that takes element tiles of all cases from a device work list), the compaction
+ * of hand-over lists, the reference-order hand-over kernel, the node gather -- optionally
+ * replayed from a captured hipGraph.  Each case reads its resident coordinates and writes
+ * its resident forces (accumulate = 0), as bf_force_eval_device on its own buffers would;
+ * per-case status, C33 and flags land in the case's context.  The batching of
+ * `biofem sweep` cases has no reference counterpart (the reference runs one case per
+ * process). */
+typedef struct bf_batch bf_batch;
+int bf_batch_create(int device, bf_batch **out);
+int bf_batch_destroy(bf_batch *batch);
+/* The cases of the batch (contexts on the batch's device, each with a mesh and params;
+ * they must stay alive while the batch uses them).  Rebuilds the work lists. */
+int bf_batch_set_cases(bf_batch *batch, bf_ctx *const *ctxs, int n);
+/* Re-reads the cases' params (after bf_params_set on a case). */
+int bf_batch_refresh(bf_batch *batch);
+/* One step for all cases; ghosts != 0 runs GlobalGhost on the resident coordinates first;
+ * use_graph != 0 captures the step once per (precision, ghosts) and replays it.
+ * Asynchronous on the batch's stream. */
+int bf_batch_eval(bf_batch *batch, int precision, int ghosts, int use_graph);
+int bf_batch_sync(bf_batch *batch);
+/* Device time of the last bf_batch_eval, in ms (synchronises). */
+int bf_batch_last_ms(bf_batch *batch, float *ms);
+/* Elements of all cases the last staged step recomputed in the reference's order. */
+int bf_batch_last_fallback_count(bf_batch *batch, int *n);
+int bf_batch_info(bf_batch *batch, int *n_cases, long long *n_elmt, int *n_tiles, int *tile_cap);
+/* Launch shape of the element phase: `passes` element groups per work item (a tile holds
+ * passes * floor(64 / n_qp) elements), threads per block (64, 128 or 256), blocks
+ * (0 = one wave per SIMD slot of the kernel's occupancy), persistent (1: waves stride over
+ * the work list, 0: one work item per wave). */
+int bf_batch_set_launch(bf_batch *batch, int passes, int block, int blocks, int persistent);
+int bf_batch_get_launch(bf_batch *batch, int *passes, int *block, int *blocks, int *persistent);
+/* Autotuner: times every launch shape of a fixed candidate set on the batch's own cases
+ * (reps steps each, precision as given) and keeps the fastest. */
+int bf_batch_tune(bf_batch *batch, int precision, int reps, int *passes, int *block,
+                  int *blocks, int *persistent, float *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

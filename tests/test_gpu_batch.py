@@ -1,0 +1,137 @@
+"""Batched evaluation (one persistent element launch over the work list of many meshes,
+hipGraph-captured steps, launch-shape autotuner): every case's forces equal its own
+single-mesh evaluation bit for bit, in every precision and launch shape."""
+import numpy as np
+import pytest
+
+from biofem_amd import _lib as L
+from biofem_amd import configs
+from biofem_amd.batch import Batch, upload_coords
+from biofem_amd.fem import ActStrainFE
+from oracle import bf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [L.BF_PREC_FP64, L.BF_PREC_FP64_FAST, L.BF_PREC_MIXED]
+
+
+def _cases(n, seed=2, nx_hi=40, mixed_qp=False):
+    cases = configs.batch_membranes(n=n, seed=seed, nx_lo=8, nx_hi=nx_hi)
+    if mixed_qp:  # lane groups sized by the largest n_qp; smaller cases leave lanes idle
+        cases[0][1].n_qp, cases[1][1].n_qp, cases[2][1].n_qp = 1, 5, 2
+        cases[3][1].c33_iters = 5
+    return cases
+
+
+def _single(cases, prec):
+    out = []
+    for m, p, (x, y, z) in cases:
+        fe = ActStrainFE(m, p)
+        out.append(fe.finternal_act(x, y, z, precision=prec))
+        fe.close()
+    return out
+
+
+def _resident(cases):
+    fes = []
+    for m, p, (x, y, z) in cases:
+        fe = ActStrainFE(m, p)
+        upload_coords(fe, x, y, z)
+        fes.append(fe)
+    return fes
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("mixed_qp", [False, True])
+def test_batch_equals_single(prec, mixed_qp):
+    cases = _cases(7, mixed_qp=mixed_qp)
+    print(f"case batch of {len(cases)} prec {prec} mixed_qp {mixed_qp}", flush=True)
+    ref = _single(cases, prec)
+    fes = _resident(cases)
+    b = Batch(fes)
+    try:
+        for graph in (False, True, True):
+            b.eval(prec, ghosts=False, graph=graph)
+            b.sync()
+            for i in range(len(fes)):
+                assert np.array_equal(b.forces(i), ref[i]), (graph, i)
+    finally:
+        b.close()
+        for fe in fes:
+            fe.close()
+
+
+def test_batch_plain_matches_oracle():
+    cases = _cases(3)
+    print("case batch plain vs oracle", flush=True)
+    fes = _resident(cases)
+    b = Batch(fes)
+    try:
+        b.eval(L.BF_PREC_FP64, graph=True)
+        b.sync()
+        for i, (m, p, (x, y, z)) in enumerate(cases):
+            ref, st, _ = O.finternal_act(m, p, x, y, z)
+            assert np.array_equal(b.forces(i), ref)
+    finally:
+        b.close()
+        for fe in fes:
+            fe.close()
+
+
+def test_batch_ghosts_and_graph_replays():
+    """Ghost rows are recomputed by the captured step; replays interleaved with eager
+    single-mesh work give the same forces every time."""
+    cases = _cases(5)
+    print("case batch ghosts + replays", flush=True)
+    ref = _single(cases, L.BF_PREC_MIXED)
+    fes = _resident(cases)
+    for fe, (m, p, (x, y, z)) in zip(fes, cases):
+        xg, yg, zg = x.copy(), y.copy(), z.copy()
+        xg[m.n_v:] = yg[m.n_v:] = zg[m.n_v:] = 123.0   # stale ghosts
+        upload_coords(fe, xg, yg, zg)
+    b = Batch(fes)
+    other = ActStrainFE(*cases[0][:2])
+    try:
+        for rep in range(4):
+            b.eval(L.BF_PREC_MIXED, ghosts=True, graph=True)
+            b.sync()
+            for i in range(len(fes)):
+                assert np.array_equal(b.forces(i), ref[i]), (rep, i)
+            other.finternal_act(*cases[0][2], precision=L.BF_PREC_FP64)
+    finally:
+        other.close()
+        b.close()
+        for fe in fes:
+            fe.close()
+
+
+def test_batch_tuner_and_launch_shapes():
+    cases = _cases(6, nx_hi=64)
+    print("case batch tuner", flush=True)
+    ref = _single(cases, L.BF_PREC_FP64_FAST)
+    fes = _resident(cases)
+    b = Batch(fes)
+    try:
+        for passes, block, persistent in ((1, 64, False), (8, 128, True), (3, 256, True)):
+            b.set_launch(passes, block, 0, persistent)
+            b.eval(L.BF_PREC_FP64_FAST, graph=False)
+            b.sync()
+            for i in range(len(fes)):
+                assert np.array_equal(b.forces(i), ref[i])
+        best = b.tune(L.BF_PREC_FP64_FAST, reps=2)
+        print("tuned", best, b.info(), flush=True)
+        assert best["ms"] > 0
+        b.eval(L.BF_PREC_FP64_FAST, graph=True)
+        b.sync()
+        for i in range(len(fes)):
+            assert np.array_equal(b.forces(i), ref[i])
+        # a smaller set of cases on the same batch
+        b.set_cases(fes[:2])
+        b.eval(L.BF_PREC_FP64_FAST, graph=True)
+        b.sync()
+        for i in range(2):
+            assert np.array_equal(b.forces(i), ref[i])
+    finally:
+        b.close()
+        for fe in fes:
+            fe.close()

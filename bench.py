@@ -147,6 +147,26 @@ def run_batched(args, L, log):
     return out
 
 
+def run_served(args, L, log):
+    """Served sweep: Poisson arrivals at 20 cases/s of flat / ellipsoid shells of 2k-400k
+    triangles (log-uniform), materials as in the batched workload, 50-500 explicit steps,
+    admitted by HBM footprint into continuous batches (FP32-mixed, hipGraph steps)."""
+    from biofem_amd.serve import Server, draw_sweep
+    specs = draw_sweep(n_cases=args.serve_cases, rate=20.0, seed=5)
+    budget = int(args.serve_budget_gib * 2**30)
+    log(f"served sweep: {len(specs)} cases over {specs[-1].arrival:.1f} s of arrivals, "
+        f"budget {args.serve_budget_gib} GiB")
+    srv = Server(hbm_budget=budget, steps_per_round=10, precision=L.BF_PREC_MIXED, log=log)
+    rep = srv.run(specs)
+    out = rep.summary()
+    out.update(workload=f"{len(specs)} cases, Poisson 20/s, flat or ellipsoid 2k-400k triangles "
+                        "(log-uniform, snapped to the cached mesh sizes), 50-500 explicit steps, "
+                        "FP32-mixed, continuous batching, 10 steps per round",
+               hbm_budget_gib=args.serve_budget_gib)
+    log(f"served sweep: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +182,9 @@ def main():
     ap.add_argument("--modes", default="fp64,fp64_ref_order,mixed")
     ap.add_argument("--batch", type=int, default=1, help="also run the batched 64-membrane workload")
     ap.add_argument("--batch-n", type=int, default=64)
+    ap.add_argument("--serve-cases", type=int, default=400,
+                    help="cases of the served sweep (2000 = the full sweep, ~100 s of arrivals)")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -253,6 +276,7 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: 3 x {args.copy_mib} MiB")
     batched = run_batched(args, L, log) if args.batch else None
+    served = run_served(args, L, log) if args.serve_cases > 0 else None
     if "fp64" not in res:
         print(json.dumps({k: v for k, v in res.items()}), flush=True)
         fe.close()
@@ -306,6 +330,7 @@ def main():
         "bytes": {"element_kernel": elem_bytes, "path": path_bytes, "overhead_element_buffer": overhead_bytes},
         "parity": parity,
         "batched": batched,
+        "served_sweep": served,
     }
     fe.close()
     print(json.dumps(out), flush=True)

@@ -62,11 +62,13 @@ class Mesh:
         for a in (x, y, z):
             a[idx] = a[g[:, 0]] + a[g[:, 1]] - a[g[:, 2]]
 
-    def footprint_bytes(self) -> int:
-        """Device bytes the library holds for this mesh (see bf_mesh_upload)."""
+    def footprint_bytes(self, state: bool = False) -> int:
+        """Device bytes the library holds for this mesh (bf_mesh_upload: patch, meta, geom0
+        cache, dA0, fibre, gamma scale, element-force buffer, C33, flags, hand-over lists,
+        node CSR; coordinates and forces), plus the time-stepping state when `state`."""
         E, N, S = self.n_elmt, self.n_nodes, max(12, int(self.nen().max()) if self.n_elmt else 12)
-        per_e = S * 4 + 4 + 15 * 8 + 8 + 24 + 8 + 3 * S * 8 + 5 * 8 + 12 * 4
-        per_n = 4 * 8 + 3 * 8 + 4
+        per_e = S * 4 + 4 + 15 * 8 + 8 + 24 + 8 + 3 * S * 8 + 5 * 8 + 1 + 12 * 4 + S * 4
+        per_n = 3 * 8 + 3 * 8 + 4 + (3 * 8 + 8 + 1 if state else 0)
         return E * per_e + N * per_n
 
     def desc(self):
@@ -226,6 +228,16 @@ def flat_membrane(nx: int, ny: int, Lx: float = 1.0, Ly: float = 0.75, jitter: f
     m.dA0 = area0(tri[:E], m.x0, m.y0, m.z0)
     m.extra.update(nx=nx, ny=ny, Lx=Lx, Ly=Ly)
     return m
+
+
+def lumped_mass(m: Mesh, rho: float, h0: float) -> np.ndarray:
+    """Row-sum lumped masses of the reference shell: each real triangle gives a third of
+    rho * h0 * dA0 to each of its corners (ghost nodes get 0)."""
+    mass = np.zeros(m.n_nodes)
+    share = rho * h0 * m.dA0 / 3.0
+    for k in range(3):
+        np.add.at(mass, m.tri[: m.n_elmt, k], share)
+    return mass
 
 
 def centroids(m: Mesh, x=None, y=None, z=None) -> np.ndarray:

@@ -278,6 +278,77 @@ hipError_t launch_batch_exact_list(const BatchDev& b, int blocks, hipStream_t s)
   return hipGetLastError();
 }
 
+// Node phase of one explicit damped time step (the substitute for the reference's
+// implicit step, see StepState): node n's force is gathered as in bf_gather_kernel, then
+//   v <- damp * v - (dt / m_n) * f,   x <- x + dt * v      (nodes not clamped)
+// in FP64 with every product rounded (no contraction), so a host restatement with the same
+// operations reproduces it bit for bit.  Ghost rows of fint are zero; ghost coordinates
+// follow from the rules at the next step's GlobalGhost.
+__device__ __forceinline__ void step_node(const int* __restrict__ ptr,
+                                          const unsigned* __restrict__ ent,
+                                          const double* __restrict__ Fe, int E, int n_v,
+                                          double* __restrict__ fint, double* x, double* y,
+                                          double* z, double* v, const double* winv,
+                                          const unsigned char* fixed, double dt, double damp,
+                                          int n) {
+  gather_node(ptr, ent, Fe, E, n_v, 0, fint, n);
+  if (n >= n_v || fixed[n]) return;
+  const double w = winv[n];
+  double* vn = v + 3 * (size_t)n;
+  const double vx = damp * vn[0] - w * fint[3 * (size_t)n + 0];
+  const double vy = damp * vn[1] - w * fint[3 * (size_t)n + 1];
+  const double vz = damp * vn[2] - w * fint[3 * (size_t)n + 2];
+  vn[0] = vx;
+  vn[1] = vy;
+  vn[2] = vz;
+  x[n] = x[n] + dt * vx;
+  y[n] = y[n] + dt * vy;
+  z[n] = z[n] + dt * vz;
+}
+
+__global__ void __launch_bounds__(256) bf_gather_step_kernel(const int* __restrict__ ptr,
+                                                             const unsigned* __restrict__ ent,
+                                                             const double* __restrict__ Fe,
+                                                             int E, int n_v, int n_nodes,
+                                                             double* __restrict__ fint,
+                                                             double* x, double* y, double* z,
+                                                             double* v, const double* winv,
+                                                             const unsigned char* fixed,
+                                                             double dt, double damp) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  step_node(ptr, ent, Fe, E, n_v, fint, x, y, z, v, winv, fixed, dt, damp, n);
+}
+
+__global__ void __launch_bounds__(256) bf_batch_gather_step_kernel(BatchDev b) {
+  const Tile tl = b.ntiles[blockIdx.x];
+  const CaseDev& cd = b.cases[tl.c];
+  const int i = threadIdx.x;
+  if (i >= tl.n) return;
+  const int n = tl.b + i;
+  if (cd.v)
+    step_node(cd.ptr, cd.ent, cd.m.Fe, cd.kp.E, cd.n_v, cd.fint, cd.x, cd.y, cd.z, cd.v,
+              cd.winv, cd.fixed, cd.dt, cd.damp, n);
+  else
+    gather_node(cd.ptr, cd.ent, cd.m.Fe, cd.kp.E, cd.n_v, 0, cd.fint, n);
+}
+
+hipError_t launch_gather_step(const int* ptr, const unsigned* ent, const double* Fe, int E,
+                              int n_v, int n_nodes, double* fint, double* x, double* y,
+                              double* z, double* v, const double* winv,
+                              const unsigned char* fixed, double dt, double damp, hipStream_t s) {
+  if (n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bf_gather_step_kernel, dim3((n_nodes + 255) / 256), dim3(256), 0, s, ptr,
+                     ent, Fe, E, n_v, n_nodes, fint, x, y, z, v, winv, fixed, dt, damp);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s) {
+  if (b.n_ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bf_batch_gather_step_kernel, dim3(b.n_ntiles), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_IN), tab, bytes);
 }

@@ -231,56 +231,6 @@ __global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(const int* __
   if (t == kCompactBlock - 1) status[4] = s_sum[t];
 }
 
-// Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + sum over the
-// node's (element, slot) entries in ascending element order.  Ghost rows (n >= n_v) are
-// zeroed when not accumulating, as after the caller's VecSet(Fint, 0).
-__device__ __forceinline__ void gather_node(const int* __restrict__ ptr,
-                                            const unsigned* __restrict__ ent,
-                                            const double* __restrict__ Fe, int E, int n_v,
-                                            int accumulate, double* __restrict__ fint, int n) {
-  if (n >= n_v) {
-    if (!accumulate) {
-      fint[3 * (size_t)n + 0] = 0.0;
-      fint[3 * (size_t)n + 1] = 0.0;
-      fint[3 * (size_t)n + 2] = 0.0;
-    }
-    return;
-  }
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  if (accumulate) {
-    ax = fint[3 * (size_t)n + 0];
-    ay = fint[3 * (size_t)n + 1];
-    az = fint[3 * (size_t)n + 2];
-  }
-  // entries are read kGatherChunk at a time (their loads overlap), then added in order
-  constexpr int kGatherChunk = 8;
-  const int b = ptr[n], en = ptr[n + 1];
-  for (int j0 = b; j0 < en; j0 += kGatherChunk) {
-    double vx[kGatherChunk], vy[kGatherChunk], vz[kGatherChunk];
-#pragma unroll
-    for (int k = 0; k < kGatherChunk; k++) {
-      if (j0 + k < en) {
-        const unsigned v = ent[j0 + k];
-        const int e = (int)(v >> 5), s = (int)(v & 31u);
-        vx[k] = Fe[(size_t)(3 * s + 0) * E + e];
-        vy[k] = Fe[(size_t)(3 * s + 1) * E + e];
-        vz[k] = Fe[(size_t)(3 * s + 2) * E + e];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kGatherChunk; k++) {
-      if (j0 + k < en) {
-        ax += vx[k];
-        ay += vy[k];
-        az += vz[k];
-      }
-    }
-  }
-  fint[3 * (size_t)n + 0] = ax;
-  fint[3 * (size_t)n + 1] = ay;
-  fint[3 * (size_t)n + 2] = az;
-}
-
 __global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ ptr,
                                                         const unsigned* __restrict__ ent,
                                                         const double* __restrict__ Fe, int E,
@@ -437,6 +387,12 @@ struct bf_ctx {
   int* d_fallback = nullptr;
   int* d_wlist = nullptr;
   int* d_wcount = nullptr;
+  // explicit time-stepping state (bf_state_init)
+  double* d_v = nullptr;
+  double* d_winv = nullptr;
+  unsigned char* d_fixed = nullptr;
+  double dt = 0.0, damp = 0.0;
+  bool has_state = false;
   int* d_ptr = nullptr;
   unsigned* d_ent = nullptr;
   int* d_rules = nullptr;
@@ -479,6 +435,8 @@ void free_mesh(bf_ctx* c) {
   dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
   dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
   dfree(c->d_flags); dfree(c->d_fallback); dfree(c->d_wlist); dfree(c->d_wcount);
+  dfree(c->d_v); dfree(c->d_winv); dfree(c->d_fixed);
+  c->has_state = false;
   dfree(c->d_ptr); dfree(c->d_ent); dfree(c->d_rules); dfree(c->d_x); dfree(c->d_y);
   dfree(c->d_z); dfree(c->d_fint);
   c->has_mesh = false;
@@ -830,7 +788,8 @@ static KParams dev_params(const bf_ctx* c) {
 }
 
 static int launch_eval(bf_ctx* c, const double* x, const double* y, const double* z,
-                       double* fint, int accumulate, int precision, hipStream_t s) {
+                       double* fint, int accumulate, int precision, hipStream_t s,
+                       bool step = false) {
   if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
   if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
     return BF_ERR_ARG;
@@ -868,7 +827,11 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, s));
   }
   BF_HIP(hipEventRecord(e2, s));
-  if (c->n_nodes > 0)
+  if (step)
+    BF_HIP(launch_gather_step(c->d_ptr, c->d_ent, c->d_Fe, c->E, c->n_v, c->n_nodes, c->d_fint,
+                              c->d_x, c->d_y, c->d_z, c->d_v, c->d_winv, c->d_fixed, c->dt,
+                              c->damp, s));
+  else if (c->n_nodes > 0)
     hipLaunchKernelGGL(bf_gather_kernel, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s,
                        c->d_ptr, c->d_ent, c->d_Fe, c->E, c->n_v, c->n_nodes, accumulate, fint);
   BF_HIP(hipEventRecord(e3, s));
@@ -1075,6 +1038,74 @@ int bf_sync(bf_ctx* c) {
   return BF_OK;
 }
 
+int bf_state_init(bf_ctx* c, const double* mass, const unsigned char* fixed, const double* v0,
+                  double dt, double damping) {
+  if (!c || !mass || !(dt > 0.0) || !(damping >= 0.0 && damping <= 1.0)) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  const size_t N = (size_t)c->n_nodes;
+  std::vector<double> winv(N, 0.0);
+  std::vector<unsigned char> fx(N, 0);
+  for (size_t n = 0; n < N; n++) {
+    fx[n] = fixed ? (fixed[n] ? 1 : 0) : 0;
+    if (n < (size_t)c->n_v && !fx[n]) {
+      if (!(mass[n] > 0.0)) return BF_ERR_ARG;
+      winv[n] = dt / mass[n];
+    }
+  }
+  dfree(c->d_v); dfree(c->d_winv); dfree(c->d_fixed);
+  c->has_state = false;
+  int rc;
+  if ((rc = dalloc(&c->d_v, 3 * N)) != BF_OK) return rc;
+  if ((rc = dalloc(&c->d_winv, N)) != BF_OK) return rc;
+  if ((rc = dalloc(&c->d_fixed, N)) != BF_OK) return rc;
+  hipStream_t s = c->stream;
+  if (v0)
+    BF_HIP(hipMemcpyAsync(c->d_v, v0, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
+  else
+    BF_HIP(hipMemsetAsync(c->d_v, 0, sizeof(double) * 3 * N, s));
+  BF_HIP(hipMemcpyAsync(c->d_winv, winv.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_fixed, fx.data(), N, hipMemcpyHostToDevice, s));
+  BF_HIP(hipStreamSynchronize(s));
+  c->dt = dt;
+  c->damp = damping;
+  c->has_state = true;
+  return BF_OK;
+}
+
+int bf_step(bf_ctx* c, int precision, int n_steps) {
+  if (!c || n_steps < 0) return BF_ERR_ARG;
+  if (!c->has_mesh || !c->has_params || !c->has_state) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  for (int k = 0; k < n_steps; k++) {
+    if (c->n_ghost > 0) {
+      hipLaunchKernelGGL(bf_ghost_kernel, dim3((c->n_ghost + 255) / 256), dim3(256), 0, s,
+                         c->d_rules, c->n_ghost, c->n_v, c->d_x, c->d_y, c->d_z);
+    }
+    const int rc = launch_eval(c, c->d_x, c->d_y, c->d_z, c->d_fint, 0, precision, s, true);
+    if (rc != BF_OK) return rc;
+  }
+  return BF_OK;
+}
+
+int bf_state_download(bf_ctx* c, double* x, double* y, double* z, double* v) {
+  if (!c) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  const size_t N = (size_t)c->n_nodes;
+  hipStream_t s = c->stream;
+  if (x) BF_HIP(hipMemcpyAsync(x, c->d_x, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  if (y) BF_HIP(hipMemcpyAsync(y, c->d_y, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  if (z) BF_HIP(hipMemcpyAsync(z, c->d_z, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  if (v) {
+    if (!c->has_state) return BF_ERR_STATE;
+    BF_HIP(hipMemcpyAsync(v, c->d_v, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
+  }
+  BF_HIP(hipStreamSynchronize(s));
+  return BF_OK;
+}
+
 int bf_coords_upload(bf_ctx* c, const double* x, const double* y, const double* z) {
   if (!c || !x || !y || !z) return BF_ERR_ARG;
   if (!c->has_mesh) return BF_ERR_STATE;
@@ -1123,7 +1154,7 @@ struct bf_batch {
   // captured step
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int g_prec = -1, g_ghosts = -1;
+  int g_prec = -1, g_ghosts = -1, g_step = -1;
 };
 
 namespace {
@@ -1139,7 +1170,7 @@ void batch_drop_graph(bf_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
-  b->g_prec = b->g_ghosts = -1;
+  b->g_prec = b->g_ghosts = b->g_step = -1;
 }
 
 // Rebuilds the case table and the element / node / ghost work lists.
@@ -1171,6 +1202,11 @@ int batch_build(bf_batch* b) {
     cd.n_nodes = c->n_nodes;
     cd.n_ghost = c->n_ghost;
     cd.pad = 0;
+    cd.v = c->has_state ? c->d_v : nullptr;
+    cd.winv = c->d_winv;
+    cd.fixed = c->d_fixed;
+    cd.dt = c->dt;
+    cd.damp = c->damp;
     for (int e0 = 0; e0 < c->E; e0 += b->tile_cap) et.push_back(Tile{i, e0, std::min(b->tile_cap, c->E - e0), 0});
     for (int n0 = 0; n0 < c->n_nodes; n0 += 256) nt.push_back(Tile{i, n0, std::min(256, c->n_nodes - n0), 0});
     for (int g0 = 0; g0 < c->n_ghost; g0 += 256) gt.push_back(Tile{i, g0, std::min(256, c->n_ghost - g0), 0});
@@ -1223,7 +1259,8 @@ BatchDev batch_dev(const bf_batch* b) {
 }
 
 // The kernels of one batched step, enqueued on the batch stream (captured or eager).
-int batch_enqueue(bf_batch* b, int precision, int ghosts) {
+// step: the node phase also advances the cases that hold a time-stepping state.
+int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
   hipStream_t s = b->stream;
   const BatchDev d = batch_dev(b);
   const int pblocks = std::max(1, std::min(b->n_cu * 2, std::max(b->n_gtiles, (d.n_cases + 255) / 256)));
@@ -1244,7 +1281,10 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts) {
     hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d);
     BF_HIP(launch_batch_exact_list(d, b->n_cu * 2, s));
   }
-  hipLaunchKernelGGL(bf_batch_gather_kernel, dim3(b->n_ntiles), dim3(256), 0, s, d);
+  if (step)
+    BF_HIP(launch_batch_gather_step(d, s));
+  else
+    hipLaunchKernelGGL(bf_batch_gather_kernel, dim3(b->n_ntiles), dim3(256), 0, s, d);
   return kernel_launch_check();
 }
 
@@ -1330,18 +1370,19 @@ int bf_batch_get_launch(bf_batch* b, int* passes, int* block, int* blocks, int* 
   return BF_OK;
 }
 
-int bf_batch_eval(bf_batch* b, int precision, int ghosts, int use_graph) {
-  if (!b) return BF_ERR_ARG;
+static int batch_run(bf_batch* b, int precision, int ghosts, bool step, int n, int use_graph) {
+  if (!b || n < 0) return BF_ERR_ARG;
   if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
     return BF_ERR_ARG;
   if (b->cases.empty()) return BF_OK;
   BF_HIP(hipSetDevice(b->device));
   BF_HIP(hipEventRecord(b->ev0, b->stream));
   if (use_graph) {
-    if (!b->exec || b->g_prec != precision || b->g_ghosts != (ghosts ? 1 : 0)) {
+    if (!b->exec || b->g_prec != precision || b->g_ghosts != (ghosts ? 1 : 0) ||
+        b->g_step != (step ? 1 : 0)) {
       batch_drop_graph(b);
       BF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = batch_enqueue(b, precision, ghosts);
+      const int rc = batch_enqueue(b, precision, ghosts, step);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(b->stream, &g);
       if (rc != BF_OK) {
@@ -1353,14 +1394,25 @@ int bf_batch_eval(bf_batch* b, int precision, int ghosts, int use_graph) {
       BF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->g_prec = precision;
       b->g_ghosts = ghosts ? 1 : 0;
+      b->g_step = step ? 1 : 0;
     }
-    BF_HIP(hipGraphLaunch(b->exec, b->stream));
+    for (int k = 0; k < n; k++) BF_HIP(hipGraphLaunch(b->exec, b->stream));
   } else {
-    const int rc = batch_enqueue(b, precision, ghosts);
-    if (rc != BF_OK) return rc;
+    for (int k = 0; k < n; k++) {
+      const int rc = batch_enqueue(b, precision, ghosts, step);
+      if (rc != BF_OK) return rc;
+    }
   }
   BF_HIP(hipEventRecord(b->ev1, b->stream));
   return BF_OK;
+}
+
+int bf_batch_eval(bf_batch* b, int precision, int ghosts, int use_graph) {
+  return batch_run(b, precision, ghosts, false, 1, use_graph);
+}
+
+int bf_batch_step(bf_batch* b, int precision, int n_steps, int use_graph) {
+  return batch_run(b, precision, 1, true, n_steps, use_graph);
 }
 
 int bf_batch_sync(bf_batch* b) {

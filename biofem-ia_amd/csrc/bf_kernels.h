@@ -121,6 +121,11 @@ struct CaseDev {
   const unsigned* ent;
   const int* rules;       // ghost rules [3 * n_ghost]
   int n_v, n_nodes, n_ghost, pad;
+  // time-stepping state (StepState); v == nullptr: force evaluation only
+  double* v;              // [3 * n_nodes] velocities
+  const double* winv;     // [n_nodes] dt / lumped mass
+  const unsigned char* fixed;  // [n_nodes] 1 = clamped (never moves)
+  double dt, damp;
 };
 // A contiguous range of one case's elements, nodes or ghosts.
 struct Tile {
@@ -149,6 +154,56 @@ __device__ inline V3<double> ld3(const double* __restrict__ x, const double* __r
   return mk(x[n], y[n], z[n]);
 }
 
+
+// Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + sum over the
+// node's (element, slot) entries in ascending element order.  Ghost rows (n >= n_v) are
+// zeroed when not accumulating, as after the caller's VecSet(Fint, 0).
+__device__ __forceinline__ void gather_node(const int* __restrict__ ptr,
+                                            const unsigned* __restrict__ ent,
+                                            const double* __restrict__ Fe, int E, int n_v,
+                                            int accumulate, double* __restrict__ fint, int n) {
+  if (n >= n_v) {
+    if (!accumulate) {
+      fint[3 * (size_t)n + 0] = 0.0;
+      fint[3 * (size_t)n + 1] = 0.0;
+      fint[3 * (size_t)n + 2] = 0.0;
+    }
+    return;
+  }
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (accumulate) {
+    ax = fint[3 * (size_t)n + 0];
+    ay = fint[3 * (size_t)n + 1];
+    az = fint[3 * (size_t)n + 2];
+  }
+  // entries are read kGatherChunk at a time (their loads overlap), then added in order
+  constexpr int kGatherChunk = 8;
+  const int b = ptr[n], en = ptr[n + 1];
+  for (int j0 = b; j0 < en; j0 += kGatherChunk) {
+    double vx[kGatherChunk], vy[kGatherChunk], vz[kGatherChunk];
+#pragma unroll
+    for (int k = 0; k < kGatherChunk; k++) {
+      if (j0 + k < en) {
+        const unsigned v = ent[j0 + k];
+        const int e = (int)(v >> 5), s = (int)(v & 31u);
+        vx[k] = Fe[(size_t)(3 * s + 0) * E + e];
+        vy[k] = Fe[(size_t)(3 * s + 1) * E + e];
+        vz[k] = Fe[(size_t)(3 * s + 2) * E + e];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kGatherChunk; k++) {
+      if (j0 + k < en) {
+        ax += vx[k];
+        ay += vy[k];
+        az += vz[k];
+      }
+    }
+  }
+  fint[3 * (size_t)n + 0] = ax;
+  fint[3 * (size_t)n + 1] = ay;
+  fint[3 * (size_t)n + 2] = az;
+}
 
 // Loop box-spline derivatives at the element centre, verbatim 4-decimal values of
 // src/active_strain.c:69-88.
@@ -182,6 +237,13 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
 // Batch forms: the reference-order kernel over the element tiles (plain FP64 batches) and
 // over the compacted hand-over list b.hlist[0 .. *b.htotal).
 hipError_t launch_batch_exact(const BatchDev& b, int blocks, hipStream_t s);
+// Node phase of a time step (one context): gather + explicit damped update, FP64 without
+// contraction.  Batch form: one block per node tile of every case.
+hipError_t launch_gather_step(const int* ptr, const unsigned* ent, const double* Fe, int E,
+                              int n_v, int n_nodes, double* fint, double* x, double* y,
+                              double* z, double* v, const double* winv,
+                              const unsigned char* fixed, double dt, double damp, hipStream_t s);
+hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s);
 hipError_t launch_batch_exact_list(const BatchDev& b, int blocks, hipStream_t s);
 
 }  // namespace bf

@@ -189,6 +189,21 @@ int bf_loop_tables(int v, double *out);
 
 const char *bf_version(void);
 
+/* ---- time stepping on resident state ----
+ * The reference advances FormFunctionFEM's residual with an implicit Newmark step solved by
+ * PETSc SNES/KSP (src/main.c:461-502, 757-1032), unavailable here.  Its substitute is an
+ * explicit damped step on the device, in FP64 without contraction:
+ *     f = Fint(x)  (the force of this library, GlobalGhost first),
+ *     v <- damping * v - (dt / m_n) f,   x <- x + dt v    (nodes not fixed),
+ * with lumped masses m_n supplied by the caller.  bf_state_init sets the state (v0 NULL =
+ * at rest; fixed[n] != 0 clamps node n, as EdgeDirectionalFix would); bf_step advances
+ * n_steps steps asynchronously on the context stream; bf_state_download copies x, y, z
+ * (n_nodes each) and v (3 * n_nodes, interleaved) back, any of them NULL to skip. */
+int bf_state_init(bf_ctx *ctx, const double *mass, const unsigned char *fixed,
+                  const double *v0, double dt, double damping);
+int bf_step(bf_ctx *ctx, int precision, int n_steps);
+int bf_state_download(bf_ctx *ctx, double *x, double *y, double *z, double *v);
+
 /* Host copies to and from a context's resident buffers (the arrays bf_device_buffers
  * returns): coordinates [n_nodes] each, forces [3 * n_nodes].  Synchronous.  No reference
  * counterpart (the reference's arrays live in host memory). */
@@ -217,6 +232,9 @@ int bf_batch_refresh(bf_batch *batch);
  * use_graph != 0 captures the step once per (precision, ghosts) and replays it.
  * Asynchronous on the batch's stream. */
 int bf_batch_eval(bf_batch *batch, int precision, int ghosts, int use_graph);
+/* n_steps time steps of every case that holds a state (bf_state_init before
+ * bf_batch_set_cases / bf_batch_refresh); cases without one are only evaluated. */
+int bf_batch_step(bf_batch *batch, int precision, int n_steps, int use_graph);
 int bf_batch_sync(bf_batch *batch);
 /* Device time of the last bf_batch_eval, in ms (synchronises). */
 int bf_batch_last_ms(bf_batch *batch, float *ms);

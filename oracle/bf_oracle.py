@@ -286,3 +286,34 @@ def element_force(m, p, ec, x, y, z):
                         C.byref(w))
     nen = 12 if m.ire[ec] == 0 else int(m.val[ec]) + 6
     return Fb[: 3 * nen], S[: 9 * p.n_qp].reshape(p.n_qp, 3, 3), c33[: p.n_qp], w.value
+
+
+def explicit_steps(m, p, x, y, z, mass, fixed, dt, damp, n_steps, v=None, nthreads=4):
+    """Host restatement of the library's explicit damped time step (bf_step), for tests:
+        GlobalGhost (x[n_v+g] = (x[b] + x[a]) - x[c]), f = finternal_act(x),
+        v <- damp * v - (dt / m_n) * f,  x <- x + dt * v   on free real nodes,
+    every product and sum rounded separately (numpy element-wise, no contraction).  This
+    step is the library's substitute for the reference's implicit Newmark/SNES step
+    (src/main.c:461-502), which needs PETSc: its parity against the reference is unpinned;
+    this function pins the GPU trajectory against the oracle's force."""
+    x, y, z = (np.array(a, dtype=np.float64, copy=True) for a in (x, y, z))
+    n, nv = m.n_nodes, m.n_v
+    v = np.zeros((n, 3)) if v is None else np.array(v, dtype=np.float64).reshape(n, 3).copy()
+    free = np.zeros(n, bool)
+    free[:nv] = ~np.asarray(fixed[:nv], bool)
+    w = np.zeros(n)
+    w[free] = dt / np.asarray(mass)[free]
+    cached = Cached(m, p, omp=True)   # -O3 build, bit-identical to the -O2 one
+    f = np.zeros(3 * n)
+    try:
+        for _ in range(n_steps):
+            m.apply_ghosts(x, y, z)
+            cached.eval(x, y, z, f, nthreads=nthreads)
+            fr = f.reshape(n, 3)
+            for c, a in enumerate((x, y, z)):
+                vc = damp * v[free, c] - w[free] * fr[free, c]
+                v[free, c] = vc
+                a[free] = a[free] + dt * vc
+    finally:
+        cached.close()
+    return x, y, z, v

@@ -391,18 +391,15 @@ def ellipsoid_dome(levels: int = 6, a: float = 2.2, b: float = 1.0, z_cut: float
         P, T, rim = _loop_refine(P, T, rim)
     # project onto the ellipsoid along (x/b, y/b, z/a), then stretch the polar angle
     # smoothly so that the rim lands on z = z_cut: th' = th (1 + (th_rim / th_b(ph) - 1) s),
-    # s = (th / th_b(ph))^2, th_b a Fourier fit of the subdivided rim's polar angle.
+    # s = (th / th_b(ph))^2, th_b the subdivided rim's polar angle interpolated linearly in
+    # ph between rim nodes (exact on the rim, so no interior node is pushed past it).
     q = P / np.array([b, b, a])
     q /= np.linalg.norm(q, axis=1, keepdims=True)
     th = np.arccos(np.clip(q[:, 2], -1.0, 1.0))
     ph = np.arctan2(q[:, 1], q[:, 0])
-    K = 12
-    A = np.concatenate([np.ones((len(rim), 1))] + [f(k * ph[rim])[:, None] for k in range(1, K + 1)
-                                                   for f in (np.cos, np.sin)], 1)
-    coef = np.linalg.lstsq(A, th[rim], rcond=None)[0]
-    Aall = np.concatenate([np.ones((len(ph), 1))] + [f(k * ph)[:, None] for k in range(1, K + 1)
-                                                     for f in (np.cos, np.sin)], 1)
-    thb = Aall @ coef
+    rim_idx = np.asarray(rim)
+    order = np.argsort(ph[rim_idx])
+    thb = np.interp(ph, ph[rim_idx][order], th[rim_idx][order], period=2.0 * np.pi)
     sfac = (th / thb) ** 2
     th2 = th * (1.0 + (th_rim / thb - 1.0) * sfac)
     th2[rim] = th_rim

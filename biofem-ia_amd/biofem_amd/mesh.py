@@ -67,7 +67,7 @@ class Mesh:
         cache, dA0, fibre, gamma scale, element-force buffer, C33, flags, hand-over lists,
         node CSR; coordinates and forces), plus the time-stepping state when `state`."""
         E, N, S = self.n_elmt, self.n_nodes, max(12, int(self.nen().max()) if self.n_elmt else 12)
-        per_e = S * 4 + 4 + 15 * 8 + 8 + 24 + 8 + 3 * S * 8 + 5 * 8 + 1 + 12 * 4 + S * 4
+        per_e = S * 4 + 4 + 15 * 8 + 8 + 24 + 8 + 3 * S * 8 + 5 * 8 + 1 + 12 * 4 + S * 4  # pos: S*4
         per_n = 3 * 8 + 3 * 8 + 4 + (3 * 8 + 8 + 1 if state else 0)
         return E * per_e + N * per_n
 

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E,
 // then the group exchanges the qp stresses by cross-lane reads and each lane writes the
 // forces of every P-th patch node, summing over qp in the reference's order.  The
 // CC3333 guard, which in the reference stops the qp loop of an iteration at the first
-// failing qp, becomes a wave ballot per iteration.  Writes the same slot-major element-force
+// failing qp, becomes a wave ballot per iteration.  Writes the same slot-row element-force
 // buffer as the staged kernels; the node gather then sums in element order
 // (active_strain.c:2238-2260).
 constexpr int kExactBlock = 128;
@@ -153,25 +153,25 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
                                                    pref0, half_h0, Sv, kp.theta, kp.w, n_qp,
                                                    kind == kIrregular);
       finite = finite && isfinite(f.x) && isfinite(f.y) && isfinite(f.z);
-      Fe[(size_t)(3 * a + 0) * E + e] = f.x;
-      Fe[(size_t)(3 * a + 1) * E + e] = f.y;
-      Fe[(size_t)(3 * a + 2) * E + e] = f.z;
+      put_force(Fe, e, E, a, f);
     }
   } else if (q == 0) {
-    for (int a = 0; a < nen; a++) {
-      Fe[(size_t)(3 * a + 0) * E + e] = 0.0;
-      Fe[(size_t)(3 * a + 1) * E + e] = 0.0;
-      Fe[(size_t)(3 * a + 2) * E + e] = 0.0;
-    }
+    // zeroed element vector, corner contributions added (:1807-1851)
     const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
-    for (int a = 0; a < 3; a++) {
-      const int s = m.corner[(size_t)(3 + a) * E + e];
-      if (s < 0) continue;
-      const V3<double> f = exact::cap_force_exact(dN[a][0], dN[a][1], in.cur, pref0, Sv, kp.w, n_qp);
-      finite = finite && isfinite(f.x) && isfinite(f.y) && isfinite(f.z);
-      Fe[(size_t)(3 * s + 0) * E + e] = f.x;
-      Fe[(size_t)(3 * s + 1) * E + e] = f.y;
-      Fe[(size_t)(3 * s + 2) * E + e] = f.z;
+    V3<double> fc[3];
+    int sl[3];
+    for (int k = 0; k < 3; k++) {
+      sl[k] = m.corner[(size_t)(3 + k) * E + e];
+      fc[k] = exact::cap_force_exact(dN[k][0], dN[k][1], in.cur, pref0, Sv, kp.w, n_qp);
+    }
+    for (int a = 0; a < nen; a++) {
+      V3<double> f = mk(0.0, 0.0, 0.0);
+      for (int k = 0; k < 3; k++)
+        if (sl[k] == a) {
+          f = mk(f.x + fc[k].x, f.y + fc[k].y, f.z + fc[k].z);
+          finite = finite && isfinite(fc[k].x) && isfinite(fc[k].y) && isfinite(fc[k].z);
+        }
+      put_force(Fe, e, E, a, f);
     }
   }
   // ---- element status from the group (non-finite from any lane's nodes) ----

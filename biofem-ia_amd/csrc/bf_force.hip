@@ -1,15 +1,19 @@
 // bf_force.hip -- gfx950 kernels and C ABI of the element internal-force path.
 //
 // Pipeline of one force evaluation (replaces FInternalAct, src/active_strain.c:2226-2267):
-//   1. bf_element_kernel<R>: one lane per element.  Gathers the element's Loop patch from
-//      SoA coordinates, forms the current midsurface geometry (FP64), runs the staged
-//      active-strain constitutive chain per through-thickness point in R, and writes the
-//      3*nen nodal contributions to an element-force buffer laid out slot-major
-//      (Fe[(3*slot + c) * E + e]), so that the 64 lanes of a wave store contiguously.
-//   2. bf_gather_kernel: one lane per node.  Sums the node's contributions in ascending
-//      (element, slot) order from a CSR list built at upload, in FP64.  This is the
-//      summation order of VecSetValues(ADD_VALUES) on one rank (active_strain.c:2238-2260),
-//      deterministic run to run, and uses no atomics.
+//   1. bf_element_kernel<R>: one group of n_qp lanes per element, one lane per
+//      through-thickness point.  Gathers the element's Loop patch from SoA coordinates, forms
+//      the current midsurface geometry (FP64), runs the staged active-strain constitutive
+//      chain of its qp in R, sums the qp resultants across the group in qp order, and writes
+//      the 3*nen nodal contributions to the slot rows of the element-force buffer.  Elements
+//      outside the well-conditioned regime are listed instead (per-wave segments, no
+//      atomics).
+//   2. bf_compact_kernel + bf_element_kernel_exact_list: the listed elements recomputed in
+//      the reference's operation order (bf_exact.hip).
+//   3. bf_gather_kernel: one lane per node.  Sums the node's (element, slot) contributions in
+//      ascending element order from a CSR list built at upload, in FP64: the summation order
+//      of VecSetValues(ADD_VALUES) on one rank (active_strain.c:2238-2260), deterministic run
+//      to run, without atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -150,26 +154,23 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
     for (int a = q; a < nen; a += n_qp) {
       const V3<double> f = node_force(na0[a], na1[a], nb0[a], nb1[a], nb2[a], G, pref0, hh, r.N0,
                                       r.N1, r.N2, r.M0, r.M1, r.M2);
-      Fe[(size_t)(3 * a + 0) * E + e] = f.x;
-      Fe[(size_t)(3 * a + 1) * E + e] = f.y;
-      Fe[(size_t)(3 * a + 2) * E + e] = f.z;
+      put_force(Fe, e, E, a, f);
     }
   } else if (q == 0) {
     // CST membrane into the patch slots holding nv1..nv3 (:1807-1851); other slots zero
-    for (int a = 0; a < nen; a++) {
-      Fe[(size_t)(3 * a + 0) * E + e] = 0.0;
-      Fe[(size_t)(3 * a + 1) * E + e] = 0.0;
-      Fe[(size_t)(3 * a + 2) * E + e] = 0.0;
-    }
     const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
-    for (int a = 0; a < 3; a++) {
-      const int sl = m.corner[(size_t)(3 + a) * E + e];
-      if (sl < 0) continue;
-      const V3<double> f = node_force(dN[a][0], dN[a][1], 0.0, 0.0, 0.0, G, pref0, hh, r.N0,
-                                      r.N1, r.N2, 0.0, 0.0, 0.0);
-      Fe[(size_t)(3 * sl + 0) * E + e] += f.x;
-      Fe[(size_t)(3 * sl + 1) * E + e] += f.y;
-      Fe[(size_t)(3 * sl + 2) * E + e] += f.z;
+    V3<double> fc[3];
+    int sl[3];
+    for (int k = 0; k < 3; k++) {
+      sl[k] = m.corner[(size_t)(3 + k) * E + e];
+      fc[k] = node_force(dN[k][0], dN[k][1], 0.0, 0.0, 0.0, G, pref0, hh, r.N0, r.N1, r.N2, 0.0,
+                         0.0, 0.0);
+    }
+    for (int a = 0; a < nen; a++) {
+      V3<double> f = mk(0.0, 0.0, 0.0);
+      for (int k = 0; k < 3; k++)
+        if (sl[k] == a) f = add(f, fc[k]);
+      put_force(Fe, e, E, a, f);
     }
   }
   // ---- status (group leader) ----
@@ -616,7 +617,8 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
       }
     }
   }
-  // ---- node -> (element, slot) CSR in ascending (element, slot) order (:2238-2260) ----
+  // ---- node -> (element, slot) CSR in ascending (element, slot) order (:2238-2260);
+  //      ghost nodes get no list (their rows are dropped, :2254) ----
   std::vector<int> ptr((size_t)m->n_v + 1, 0);
   for (int e = 0; e < E; e++)
     for (int i = 0; i < nenv[e]; i++) {

@@ -33,7 +33,7 @@ struct DevMesh {
   const double* fib;   // [3][E]
   const double* gscale;
   const double* delay;
-  double* Fe;          // [3*NS][E]
+  double* Fe;          // [NS][E][3] element contributions: slot rows of (x, y, z)
   double* c33;         // [n_qp][E]
   int* status;         // [8] first error elem, code, first warn elem, warn count, fallback count
   unsigned char* flags;  // [E] per-element status bits of the last evaluation
@@ -117,8 +117,8 @@ struct CaseDev {
   KParams kp;
   double *x, *y, *z;      // resident coordinates (the ghost phase writes ghost rows)
   double* fint;           // resident interleaved forces [3 * n_nodes]
-  const int* ptr;         // node -> (element, slot) CSR
-  const unsigned* ent;
+  const int* ptr;         // node -> range of its (element, slot) entries in ent
+  const unsigned* ent;    // element << 5 | slot, ascending per node
   const int* rules;       // ghost rules [3 * n_ghost]
   int n_v, n_nodes, n_ghost, pad;
   // time-stepping state (StepState); v == nullptr: force evaluation only
@@ -155,9 +155,14 @@ __device__ inline V3<double> ld3(const double* __restrict__ x, const double* __r
 }
 
 
-// Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + sum over the
-// node's (element, slot) entries in ascending element order.  Ghost rows (n >= n_v) are
-// zeroed when not accumulating, as after the caller's VecSet(Fint, 0).
+// Node-ordered deterministic assembly: fint[3n+c] = (accumulate ? fint : 0) + the node's
+// (element, slot) contributions in ascending element order, from the list
+// ent[ptr[n] .. ptr[n+1]) built at upload: the summation order of VecSetValues(ADD_VALUES)
+// on one rank (active_strain.c:2238-2260).  The element-force buffer holds one row per
+// patch slot with the (x, y, z) of every element side by side: a wave's stores for one
+// slot cover consecutive elements, and each entry the gather reads is one 24-byte piece.
+// Ghost rows (n >= n_v) are zeroed when not accumulating, as after the caller's
+// VecSet(Fint, 0).
 __device__ __forceinline__ void gather_node(const int* __restrict__ ptr,
                                             const unsigned* __restrict__ ent,
                                             const double* __restrict__ Fe, int E, int n_v,
@@ -185,10 +190,10 @@ __device__ __forceinline__ void gather_node(const int* __restrict__ ptr,
     for (int k = 0; k < kGatherChunk; k++) {
       if (j0 + k < en) {
         const unsigned v = ent[j0 + k];
-        const int e = (int)(v >> 5), s = (int)(v & 31u);
-        vx[k] = Fe[(size_t)(3 * s + 0) * E + e];
-        vy[k] = Fe[(size_t)(3 * s + 1) * E + e];
-        vz[k] = Fe[(size_t)(3 * s + 2) * E + e];
+        const double* f = Fe + 3 * ((size_t)(v & 31u) * E + (v >> 5));
+        vx[k] = f[0];
+        vy[k] = f[1];
+        vz[k] = f[2];
       }
     }
 #pragma unroll
@@ -203,6 +208,15 @@ __device__ __forceinline__ void gather_node(const int* __restrict__ ptr,
   fint[3 * (size_t)n + 0] = ax;
   fint[3 * (size_t)n + 1] = ay;
   fint[3 * (size_t)n + 2] = az;
+}
+
+// Slot a of element e in the element-force buffer.
+__device__ __forceinline__ void put_force(double* __restrict__ Fe, int e, int E, int a,
+                                          const V3<double>& f) {
+  double* d = Fe + 3 * ((size_t)a * E + e);
+  d[0] = f.x;
+  d[1] = f.y;
+  d[2] = f.z;
 }
 
 // Loop box-spline derivatives at the element centre, verbatim 4-decimal values of

@@ -196,6 +196,8 @@ __global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact(DevMesh m
   QpGroup g;
   int slot;
   lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
+  // natural block order: the XCD-contiguous remap used by the memory-bound kernels measured
+  // 16 % slower here (this kernel is ALU-bound and gains nothing from L2 locality)
   const int wave = blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6);
   g.e = wave * per_wave + slot;
   if (slot >= per_wave || g.e >= kp.E) return;
@@ -228,9 +230,9 @@ __global__ void __launch_bounds__(kExactBlock) bf_batch_exact_kernel(BatchDev b)
   QpGroup g;
   int slot;
   lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
+  // natural order over the work list (see bf_element_kernel_exact)
   const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  const int n_waves = gridDim.x * (blockDim.x >> 6);
-  const int stride = b.persistent ? n_waves : b.n_etiles;
+  const int stride = b.persistent ? (int)(gridDim.x * (blockDim.x >> 6)) : b.n_etiles;
   for (int t = wave; t < b.n_etiles; t += stride) {
     const Tile tl = b.etiles[t];
     const int c = __builtin_amdgcn_readfirstlane(tl.c);
@@ -315,13 +317,13 @@ __global__ void __launch_bounds__(256) bf_gather_step_kernel(const int* __restri
                                                              double* v, const double* winv,
                                                              const unsigned char* fixed,
                                                              double dt, double damp) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = logical_block() * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   step_node(ptr, ent, Fe, E, n_v, fint, x, y, z, v, winv, fixed, dt, damp, n);
 }
 
 __global__ void __launch_bounds__(256) bf_batch_gather_step_kernel(BatchDev b) {
-  const Tile tl = b.ntiles[blockIdx.x];
+  const Tile tl = b.ntiles[logical_block()];
   const CaseDev& cd = b.cases[tl.c];
   const int i = threadIdx.x;
   if (i >= tl.n) return;

@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(kStagedBlock) bf_element_kernel(DevMesh m, KPa
   const int P = kp.n_qp, per_wave = 64 / P;
   int slot, q, base;
   lane_group(P, threadIdx.x & 63, slot, q, base);
-  const int wave = blockIdx.x * (kStagedBlock / 64) + (threadIdx.x >> 6);
+  const int wave = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
   const int e = wave * per_wave + slot;
   if (slot >= per_wave || e >= kp.E) return;
   staged_element<R>(m, kp, x, y, z, e, q, base, WaveSink{m.wlist, m.wcount, per_wave, wave});
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ 
                                                         const double* __restrict__ Fe, int E,
                                                         int n_v, int n_nodes, int accumulate,
                                                         double* __restrict__ fint) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = logical_block() * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   gather_node(ptr, ent, Fe, E, n_v, accumulate, fint, n);
 }
@@ -277,10 +277,9 @@ __global__ void __launch_bounds__(kStagedBlock) bf_batch_element_kernel(BatchDev
   const int P = b.P, per_wave = 64 / P;
   int slot, q, base;
   lane_group(P, threadIdx.x & 63, slot, q, base);
-  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  const int n_waves = gridDim.x * (blockDim.x >> 6);
-  const int stride = b.persistent ? n_waves : b.n_etiles;
-  for (int t = wave; t < b.n_etiles; t += stride) {
+  int t0, t1, step;
+  batch_tile_range(b, t0, t1, step);
+  for (int t = t0; t < t1; t += step) {
     const Tile tl = b.etiles[t];
     const int c = __builtin_amdgcn_readfirstlane(tl.c);
     const int e0 = __builtin_amdgcn_readfirstlane(tl.b);
@@ -325,7 +324,7 @@ __global__ void __launch_bounds__(kCompactBlock) bf_batch_compact_kernel(BatchDe
 
 // Node gather of a batch: one block per node tile.
 __global__ void __launch_bounds__(256) bf_batch_gather_kernel(BatchDev b) {
-  const Tile tl = b.ntiles[blockIdx.x];
+  const Tile tl = b.ntiles[logical_block()];
   const CaseDev& cd = b.cases[tl.c];
   const int i = threadIdx.x;
   if (i >= tl.n) return;

@@ -66,6 +66,20 @@ __device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) 
   return p.gamma * s * s;
 }
 
+// XCD-aware block order.  The dispatcher hands consecutive workgroups to the 8 XCDs in
+// turn, and each XCD has its own L2; neighbouring elements share patch nodes and
+// neighbouring nodes share element entries, so the logical block index is remapped to give
+// each XCD one contiguous range of blocks (a bijection on [0, n)).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n / kXcds, r = n % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+__device__ __forceinline__ int logical_block() {
+  return xcd_block((int)blockIdx.x, (int)gridDim.x);
+}
+
 // Lane -> (group slot in the wave, q, first lane of the group) for groups of P adjacent
 // lanes (one element per group, one through-thickness point per lane; floor(64 / P)
 // groups per wave, the remaining lanes idle).
@@ -237,6 +251,35 @@ static __constant__ double c_Nab[3][12] = {
 // Each kernel translation unit holds its own copy (no relocatable device code).
 static __constant__ double c_IN[13][5][kMaxNen];
 
+
+// The element tiles this wave works on, [t0, t1) with stride `step`.  Persistent launches
+// give each XCD one contiguous share of the work list, strided over that XCD's waves; a
+// launch with one tile per wave takes the tile of its XCD-remapped wave index.
+__device__ __forceinline__ void batch_tile_range(const BatchDev& b, int& t0, int& t1, int& step) {
+  const int wpb = (int)(blockDim.x >> 6);
+  const int wib = (int)(threadIdx.x >> 6);
+  if (!b.persistent) {
+    const int w = __builtin_amdgcn_readfirstlane(logical_block() * wpb + wib);
+    t0 = w;
+    t1 = (w < b.n_etiles) ? w + 1 : w;
+    step = 1;
+    return;
+  }
+  const int nb = (int)gridDim.x, bid = (int)blockIdx.x;
+  if (nb < kXcds) {  // too few blocks to give every XCD a share: plain stride
+    t0 = __builtin_amdgcn_readfirstlane(bid * wpb + wib);
+    t1 = b.n_etiles;
+    step = nb * wpb;
+    return;
+  }
+  const int x = bid % kXcds, k = bid / kXcds;
+  const int nbx = nb / kXcds + (x < nb % kXcds ? 1 : 0);     // blocks on this XCD
+  const long long T = b.n_etiles;
+  const int lo = (int)(T * x / kXcds), hi = (int)(T * (x + 1) / kXcds);
+  t0 = __builtin_amdgcn_readfirstlane(lo + k * wpb + wib);
+  t1 = hi;
+  step = nbx * wpb;
+}
 
 // Launchers of the kernels compiled without floating-point contraction (bf_exact.hip).
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes);

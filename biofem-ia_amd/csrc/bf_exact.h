@@ -104,6 +104,16 @@ BF_HD void col3(const Mat& M, int j, double c[3]) {
   c[2] = sel3(j, M.m[2][0], M.m[2][1], M.m[2][2]);
 }
 
+// Zero-skip guard of the CC^{2222} sum: factors below this magnitude cannot overflow the
+// 8-factor products, so a zero factor makes an exact zero term.
+constexpr double kZeroSkipMax = 1e25;
+BF_HD bool bounded(const Mat& A) {
+  bool ok = true;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) ok = ok && fabs(A.m[i][j]) < kZeroSkipMax;
+  return ok;
+}
+
 // Returns v unchanged, but the device compiler can no longer see that it is a constant:
 // products that start from it stay inside the loop that calls this instead of being
 // hoisted out of it and kept live (81 hoisted FP64 products per contraction would
@@ -255,6 +265,16 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
     const double beta = 0.5 * in.K * Je * (2.0 * Je - 1.0);
     const double b2 = 2.0 * beta;
     // ---- ElemTotTangMatTens :1603-1643, [2][2][2][2]; loop order g,h,m,n,p,s,w,z ----
+    // Blocks of terms with an exactly zero outer factor (Fi[g][h], G[m][2], Fi[n][m]) are
+    // skipped when every factor is finite and below 1e25 in magnitude: each such term is
+    // then an exact (signed) zero, and CC, which starts at +0 and can never become -0 under
+    // round-to-nearest addition, is unchanged by it -- the sum stays bit for bit the
+    // reference's.  (Flat reference shells have G[a][2] = 0 and Fa_inv.Cov[a][2] =
+    // Fa_inv.Cov[2][a] = 0 at the mid-surface point, theta = 0: 5 of the 81 blocks remain.
+    // The inner factors are left alone: a branch there would split the unrolled block and
+    // cost more than it saves.)
+    const bool zskip = bounded(Fi) && bounded(G) && bounded(Ch) && fabs(alpha) < kZeroSkipMax &&
+                       fabs(beta) < kZeroSkipMax;
     double CC = 0.0;
 #pragma unroll 1
     for (int gg = 0; gg < 3; gg++) {
@@ -267,9 +287,11 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
         double Chr[3];
         row3(Ch, h, Chr);    // Ch[h][*]
         const double Figh = sel3(h, Fgr[0], Fgr[1], Fgr[2]);
+        if (zskip && Figh == 0.0) continue;
 #pragma unroll 1
         for (int mm = 0; mm < 3; mm++) {
           const double Gm2 = sel3(mm, G.m[0][2], G.m[1][2], G.m[2][2]);
+          if (zskip && Gm2 == 0.0) continue;
           double Fmc[3];
           col3(Fi, mm, Fmc);  // Fi[*][m]
 #pragma unroll 1
@@ -277,6 +299,7 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
             double Chc[3];
             col3(Ch, n, Chc);  // Ch[*][n]
             const double Finm = sel3(n, Fmc[0], Fmc[1], Fmc[2]);
+            if (zskip && Finm == 0.0) continue;
             const double Ggn = sel3(n, Gr[0], Gr[1], Gr[2]);
             const double Chhn = sel3(n, Chr[0], Chr[1], Chr[2]);
             const double quarter = opaque(0.25);
@@ -286,7 +309,7 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
               const double e_np2 = b2 * Chhn * Ch.m[pp][2] - alpha * (Chr[pp] * Chc[2] + Chr[2] * Chc[pp]);
               const double e_2p2 = b2 * Chr[2] * Ch.m[pp][2] - alpha * (Chr[pp] * Ch.m[2][2] + Chr[2] * Ch.m[pp][2]);
 #pragma unroll
-              for (int s = 0; s < 3; s++)
+              for (int s = 0; s < 3; s++) {
 #pragma unroll
                 for (int w = 0; w < 3; w++)
 #pragma unroll
@@ -297,6 +320,7 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
                           (Gg2 * (e_npz * G.m[w][2] + e_np2 * G.m[w][z]) +
                            Ggn * (e_2pz * G.m[w][2] + e_2p2 * G.m[w][z]));
                   }
+              }
             }
           }
         }

@@ -44,51 +44,54 @@ __global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E,
 }
 
 
-// Plain FP64 element kernel: ElemUpdateGeomSubdiv + ElemUpdateG + ElemActDefGrad +
-// ElemCGDefTens + ElemC33Solve + ElemUpdFint, every operation in the order of
-// src/active_strain.c.  An element occupies a group of P adjacent lanes of one wave (P = the
-// largest n_qp of the launch; floor(64 / P) groups per wave), lane q of the group running
-// through-thickness point q: each lane forms the element geometry, runs its qp's C33 loop,
-// then the group exchanges the qp stresses by cross-lane reads and each lane writes the
-// forces of every P-th patch node, summing over qp in the reference's order.  The
-// CC3333 guard, which in the reference stops the qp loop of an iteration at the first
-// failing qp, becomes a wave ballot per iteration.  Writes the same slot-row element-force
-// buffer as the staged kernels; the node gather then sums in element order
-// (active_strain.c:2238-2260).
+// ---- reference-order element path ---------------------------------------------------------
+// ElemUpdateGeomSubdiv + ElemUpdateG + ElemActDefGrad + ElemCGDefTens + ElemC33Solve +
+// ElemUpdFint, every operation in the order of src/active_strain.c, in three kernels:
+//   1. solve (qp-major): each wave runs ONE through-thickness point q for 64 elements and
+//      stores that qp's stresses, C33 and status.  Waves are qp-uniform, so structural
+//      zeros of the contraction (a flat reference shell at its mid-surface point) are
+//      skipped by whole waves, and cheap and expensive waves interleave on the machine.
+//      The qps run independently: the CC3333 guard's cross-qp rule (ModElemC33 returns at
+//      the first failing qp, :1674-1681) is not applied here, a failing element is only
+//      flagged.
+//   2. finish: one lane per element gathers the qp stresses and writes the nodal forces
+//      (node sums over qp in the reference's order), flags and status; flagged elements are
+//      listed for
+//   3. redo: the element again with its qps in one lane group, the guard's cross-qp rule
+//      as a wave ballot per iteration (exact_element).
+// The forces of every element are then bit for bit the reference's.
 constexpr int kExactBlock = 128;
 
-struct QpGroup {
-  int e;      // element of this lane's group, -1 when the lane has no work
-  int q;      // through-thickness point of this lane
-  int base;   // wave lane of the group's q = 0
-};
-
-__device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& kp,
-                                              const double* __restrict__ x,
-                                              const double* __restrict__ y,
-                                              const double* __restrict__ z, const QpGroup& grp,
-                                              unsigned char extra_flags) {
-  const int E = kp.E;
-  const int e = grp.e, q = grp.q;
-  const int n_qp = kp.n_qp;
-  if (q >= n_qp) return;  // lanes beyond this element's qps (mixed n_qp in one launch)
-  const int meta = m.meta[e];
-  const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
-  auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
-  exact::ElemIn in;
+// Midsurface geometry, shape-function rows and all inputs of the C33 loop of element e.
+struct ExactRows {
   const double *na0, *na1, *nb0, *nb1, *nb2;
-  if (kind == kRegular) {
-    na0 = c_Na[0]; na1 = c_Na[1]; nb0 = c_Nab[0]; nb1 = c_Nab[1]; nb2 = c_Nab[2];
-    in.cur = mid_geom(12, na0, na1, nb0, nb1, nb2, xyz);
-  } else if (kind == kIrregular) {
+  int kind, nen;
+};
+__device__ __forceinline__ ExactRows exact_inputs(const DevMesh& m, const KParams& kp,
+                                                  const double* __restrict__ x,
+                                                  const double* __restrict__ y,
+                                                  const double* __restrict__ z, int e,
+                                                  exact::ElemIn& in, bool full) {
+  const int E = kp.E;
+  const int meta = m.meta[e];
+  ExactRows r;
+  r.kind = meta & 3;
+  const int val = (meta >> 8) & 255;
+  r.nen = (meta >> 16) & 255;
+  auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
+  if (r.kind == kRegular) {
+    r.na0 = c_Na[0]; r.na1 = c_Na[1]; r.nb0 = c_Nab[0]; r.nb1 = c_Nab[1]; r.nb2 = c_Nab[2];
+    in.cur = mid_geom(12, r.na0, r.na1, r.nb0, r.nb1, r.nb2, xyz);
+  } else if (r.kind == kIrregular) {
     const double(*t)[kMaxNen] = c_IN[val];
-    na0 = t[0]; na1 = t[1]; nb0 = t[2]; nb1 = t[3]; nb2 = t[4];
-    in.cur = mid_geom(nen, na0, na1, nb0, nb1, nb2, xyz);
+    r.na0 = t[0]; r.na1 = t[1]; r.nb0 = t[2]; r.nb1 = t[3]; r.nb2 = t[4];
+    in.cur = mid_geom(r.nen, r.na0, r.na1, r.nb0, r.nb1, r.nb2, xyz);
   } else {
-    na0 = na1 = nb0 = nb1 = nb2 = nullptr;
+    r.na0 = r.na1 = r.nb0 = r.nb1 = r.nb2 = nullptr;
     in.cur = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),
                       ld3(x, y, z, m.corner[(size_t)2 * E + e]));
   }
+  if (!full) return r;
   a3_alpha(in.cur.a1, in.cur.a2, in.cur.Aaa, in.cur.Aab, in.cur.Abb, in.a31c, in.a32c);
   const double* rf = m.ref;
   in.ref.a1 = mk(rf[(size_t)0 * E + e], rf[(size_t)1 * E + e], rf[(size_t)2 * E + e]);
@@ -117,17 +120,139 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
   in.h0 = kp.h0;
   in.mu = kp.mu;
   in.K = kp.K;
-  in.n_qp = n_qp;
+  in.n_qp = kp.n_qp;
   in.iters = kp.iters;
   in.theta = kp.theta;
   in.wq = kp.w;
-  // ---- this lane's qp; the guard of iteration it skips qp q if any qp <= q failed ----
+  return r;
+}
+
+// ElemUpdFint of element e from its qp stresses Sv (:1799-1801, :1807-1851 for the cap):
+// nodal forces into the element-force buffer; returns false if any is not finite.
+__device__ __forceinline__ bool exact_forces(const DevMesh& m, const KParams& kp, int e,
+                                             const ExactRows& r, const MidGeom& cur,
+                                             const double (*Sv)[3], int a0, int astep) {
+  const int E = kp.E;
+  const double A0 = m.dA0[e];
+  const double half_h0 = 0.5 * kp.h0;
+  const double pref0 = 0.5 * kp.h0 * A0;
+  double* Fe = m.Fe;
+  bool finite = true;
+  if (r.kind != kCap) {
+    for (int a = a0; a < r.nen; a += astep) {
+      const V3<double> f = exact::node_force_exact(r.na0[a], r.na1[a], r.nb0[a], r.nb1[a],
+                                                   r.nb2[a], cur, pref0, half_h0, Sv, kp.theta,
+                                                   kp.w, kp.n_qp, r.kind == kIrregular);
+      finite = finite && isfinite(f.x) && isfinite(f.y) && isfinite(f.z);
+      put_force(Fe, e, E, a, f);
+    }
+  } else if (a0 == 0) {
+    // zeroed element vector, corner contributions added
+    const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+    V3<double> fc[3];
+    int sl[3];
+    for (int k = 0; k < 3; k++) {
+      sl[k] = m.corner[(size_t)(3 + k) * E + e];
+      fc[k] = exact::cap_force_exact(dN[k][0], dN[k][1], cur, pref0, Sv, kp.w, kp.n_qp);
+    }
+    for (int a = 0; a < r.nen; a++) {
+      V3<double> f = mk(0.0, 0.0, 0.0);
+      for (int k = 0; k < 3; k++)
+        if (sl[k] == a) {
+          f = mk(f.x + fc[k].x, f.y + fc[k].y, f.z + fc[k].z);
+          finite = finite && isfinite(fc[k].x) && isfinite(fc[k].y) && isfinite(fc[k].z);
+        }
+      put_force(Fe, e, E, a, f);
+    }
+  }
+  return finite;
+}
+
+// Final per-element bookkeeping (flags, first non-finite element, CC3333 count).
+__device__ __forceinline__ void exact_status(const DevMesh& m, int e, int status, bool finite,
+                                             unsigned char extra_flags) {
+  if (!finite) atomicMin(&m.status[0], e);
+  m.flags[e] = (unsigned char)(status | (finite ? 0 : kStatNonFinite) | extra_flags);
+  if (status & kStatCC3333) {
+    atomicMin(&m.status[2], e);
+    atomicAdd(&m.status[3], 1);
+  }
+}
+
+// 1. solve: qp q of element e, qps independent.
+__device__ __forceinline__ void exact_solve(const DevMesh& m, const KParams& kp,
+                                            const double* __restrict__ x,
+                                            const double* __restrict__ y,
+                                            const double* __restrict__ z, int e, int q) {
+  const int E = kp.E;
+  exact::ElemIn in;
+  exact_inputs(m, kp, x, y, z, e, in, true);
+  const exact::QpOut r = exact::integrate_qp(in, q, [](bool fail, int) { return fail; });
+  m.c33[(size_t)q * E + e] = r.c33;
+  double* o = m.qsv + 3 * ((size_t)q * E + e);
+  o[0] = r.S00;
+  o[1] = r.S11;
+  o[2] = r.S01;
+  m.qst[(size_t)q * E + e] = r.status;
+}
+
+// 2. finish: element e from the solve's qp results; guard trips go to the redo list.
+template <class Redo>
+__device__ __forceinline__ void exact_finish(const DevMesh& m, const KParams& kp,
+                                             const double* __restrict__ x,
+                                             const double* __restrict__ y,
+                                             const double* __restrict__ z, int e,
+                                             unsigned char extra_flags, const Redo& redo) {
+  const int E = kp.E;
+  int status = 0;
+  double Sv[kMaxQp][3];
+#pragma unroll
+  for (int j = 0; j < kMaxQp; j++) {
+    if (j < kp.n_qp) {
+      const double* o = m.qsv + 3 * ((size_t)j * E + e);
+      Sv[j][0] = o[0];
+      Sv[j][1] = o[1];
+      Sv[j][2] = o[2];
+      status |= m.qst[(size_t)j * E + e];
+    } else {
+      Sv[j][0] = Sv[j][1] = Sv[j][2] = 0.0;
+    }
+  }
+  if (status & kStatCC3333) {  // the cross-qp guard rule matters: redo with coupled qps
+    redo(e);
+    return;
+  }
+  exact::ElemIn in;
+  const ExactRows r = exact_inputs(m, kp, x, y, z, e, in, false);
+  const bool finite = exact_forces(m, kp, e, r, in.cur, Sv, 0, 1);
+  exact_status(m, e, status, finite, extra_flags);
+}
+
+// 3. redo: one lane group per element, lane q runs qp q; the guard of iteration it skips
+//    qp q if any qp <= q failed (wave ballot), then the group exchanges the qp stresses and
+//    each lane writes every n_qp-th node.
+struct QpGroup {
+  int e;      // element of this lane's group
+  int q;      // through-thickness point of this lane
+  int base;   // wave lane of the group's q = 0
+};
+
+__device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& kp,
+                                              const double* __restrict__ x,
+                                              const double* __restrict__ y,
+                                              const double* __restrict__ z, const QpGroup& grp,
+                                              unsigned char extra_flags) {
+  const int E = kp.E;
+  const int e = grp.e, q = grp.q;
+  const int n_qp = kp.n_qp;
+  if (q >= n_qp) return;  // lanes beyond this element's qps (mixed n_qp in one launch)
+  exact::ElemIn in;
+  const ExactRows rows = exact_inputs(m, kp, x, y, z, e, in, true);
   const unsigned long long upto = (2ull << q) - 1ull;
   const exact::QpOut r = exact::integrate_qp(in, q, [&](bool fail, int) {
     return ((__ballot(fail) >> grp.base) & upto) != 0ull;
   });
   m.c33[(size_t)q * E + e] = r.c33;
-  // ---- the group's qp stresses, in qp order ----
   double Sv[kMaxQp][3];
   int status = 0;
 #pragma unroll
@@ -141,76 +266,103 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
       Sv[j][0] = Sv[j][1] = Sv[j][2] = 0.0;
     }
   }
-  // ---- ElemUpdFint :1799-1801, nodes q, q + n_qp, ... of the patch ----
-  const double A0 = m.dA0[e];
-  const double half_h0 = 0.5 * kp.h0;
-  const double pref0 = 0.5 * kp.h0 * A0;
-  double* Fe = m.Fe;
-  bool finite = true;
-  if (kind != kCap) {
-    for (int a = q; a < nen; a += n_qp) {
-      const V3<double> f = exact::node_force_exact(na0[a], na1[a], nb0[a], nb1[a], nb2[a], in.cur,
-                                                   pref0, half_h0, Sv, kp.theta, kp.w, n_qp,
-                                                   kind == kIrregular);
-      finite = finite && isfinite(f.x) && isfinite(f.y) && isfinite(f.z);
-      put_force(Fe, e, E, a, f);
-    }
-  } else if (q == 0) {
-    // zeroed element vector, corner contributions added (:1807-1851)
-    const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
-    V3<double> fc[3];
-    int sl[3];
-    for (int k = 0; k < 3; k++) {
-      sl[k] = m.corner[(size_t)(3 + k) * E + e];
-      fc[k] = exact::cap_force_exact(dN[k][0], dN[k][1], in.cur, pref0, Sv, kp.w, n_qp);
-    }
-    for (int a = 0; a < nen; a++) {
-      V3<double> f = mk(0.0, 0.0, 0.0);
-      for (int k = 0; k < 3; k++)
-        if (sl[k] == a) {
-          f = mk(f.x + fc[k].x, f.y + fc[k].y, f.z + fc[k].z);
-          finite = finite && isfinite(fc[k].x) && isfinite(fc[k].y) && isfinite(fc[k].z);
-        }
-      put_force(Fe, e, E, a, f);
-    }
-  }
-  // ---- element status from the group (non-finite from any lane's nodes) ----
+  const bool finite = exact_forces(m, kp, e, rows, in.cur, Sv, q, n_qp);
   int bad = finite ? 0 : 1;
 #pragma unroll
   for (int j = 0; j < kMaxQp; j++)
     if (j < n_qp) bad |= __shfl(bad, grp.base + j);
   if (q != 0) return;
-  if (bad) atomicMin(&m.status[0], e);
-  m.flags[e] = (unsigned char)(status | (bad ? kStatNonFinite : 0) | extra_flags);
-  if (status & kStatCC3333) {
-    atomicMin(&m.status[2], e);
-    atomicAdd(&m.status[3], 1);
+  exact_status(m, e, status, bad == 0, extra_flags);
+}
+
+// ---- single context ----
+// Solve work items are issued qp by qp, the mid-surface point (theta = 0, odd n_qp) last:
+// on shells with a flat reference its blocks of exactly zero terms are skipped and its
+// solve costs a fraction of the others', so the expensive qps start first and the cheap
+// ones fill the tail.
+__device__ __forceinline__ int qp_of_rank(int r, int P) {
+  const int mid = P >> 1;
+  if (!(P & 1)) return r;
+  return r < mid ? r : (r == P - 1 ? mid : r + 1);
+}
+
+// Work item i of the solve = (qp rank i / chunks, chunk of 64 list entries i % chunks).
+// list == nullptr: element = entry index.
+__global__ void __launch_bounds__(kExactBlock) bf_exact_solve_kernel(DevMesh m, KParams kp,
+                                                                 const double* __restrict__ x,
+                                                                 const double* __restrict__ y,
+                                                                 const double* __restrict__ z,
+                                                                 const int* __restrict__ list,
+                                                                 const int* __restrict__ count,
+                                                                 int n_static) {
+  const int N = count ? *count : n_static;
+  const int P = kp.n_qp;
+  const int chunks = (N + 63) >> 6;
+  const int items = chunks * P;
+  const int lane = threadIdx.x & 63;
+  const int waves = gridDim.x * (kExactBlock / 64);
+  for (int i = blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6); i < items; i += waves) {
+    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * 64 + lane;
+    if (idx < N) exact_solve(m, kp, x, y, z, list ? list[idx] : idx, q);
   }
 }
 
-__global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact(DevMesh m, KParams kp,
-                                                               const double* __restrict__ x,
-                                                               const double* __restrict__ y,
-                                                               const double* __restrict__ z) {
-  const int P = kp.n_qp, per_wave = 64 / P;
-  QpGroup g;
-  int slot;
-  lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
-  // natural block order: the XCD-contiguous remap used by the memory-bound kernels measured
-  // 16 % slower here (this kernel is ALU-bound and gains nothing from L2 locality)
-  const int wave = blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6);
-  g.e = wave * per_wave + slot;
-  if (slot >= per_wave || g.e >= kp.E) return;
-  exact_element(m, kp, x, y, z, g, 0);
+// Work items of a persistent grid: wave w takes item w first, then items waves + k in the
+// order the waves ask a device counter (zero on entry) for them, so the cheap items of the
+// tail go to the waves that are free first.  Every wave leaves once the counter has passed
+// the item count.
+__device__ __forceinline__ int wave_id() {
+  return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+}
+__device__ __forceinline__ int next_item(int* counter) {
+  int i = 0;
+  if ((threadIdx.x & 63) == 0) i = atomicAdd(counter, 1);
+  return __shfl(i, 0) + gridDim.x * (blockDim.x >> 6);
 }
 
-// Persistent drain of the hand-over list: groups stride over [0, count); count is final
-// because the producing kernel completed earlier on the same stream.
-__global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact_list(DevMesh m, KParams kp,
-                                                                    const double* __restrict__ x,
-                                                                    const double* __restrict__ y,
-                                                                    const double* __restrict__ z) {
-  const int count = m.status[4];
+// Solve over the hand-over list m.fallback[0 .. *count), items taken from a counter.
+__global__ void __launch_bounds__(kExactBlock) bf_exact_solve_queue_kernel(
+    DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
+    int* __restrict__ counter) {
+  const int N = *count;
+  const int P = kp.n_qp;
+  const int chunks = (N + 63) >> 6;
+  const int items = chunks * P;
+  const int lane = threadIdx.x & 63;
+  for (int i = wave_id(); i < items; i = next_item(counter)) {
+    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * 64 + lane;
+    if (idx < N) exact_solve(m, kp, x, y, z, list[idx], q);
+  }
+}
+
+struct RedoSink {
+  int* list;
+  int* count;
+  __device__ void operator()(int e) const { list[atomicAdd(count, 1)] = e; }
+};
+
+__global__ void __launch_bounds__(256) bf_exact_finish_kernel(DevMesh m, KParams kp,
+                                                              const double* __restrict__ x,
+                                                              const double* __restrict__ y,
+                                                              const double* __restrict__ z,
+                                                              const int* __restrict__ list,
+                                                              const int* __restrict__ count,
+                                                              int n_static,
+                                                              unsigned char extra_flags) {
+  const int N = count ? *count : n_static;
+  const RedoSink redo{m.redo, &m.status[5]};
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += gridDim.x * blockDim.x)
+    exact_finish(m, kp, x, y, z, list ? list[idx] : idx, extra_flags, redo);
+}
+
+// Lane groups over the redo list (count read on the device; normally empty).
+__global__ void __launch_bounds__(kExactBlock) bf_exact_redo_kernel(DevMesh m, KParams kp,
+                                                                const double* __restrict__ x,
+                                                                const double* __restrict__ y,
+                                                                const double* __restrict__ z,
+                                                                unsigned char extra_flags) {
+  const int count = m.status[5];
   const int P = kp.n_qp, per_wave = 64 / P;
   QpGroup g;
   int slot;
@@ -219,39 +371,78 @@ __global__ void __launch_bounds__(kExactBlock) bf_element_kernel_exact_list(DevM
   const int waves = gridDim.x * (kExactBlock / 64);
   for (int i = (blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6)) * per_wave + slot; i < count;
        i += waves * per_wave) {
-    g.e = m.fallback[i];
-    exact_element(m, kp, x, y, z, g, kStatFallback);
+    g.e = m.redo[i];
+    exact_element(m, kp, x, y, z, g, extra_flags);
   }
 }
 
-// Plain FP64 element phase of a batch over the element tiles (one case per tile).
-__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_kernel(BatchDev b) {
-  const int P = b.P, per_wave = 64 / P;
-  QpGroup g;
-  int slot;
-  lane_group(P, threadIdx.x & 63, slot, g.q, g.base);
-  // natural order over the work list (see bf_element_kernel_exact)
-  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  const int stride = b.persistent ? (int)(gridDim.x * (blockDim.x >> 6)) : b.n_etiles;
-  for (int t = wave; t < b.n_etiles; t += stride) {
-    const Tile tl = b.etiles[t];
-    const int c = __builtin_amdgcn_readfirstlane(tl.c);
-    const int e0 = __builtin_amdgcn_readfirstlane(tl.b);
-    const int n = __builtin_amdgcn_readfirstlane(tl.n);
-    const CaseDev& cd = b.cases[c];
-    for (int k0 = 0; k0 < n; k0 += per_wave) {
-      const int i = k0 + slot;
-      if (slot < per_wave && i < n) {
-        g.e = e0 + i;
-        exact_element(cd.m, cd.kp, cd.x, cd.y, cd.z, g, 0);
-      }
+// ---- batches ----
+// Solve work item i = (qp rank i / n_xtiles, 64-element tile i % n_xtiles).
+__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_solve_kernel(BatchDev b) {
+  const int P = b.P;
+  const int items = b.n_xtiles * P;
+  const int lane = threadIdx.x & 63;
+  const int waves = gridDim.x * (kExactBlock / 64);
+  for (int i = blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6); i < items; i += waves) {
+    const int q = qp_of_rank(i / b.n_xtiles, P);
+    const Tile tl = b.xtiles[i % b.n_xtiles];
+    const CaseDev& cd = b.cases[tl.c];
+    if (lane < tl.n && q < cd.kp.n_qp) exact_solve(cd.m, cd.kp, cd.x, cd.y, cd.z, tl.b + lane, q);
+  }
+}
+
+// Solve over the compacted hand-over list: item i = (qp rank i / chunks, chunk of 64
+// entries i % chunks), taken from a counter.
+__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_solve_list_kernel(BatchDev b) {
+  const int N = *b.htotal;
+  const int P = b.P;
+  const int chunks = (N + 63) >> 6;
+  const int items = chunks * P;
+  const int lane = threadIdx.x & 63;
+  for (int i = wave_id(); i < items; i = next_item(b.rtotal + 1)) {
+    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * 64 + lane;
+    if (idx >= N) continue;
+    const int2 v = b.hlist[idx];
+    const CaseDev& cd = b.cases[v.x];
+    if (q < cd.kp.n_qp) exact_solve(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, q);
+  }
+}
+
+struct BatchRedoSink {
+  int2* list;
+  int* count;
+  int c;
+  __device__ void operator()(int e) const { list[atomicAdd(count, 1)] = make_int2(c, e); }
+};
+
+// Finish over the element tiles (list = 0) or the hand-over list (list = 1).
+__global__ void __launch_bounds__(256) bf_batch_exact_finish_kernel(BatchDev b, int from_list,
+                                                                    unsigned char extra_flags) {
+  const int stride = gridDim.x * blockDim.x;
+  if (from_list) {
+    const int N = *b.htotal;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += stride) {
+      const int2 v = b.hlist[idx];
+      const CaseDev& cd = b.cases[v.x];
+      exact_finish(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, extra_flags,
+                   BatchRedoSink{b.rlist, b.rtotal, v.x});
     }
+    return;
+  }
+  const long long total = (long long)b.n_xtiles * 64;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += stride) {
+    const int t = (int)(i >> 6), k = (int)(i & 63);
+    const Tile tl = b.xtiles[t];
+    if (k >= tl.n) continue;
+    const CaseDev& cd = b.cases[tl.c];
+    exact_finish(cd.m, cd.kp, cd.x, cd.y, cd.z, tl.b + k, extra_flags,
+                 BatchRedoSink{b.rlist, b.rtotal, tl.c});
   }
 }
 
-// Hand-over drain of a batch: element groups stride over b.hlist[0 .. *b.htotal).
-__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_list_kernel(BatchDev b) {
-  const int count = *b.htotal;
+__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_redo_kernel(BatchDev b,
+                                                                      unsigned char extra_flags) {
+  const int count = *b.rtotal;
   const int P = b.P, per_wave = 64 / P;
   QpGroup g;
   int slot;
@@ -260,23 +451,34 @@ __global__ void __launch_bounds__(kExactBlock) bf_batch_exact_list_kernel(BatchD
   const int waves = gridDim.x * (kExactBlock / 64);
   for (int i = (blockIdx.x * (kExactBlock / 64) + (threadIdx.x >> 6)) * per_wave + slot; i < count;
        i += waves * per_wave) {
-    const int2 v = b.hlist[i];
+    const int2 v = b.rlist[i];
     const CaseDev& cd = b.cases[v.x];
     g.e = v.y;
-    exact_element(cd.m, cd.kp, cd.x, cd.y, cd.z, g, kStatFallback);
+    exact_element(cd.m, cd.kp, cd.x, cd.y, cd.z, g, extra_flags);
   }
 }
 
-hipError_t launch_batch_exact(const BatchDev& b, int blocks, hipStream_t s) {
-  if (b.n_etiles <= 0) return hipSuccess;
-  if (blocks <= 0) blocks = (b.n_etiles + (kExactBlock / 64) - 1) / (kExactBlock / 64);
-  hipLaunchKernelGGL(bf_batch_exact_kernel, dim3(blocks), dim3(kExactBlock), 0, s, b);
+hipError_t launch_batch_exact(const BatchDev& b, int n_cu, hipStream_t s) {
+  if (b.n_xtiles <= 0) return hipSuccess;
+  const int items = b.n_xtiles * b.P;
+  hipLaunchKernelGGL(bf_batch_exact_solve_kernel, dim3((items + 1) / 2), dim3(kExactBlock), 0, s, b);
+  const long long total = (long long)b.n_xtiles * 64;
+  const int fblocks = (int)std::min<long long>((total + 255) / 256, 8LL * n_cu);
+  hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(fblocks), dim3(256), 0, s, b, 0,
+                     (unsigned char)0);
+  hipLaunchKernelGGL(bf_batch_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b,
+                     (unsigned char)0);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_exact_list(const BatchDev& b, int blocks, hipStream_t s) {
+hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, hipStream_t s) {
   if (b.n_etiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(bf_batch_exact_list_kernel, dim3(blocks), dim3(kExactBlock), 0, s, b);
+  // persistent: one wave per SIMD at the solve's occupancy of one
+  hipLaunchKernelGGL(bf_batch_exact_solve_list_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b);
+  hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(2 * n_cu), dim3(256), 0, s, b, 1,
+                     (unsigned char)kStatFallback);
+  hipLaunchKernelGGL(bf_batch_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b,
+                     (unsigned char)kStatFallback);
   return hipGetLastError();
 }
 
@@ -363,22 +565,30 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 }
 
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
-                                         const double* y, const double* z, hipStream_t s) {
+                                         const double* y, const double* z, int n_cu,
+                                         hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
-  // persistent: one wave per SIMD of the 256-CU part at the kernel's occupancy of one
-  // (the count is known on the device only; extra waves of a larger grid would only exit)
-  const int per_block = (kExactBlock / 64) * (64 / kp.n_qp);
-  const int blocks = std::min((kp.E + per_block - 1) / per_block, 512);
-  hipLaunchKernelGGL(bf_element_kernel_exact_list, dim3(blocks), dim3(kExactBlock), 0, s, m, kp, x, y, z);
+  // persistent grids (the list length is known on the device only)
+  hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp,
+                     x, y, z, (const int*)m.fallback, (const int*)&m.status[4], &m.status[6]);
+  hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(2 * n_cu), dim3(256), 0, s, m, kp, x, y, z,
+                     (const int*)m.fallback, (const int*)&m.status[4], 0,
+                     (unsigned char)kStatFallback);
+  hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,
+                     z, (unsigned char)kStatFallback);
   return hipGetLastError();
 }
 
 hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
-                                const double* y, const double* z, hipStream_t s) {
+                                const double* y, const double* z, int n_cu, hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
-  const int per_block = (kExactBlock / 64) * (64 / kp.n_qp);
-  hipLaunchKernelGGL(bf_element_kernel_exact, dim3((kp.E + per_block - 1) / per_block),
-                     dim3(kExactBlock), 0, s, m, kp, x, y, z);
+  const int items = ((kp.E + 63) / 64) * kp.n_qp;
+  hipLaunchKernelGGL(bf_exact_solve_kernel, dim3((items + 1) / 2), dim3(kExactBlock), 0, s, m,
+                     kp, x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E);
+  hipLaunchKernelGGL(bf_exact_finish_kernel, dim3((kp.E + 255) / 256), dim3(256), 0, s, m, kp,
+                     x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E, (unsigned char)0);
+  hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,
+                     z, (unsigned char)0);
   return hipGetLastError();
 }
 

@@ -8,8 +8,8 @@
 //      the 3*nen nodal contributions to the slot rows of the element-force buffer.  Elements
 //      outside the well-conditioned regime are listed instead (per-wave segments, no
 //      atomics).
-//   2. bf_compact_kernel + bf_element_kernel_exact_list: the listed elements recomputed in
-//      the reference's operation order (bf_exact.hip).
+//   2. bf_compact_kernel + the reference-order solve / finish / redo kernels over the list:
+//      the listed elements recomputed in the reference's operation order (bf_exact.hip).
 //   3. bf_gather_kernel: one lane per node.  Sums the node's (element, slot) contributions in
 //      ascending element order from a CSR list built at upload, in FP64: the summation order
 //      of VecSetValues(ADD_VALUES) on one rank (active_strain.c:2238-2260), deterministic run
@@ -135,7 +135,7 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
       o.S11 = __shfl(s.S11, base + j);
       o.S01 = __shfl(s.S01, base + j);
       add_resultants(r, o, kp.w[j], kp.theta[j]);
-      st |= __shfl(s.status | c33_range_bit(s.c33, sizeof(R) == 4), base + j);
+      st |= __shfl(s.status | c33_range_bit(s.c33), base + j);
     }
   }
   const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) && isfinite(r.M0) &&
@@ -253,6 +253,11 @@ __global__ void __launch_bounds__(256) bf_batch_prologue(BatchDev b, int ghosts)
     st[2] = INT_MAX;
     st[3] = 0;
     st[4] = 0;
+    st[5] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.rtotal[0] = 0;
+    b.rtotal[1] = 0;
   }
   if (!ghosts) return;
   for (int t = blockIdx.x; t < b.n_gtiles; t += gridDim.x) {
@@ -351,12 +356,15 @@ __global__ void bf_status_reset(int* st) {
     st[2] = INT_MAX;
     st[3] = 0;
     st[4] = 0;
+    st[5] = 0;
+    st[6] = 0;
   }
 }
 
 // ------------------------------------------------------------------ host side ----------
 struct bf_ctx {
   int device = 0;
+  int n_cu = 256;  // compute units of the device (persistent grid sizes)
   hipStream_t stream = nullptr;
   // per evaluation: before the element kernel, after it, after the hand-over kernel,
   // after the gather
@@ -387,6 +395,9 @@ struct bf_ctx {
   int* d_fallback = nullptr;
   int* d_wlist = nullptr;
   int* d_wcount = nullptr;
+  double* d_qsv = nullptr;
+  int* d_qst = nullptr;
+  int* d_redo = nullptr;
   // explicit time-stepping state (bf_state_init)
   double* d_v = nullptr;
   double* d_winv = nullptr;
@@ -435,6 +446,7 @@ void free_mesh(bf_ctx* c) {
   dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
   dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
   dfree(c->d_flags); dfree(c->d_fallback); dfree(c->d_wlist); dfree(c->d_wcount);
+  dfree(c->d_qsv); dfree(c->d_qst); dfree(c->d_redo);
   dfree(c->d_v); dfree(c->d_winv); dfree(c->d_fixed);
   c->has_state = false;
   dfree(c->d_ptr); dfree(c->d_ent); dfree(c->d_rules); dfree(c->d_x); dfree(c->d_y);
@@ -501,6 +513,11 @@ int bf_ctx_create(int device, bf_ctx** out) {
   bf_ctx* c = new (std::nothrow) bf_ctx();
   if (!c) return BF_ERR_NOMEM;
   c->device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->n_cu = prop.multiProcessorCount;
+  }
   int rc = upload_tables(device);
   if (rc != BF_OK) { delete c; return rc; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -669,6 +686,9 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
   ALLOC(c->d_fallback, En);
   ALLOC(c->d_wlist, En + 64);
   ALLOC(c->d_wcount, En);
+  ALLOC(c->d_qsv, (size_t)3 * kMaxQp * En);
+  ALLOC(c->d_qst, (size_t)kMaxQp * En);
+  ALLOC(c->d_redo, En);
   ALLOC(c->d_ptr, (size_t)m->n_v + 1);
   ALLOC(c->d_ent, (size_t)std::max(nnz, 1));
   ALLOC(c->d_x, Nn);
@@ -778,6 +798,9 @@ static DevMesh dev_mesh(const bf_ctx* c) {
   dm.fallback = c->d_fallback;
   dm.wlist = c->d_wlist;
   dm.wcount = c->d_wcount;
+  dm.qsv = c->d_qsv;
+  dm.qst = c->d_qst;
+  dm.redo = c->d_redo;
   return dm;
 }
 static KParams dev_params(const bf_ctx* c) {
@@ -813,7 +836,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     const int per_block = (kStagedBlock / 64) * (64 / kp.n_qp);
     const dim3 grid((c->E + per_block - 1) / per_block), block(kStagedBlock);
     if (precision == BF_PREC_FP64)
-      BF_HIP(launch_element_exact(dm, kp, x, y, z, s));
+      BF_HIP(launch_element_exact(dm, kp, x, y, z, c->n_cu, s));
     else if (precision == BF_PREC_FP64_FAST)
       hipLaunchKernelGGL(bf_element_kernel<double>, grid, block, 0, s, dm, kp, x, y, z);
     else
@@ -825,7 +848,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     const int n_waves = (c->E + per_wave - 1) / per_wave;
     hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, c->d_wcount,
                        c->d_wlist, n_waves, per_wave, c->d_fallback, c->d_status);
-    BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, s));
+    BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, s));
   }
   BF_HIP(hipEventRecord(e2, s));
   if (step)
@@ -1146,10 +1169,10 @@ struct bf_batch {
   int persistent = 1;
   // device state
   CaseDev* d_cases = nullptr;
-  Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr;
-  int *d_tlist = nullptr, *d_tcount = nullptr, *d_htotal = nullptr;
-  int2* d_hlist = nullptr;
-  int n_etiles = 0, n_ntiles = 0, n_gtiles = 0, tile_cap = 0;
+  Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr, *d_xtiles = nullptr;
+  int *d_tlist = nullptr, *d_tcount = nullptr, *d_htotal = nullptr, *d_rtotal = nullptr;
+  int2 *d_hlist = nullptr, *d_rlist = nullptr;
+  int n_etiles = 0, n_ntiles = 0, n_gtiles = 0, n_xtiles = 0, tile_cap = 0;
   long long E_total = 0;
   int* h_total = nullptr;  // pinned
   // captured step
@@ -1162,8 +1185,10 @@ namespace {
 
 void batch_free_lists(bf_batch* b) {
   dfree(b->d_cases); dfree(b->d_etiles); dfree(b->d_ntiles); dfree(b->d_gtiles);
+  dfree(b->d_xtiles);
   dfree(b->d_tlist); dfree(b->d_tcount); dfree(b->d_hlist); dfree(b->d_htotal);
-  b->n_etiles = b->n_ntiles = b->n_gtiles = 0;
+  dfree(b->d_rlist); dfree(b->d_rtotal);
+  b->n_etiles = b->n_ntiles = b->n_gtiles = b->n_xtiles = 0;
 }
 
 void batch_drop_graph(bf_batch* b) {
@@ -1185,7 +1210,7 @@ int batch_build(bf_batch* b) {
   b->P = P;
   b->tile_cap = b->passes * (64 / P);
   std::vector<CaseDev> cases((size_t)nc);
-  std::vector<Tile> et, nt, gt;
+  std::vector<Tile> et, nt, gt, xt;
   long long Et = 0;
   for (int i = 0; i < nc; i++) {
     bf_ctx* c = b->cases[i];
@@ -1211,12 +1236,14 @@ int batch_build(bf_batch* b) {
     for (int e0 = 0; e0 < c->E; e0 += b->tile_cap) et.push_back(Tile{i, e0, std::min(b->tile_cap, c->E - e0), 0});
     for (int n0 = 0; n0 < c->n_nodes; n0 += 256) nt.push_back(Tile{i, n0, std::min(256, c->n_nodes - n0), 0});
     for (int g0 = 0; g0 < c->n_ghost; g0 += 256) gt.push_back(Tile{i, g0, std::min(256, c->n_ghost - g0), 0});
+    for (int e0 = 0; e0 < c->E; e0 += 64) xt.push_back(Tile{i, e0, std::min(64, c->E - e0), 0});
     Et += c->E;
   }
   b->E_total = Et;
   b->n_etiles = (int)et.size();
   b->n_ntiles = (int)nt.size();
   b->n_gtiles = (int)gt.size();
+  b->n_xtiles = (int)xt.size();
   int rc;
 #define BALLOC(p, n) \
   if ((rc = dalloc(&(p), (n))) != BF_OK) { batch_free_lists(b); return rc; }
@@ -1224,10 +1251,13 @@ int batch_build(bf_batch* b) {
   BALLOC(b->d_etiles, et.size());
   BALLOC(b->d_ntiles, nt.size());
   BALLOC(b->d_gtiles, std::max<size_t>(gt.size(), 1));
+  BALLOC(b->d_xtiles, std::max<size_t>(xt.size(), 1));
   BALLOC(b->d_tlist, (size_t)b->n_etiles * b->tile_cap);
   BALLOC(b->d_tcount, (size_t)b->n_etiles);
   BALLOC(b->d_hlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_htotal, 1);
+  BALLOC(b->d_rlist, (size_t)std::max<long long>(Et, 1));
+  BALLOC(b->d_rtotal, 2);
 #undef BALLOC
   hipStream_t s = b->stream;
   BF_HIP(hipMemcpyAsync(b->d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, s));
@@ -1235,6 +1265,8 @@ int batch_build(bf_batch* b) {
   BF_HIP(hipMemcpyAsync(b->d_ntiles, nt.data(), sizeof(Tile) * nt.size(), hipMemcpyHostToDevice, s));
   if (!gt.empty())
     BF_HIP(hipMemcpyAsync(b->d_gtiles, gt.data(), sizeof(Tile) * gt.size(), hipMemcpyHostToDevice, s));
+  if (!xt.empty())
+    BF_HIP(hipMemcpyAsync(b->d_xtiles, xt.data(), sizeof(Tile) * xt.size(), hipMemcpyHostToDevice, s));
   BF_HIP(hipStreamSynchronize(s));
   return BF_OK;
 }
@@ -1252,10 +1284,14 @@ BatchDev batch_dev(const bf_batch* b) {
   d.n_ntiles = b->n_ntiles;
   d.gtiles = b->d_gtiles;
   d.n_gtiles = b->n_gtiles;
+  d.xtiles = b->d_xtiles;
+  d.n_xtiles = b->n_xtiles;
   d.tlist = b->d_tlist;
   d.tcount = b->d_tcount;
   d.hlist = b->d_hlist;
   d.htotal = b->d_htotal;
+  d.rlist = b->d_rlist;
+  d.rtotal = b->d_rtotal;
   return d;
 }
 
@@ -1268,9 +1304,7 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
   hipLaunchKernelGGL(bf_batch_prologue, dim3(pblocks), dim3(256), 0, s, d, ghosts);
   const int wpb = b->block / 64;
   if (precision == BF_PREC_FP64) {
-    // reference order: occupancy of one wave per SIMD, 2 waves per block
-    const int blocks = b->persistent ? (b->blocks > 0 ? b->blocks : b->n_cu * 2) : 0;
-    BF_HIP(launch_batch_exact(d, blocks, s));
+    BF_HIP(launch_batch_exact(d, b->n_cu, s));
   } else {
     // the staged kernels hold two waves per SIMD: 8 per CU
     int blocks = b->blocks > 0 ? b->blocks : (b->n_cu * 8 + wpb - 1) / wpb;
@@ -1280,7 +1314,7 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
     else
       hipLaunchKernelGGL(bf_batch_element_kernel<float>, dim3(blocks), dim3(b->block), 0, s, d);
     hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d);
-    BF_HIP(launch_batch_exact_list(d, b->n_cu * 2, s));
+    BF_HIP(launch_batch_exact_list(d, b->n_cu, s));
   }
   if (step)
     BF_HIP(launch_batch_gather_step(d, s));

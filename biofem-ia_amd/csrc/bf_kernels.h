@@ -40,7 +40,12 @@ struct DevMesh {
   int* fallback;       // [E] elements handed from a staged kernel to the plain kernel
   int* wlist;          // [E + 64] per-wave segments of the staged kernel's hand-over list
   int* wcount;         // [E] per-wave hand-over counts
+  double* qsv;         // [kMaxQp][E][3] reference-order path: S^00, S^11, S^01 per qp
+  int* qst;            // [kMaxQp][E] reference-order path: status bits per qp
+  int* redo;           // [E] elements whose CC3333 guard tripped: redone with coupled qps
 };
+// status[5]: number of entries in redo
+// status[6]: work counter of the hand-over solve
 
 // GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
 __device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) {
@@ -157,10 +162,14 @@ struct BatchDev {
   int n_ntiles;
   const Tile* gtiles;     // ghost tiles
   int n_gtiles;
+  const Tile* xtiles;     // element tiles of 64 for the reference-order path
+  int n_xtiles;
   int* tlist;             // [n_etiles * tile_cap] hand-over segments
   int* tcount;            // [n_etiles]
   int2* hlist;            // compacted hand-over list: (case, element)
   int* htotal;
+  int2* rlist;            // elements whose CC3333 guard tripped: (case, element)
+  int* rtotal;            // [2]: rlist count, work counter of the hand-over solve
 };
 
 __device__ inline V3<double> ld3(const double* __restrict__ x, const double* __restrict__ y,
@@ -285,15 +294,21 @@ __device__ __forceinline__ void batch_tile_range(const BatchDev& b, int& t0, int
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes);
 hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* y,
                         const double* z, double* ref, hipStream_t s);
+// The reference-order path: a qp-major solve kernel (every wave one through-thickness point
+// of 64 elements), a per-element force kernel, and a redo of the elements whose CC3333
+// guard tripped with the qps coupled.  status[5] must be zero on entry.
 hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
-                                const double* y, const double* z, hipStream_t s);
+                                const double* y, const double* z, int n_cu, hipStream_t s);
 // Re-evaluates, in the reference's operation order, the elements a staged kernel listed in
-// m.fallback[0 .. m.status[4]) (persistent grid, count read on the device).
+// m.fallback[0 .. m.status[4]) (persistent grids, count read on the device; the solve takes
+// its work items from status[6], which must be zero on entry).
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
-                                         const double* y, const double* z, hipStream_t s);
-// Batch forms: the reference-order kernel over the element tiles (plain FP64 batches) and
-// over the compacted hand-over list b.hlist[0 .. *b.htotal).
-hipError_t launch_batch_exact(const BatchDev& b, int blocks, hipStream_t s);
+                                         const double* y, const double* z, int n_cu,
+                                         hipStream_t s);
+// Batch forms: the reference-order path over the element tiles (plain FP64 batches) and
+// over the compacted hand-over list b.hlist[0 .. *b.htotal).  b.rtotal[0..1] must be zero
+// on entry (the prologue clears them).
+hipError_t launch_batch_exact(const BatchDev& b, int n_cu, hipStream_t s);
 // Node phase of a time step (one context): gather + explicit damped update, FP64 without
 // contraction.  Batch form: one block per node tile of every case.
 hipError_t launch_gather_step(const int* ptr, const unsigned* ent, const double* Fe, int E,
@@ -301,6 +316,6 @@ hipError_t launch_gather_step(const int* ptr, const unsigned* ent, const double*
                               double* z, double* v, const double* winv,
                               const unsigned char* fixed, double dt, double damp, hipStream_t s);
 hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s);
-hipError_t launch_batch_exact_list(const BatchDev& b, int blocks, hipStream_t s);
+hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, hipStream_t s);
 
 }  // namespace bf

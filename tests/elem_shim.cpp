@@ -80,7 +80,7 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
         cc = rf.c33; stt = rf.status; fin = std::isfinite(rf.N0) && std::isfinite(rf.M0);
         N0 = rf.N0; N1 = rf.N1; N2 = rf.N2; M0 = rf.M0; M1 = rf.M1; M2 = rf.M2;
       }
-      if (needs_exact(cc, n_qp, stt, fin, precision == 5)) {
+      if (needs_exact(cc, n_qp, stt, fin)) {
         do_exact = true;
         warn++;
       } else {

@@ -33,6 +33,7 @@ BF_ELEM_NONFINITE = 7
 FLAG_CC3333 = 1
 FLAG_NONFINITE = 2
 FLAG_CLAMP = 4
+FLAG_FALLBACK = 8
 BF_PREC_FP64 = 0
 BF_PREC_MIXED = 1
 BF_PREC_FP64_FAST = 2
@@ -102,6 +103,7 @@ SIGNATURES = {
     "bf_batch_last_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bf_batch_last_fallback_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "bf_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bf_batch_case_status": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bf_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "bf_batch_get_launch": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bf_batch_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),

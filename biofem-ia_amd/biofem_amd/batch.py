@@ -45,6 +45,18 @@ class Batch:
         L.check("bf_batch_last_ms", self.lib.bf_batch_last_ms(self.h, C.byref(ms)))
         return ms.value
 
+    def case_status(self):
+        """Per case of the batch: (first non-finite element, first CC3333 trip) of the last
+        step, -1 for none."""
+        n = len(self.cases)
+        nf = np.full(n, -1, np.int32)
+        cc = np.full(n, -1, np.int32)
+        if n:
+            ip = C.POINTER(C.c_int)
+            L.check("bf_batch_case_status",
+                    self.lib.bf_batch_case_status(self.h, nf.ctypes.data_as(ip), cc.ctypes.data_as(ip)))
+        return nf, cc
+
     def fallback_count(self) -> int:
         n = C.c_int()
         L.check("bf_batch_last_fallback_count", self.lib.bf_batch_last_fallback_count(self.h, C.byref(n)))

@@ -101,6 +101,7 @@ class Case:
     admitted: float = 0.0
     finished: float = 0.0
     result: np.ndarray | None = None
+    failed: str | None = None   # why the case was retired before its last step
 
 
 def stable_dt(m: Mesh, p: ActParams, rho: float = 1.0, safety: float = 0.1) -> float:
@@ -158,11 +159,12 @@ class SweepReport:
     peak_footprint: int = 0
 
     def summary(self) -> dict:
-        lat = np.array([c.finished - c.spec.arrival for c in self.cases])
+        ok = [c for c in self.cases if c.failed is None]
+        lat = np.array([c.finished - c.spec.arrival for c in ok])
         wait = np.array([c.admitted - c.spec.arrival for c in self.cases])
-        work = sum(c.mesh.n_elmt * c.spec.steps for c in self.cases)
+        work = sum(c.mesh.n_elmt * (c.spec.steps - max(c.remaining, 0)) for c in self.cases)
         return dict(
-            cases=len(self.cases), wall_s=self.wall,
+            cases=len(self.cases), failed=len(self.cases) - len(ok), wall_s=self.wall,
             throughput_cases_per_s=len(self.cases) / self.wall if self.wall else 0.0,
             elem_steps_per_s=work / self.wall if self.wall else 0.0,
             elem_qp_per_s_gpu=3 * work / (self.gpu_ms * 1e-3) if self.gpu_ms else 0.0,
@@ -241,19 +243,28 @@ class Server:
                 batch.sync()
                 rep.gpu_ms += batch.last_ms()
                 rep.rounds += 1
-                done = [c for c in active if c.remaining - k <= 0]
+                # a case whose state diverged (non-finite forces) is retired with its reason
+                # instead of holding the GPU for the rest of its steps
+                nonfinite, _ = batch.case_status()
+                for c, e in zip(active, nonfinite):
+                    if e >= 0:
+                        c.failed = (f"non-finite force at element {int(e)} within steps "
+                                    f"{c.spec.steps - c.remaining + 1}..{c.spec.steps - c.remaining + k}")
+                done = [c for c in active if c.remaining - k <= 0 or c.failed]
                 for c in active:
                     c.remaining -= k
                 if done:
                     tnow = time.perf_counter() - t0
                     for c in done:
-                        if self.keep:
+                        if self.keep and c.failed is None:
                             c.result = case_result(c)
+                        if c.failed:
+                            self.log(f"case {c.spec.cid} retired: {c.failed}")
                         c.finished = tnow
                         used -= c.footprint
                         c.fe.close()
                         rep.cases.append(c)
-                    active = [c for c in active if c.remaining > 0]
+                    active = [c for c in active if c.remaining > 0 and not c.failed]
                     changed = True
             rep.wall = time.perf_counter() - t0
         finally:

@@ -334,7 +334,9 @@ struct Resultants {
 };
 
 // Bound on the reference's rounding-driven change of C33 in one update above which the
-// staged paths hand the element to the reference-order kernel.
+// staged paths hand the element to the reference-order kernel (both staged paths: the
+// FP32-mixed path with a looser 1e-8 stays within its 1e-5 budget on the 256 x 192
+// membrane but not on noisier states of the same mesh).
 constexpr double kC33NoiseMax = 1e-11;
 
 // Result of one through-thickness point of a staged chain: the total stress entries
@@ -863,18 +865,24 @@ BF_HD Resultants<float> integrate_element_mixed(const V3<double>& a1c, const V3<
 // reference rounding bound c33_update_noise exceeds kC33NoiseMax, the CC3333 guard, or a
 // non-finite value).  There the 3 half-Newton steps of the reference amplify any rounding
 // difference by orders of magnitude, so the fast paths hand such elements to the plain
-// kernel.  The rounding bound is the sharp test; the range only keeps out near-degenerate
-// thickness stretches (C33 -> 0) where the bound's first-order model no longer holds.  On
-// the noisy 256 x 192 membrane these settings hand over 6.3 % of the elements and leave
-// the staged FP64 path within 1.2e-14 (force, norm-wise; C33 within 3.4e-13 per element)
-// and the FP32-mixed path within 5e-7 of the reference order.
+// kernel.  The rounding bound is the sharp test; the range keeps out near-degenerate
+// thickness stretches (C33 -> 0), where the bound's first-order model no longer holds and,
+// for FP32, where the staged C33 itself loses digits.  On the noisy 256 x 192 membrane the
+// FP64 settings hand over 6.3 % of the elements and leave the staged result within 1.2e-14
+// of the reference order (force, norm-wise; C33 within 3.4e-13 per element); the FP32-mixed
+// settings hand over 15 % and stay within 4e-7 (CPU build of this header; a range of
+// [0.3, 3] there reaches 4e-6).
 constexpr double kC33Lo = 0.1, kC33Hi = 10.0;
-BF_HD int c33_range_bit(double c33) { return (c33 >= kC33Lo && c33 <= kC33Hi) ? 0 : kStatRange; }
+constexpr double kC33LoMixed = 0.4, kC33HiMixed = 2.5;
+BF_HD int c33_range_bit(double c33, bool mixed) {
+  const double lo = mixed ? kC33LoMixed : kC33Lo, hi = mixed ? kC33HiMixed : kC33Hi;
+  return (c33 >= lo && c33 <= hi) ? 0 : kStatRange;
+}
 BF_HD bool needs_exact_bits(int status, bool finite) {
   return !finite || (status & (kStatClamp | kStatCC3333 | kStatIllCond | kStatRange)) != 0;
 }
-BF_HD bool needs_exact(const double* c33, int n_qp, int status, bool finite) {
-  for (int q = 0; q < n_qp; q++) status |= c33_range_bit(c33[q]);
+BF_HD bool needs_exact(const double* c33, int n_qp, int status, bool finite, bool mixed) {
+  for (int q = 0; q < n_qp; q++) status |= c33_range_bit(c33[q], mixed);
   return needs_exact_bits(status, finite);
 }
 

@@ -125,6 +125,117 @@ BF_HD double opaque(double v) {
   return v;
 }
 
+// ---- ElemTotTangMatTens :1603-1643: CC^{2222}, one sum of 3^8 terms in the loop order
+// g, h, m, n, p, s, w, z.  The 81 blocks (g, h, m, n) of 81 terms (p, s, w, z) each are
+// formed by cc_block_terms; the sum itself is cc_sum_serial here (one lane adds every term
+// in order) or a split form on the device that forms several blocks at once and still adds
+// the terms in this order.
+//
+// Blocks of terms with an exactly zero outer factor (Fi[g][h], G[m][2], Fi[n][m]) are
+// skipped when every factor is finite and below 1e25 in magnitude: each such term is then an
+// exact (signed) zero, and CC, which starts at +0 and can never become -0 under
+// round-to-nearest addition, is unchanged by it -- the sum stays bit for bit the
+// reference's.  (Flat reference shells have G[a][2] = 0 and Fa_inv.Cov[a][2] =
+// Fa_inv.Cov[2][a] = 0 at the mid-surface point, theta = 0: 5 of the 81 blocks remain.  The
+// inner factors are left alone: a branch there would split the unrolled block and cost more
+// than it saves.)
+struct CcIn {
+  Mat Fi, G, Ch;  // Fa_inv.Cov, G^{ij} (contravariant reference metric), C_h = Ce_inv.Cont
+  double alpha, b2;
+  bool zskip;
+};
+// Factors of block (g, h, m, n).
+struct CcBlock {
+  double Figh, Finm, Gm2, Gg2, Ggn, Chhn;
+  double Chr[3], Chc[3];  // Ch[h][*], Ch[*][n]
+};
+BF_HD bool cc_block_active(const CcIn& c, const CcBlock& k) {
+  return !c.zskip || (k.Figh != 0.0 && k.Gm2 != 0.0 && k.Finm != 0.0);
+}
+// Factors of block b = 27 g + 9 h + 3 m + n.
+BF_HD CcBlock cc_block_at(const CcIn& c, int b) {
+  const int gg = b / 27, h = (b / 9) % 3, mm = (b / 3) % 3, n = b % 3;
+  double Gr[3], Fgr[3], Fmc[3];
+  CcBlock k;
+  row3(c.G, gg, Gr);
+  row3(c.Fi, gg, Fgr);
+  row3(c.Ch, h, k.Chr);
+  col3(c.Fi, mm, Fmc);
+  col3(c.Ch, n, k.Chc);
+  k.Figh = sel3(h, Fgr[0], Fgr[1], Fgr[2]);
+  k.Gm2 = sel3(mm, c.G.m[0][2], c.G.m[1][2], c.G.m[2][2]);
+  k.Finm = sel3(n, Fmc[0], Fmc[1], Fmc[2]);
+  k.Gg2 = Gr[2];
+  k.Ggn = sel3(n, Gr[0], Gr[1], Gr[2]);
+  k.Chhn = sel3(n, k.Chr[0], k.Chr[1], k.Chr[2]);
+  return k;
+}
+// The 81 terms of one block in the sum's order: emit(i, term), i = 27 p + 9 s + 3 w + z.
+template <class Emit>
+BF_HD void cc_block_terms(const CcIn& c, const CcBlock& k, Emit&& emit) {
+  const Mat& Fi = c.Fi;
+  const Mat& G = c.G;
+  const Mat& Ch = c.Ch;
+  const double alpha = c.alpha, b2 = c.b2;
+  const double quarter = opaque(0.25);
+#pragma unroll
+  for (int pp = 0; pp < 3; pp++) {
+    // CCe(h,n,p,2), CCe(h,2,p,2) (:1578-1581)
+    const double e_np2 = b2 * k.Chhn * Ch.m[pp][2] - alpha * (k.Chr[pp] * k.Chc[2] + k.Chr[2] * k.Chc[pp]);
+    const double e_2p2 = b2 * k.Chr[2] * Ch.m[pp][2] - alpha * (k.Chr[pp] * Ch.m[2][2] + k.Chr[2] * Ch.m[pp][2]);
+#pragma unroll
+    for (int s = 0; s < 3; s++)
+#pragma unroll
+      for (int w = 0; w < 3; w++)
+#pragma unroll
+        for (int z = 0; z < 3; z++) {
+          const double e_npz = b2 * k.Chhn * Ch.m[pp][z] - alpha * (k.Chr[pp] * k.Chc[z] + k.Chr[z] * k.Chc[pp]);
+          const double e_2pz = b2 * k.Chr[2] * Ch.m[pp][z] - alpha * (k.Chr[pp] * Ch.m[z][2] + k.Chr[z] * Ch.m[pp][2]);
+          emit(27 * pp + 9 * s + 3 * w + z,
+               quarter * Fi.m[w][pp] * Fi.m[z][s] * k.Figh * k.Finm * G.m[s][2] * k.Gm2 *
+                   (k.Gg2 * (e_npz * G.m[w][2] + e_np2 * G.m[w][z]) +
+                    k.Ggn * (e_2pz * G.m[w][2] + e_2p2 * G.m[w][z])));
+        }
+  }
+}
+BF_HD double cc_sum_serial(const CcIn& c) {
+  double CC = 0.0;
+#pragma unroll 1
+  for (int gg = 0; gg < 3; gg++) {
+    double Gr[3], Fgr[3];
+    row3(c.G, gg, Gr);    // G[g][*]
+    row3(c.Fi, gg, Fgr);  // Fi[g][*]
+#pragma unroll 1
+    for (int h = 0; h < 3; h++) {
+      CcBlock k;
+      k.Gg2 = Gr[2];
+      row3(c.Ch, h, k.Chr);
+      k.Figh = sel3(h, Fgr[0], Fgr[1], Fgr[2]);
+      if (c.zskip && k.Figh == 0.0) continue;
+#pragma unroll 1
+      for (int mm = 0; mm < 3; mm++) {
+        k.Gm2 = sel3(mm, c.G.m[0][2], c.G.m[1][2], c.G.m[2][2]);
+        if (c.zskip && k.Gm2 == 0.0) continue;
+        double Fmc[3];
+        col3(c.Fi, mm, Fmc);  // Fi[*][m]
+#pragma unroll 1
+        for (int n = 0; n < 3; n++) {
+          col3(c.Ch, n, k.Chc);
+          k.Finm = sel3(n, Fmc[0], Fmc[1], Fmc[2]);
+          if (c.zskip && k.Finm == 0.0) continue;
+          k.Ggn = sel3(n, Gr[0], Gr[1], Gr[2]);
+          k.Chhn = sel3(n, k.Chr[0], k.Chr[1], k.Chr[2]);
+          cc_block_terms(c, k, [&](int, double t) { CC += t; });
+        }
+      }
+    }
+  }
+  return CC;
+}
+struct CcSerial {
+  BF_HD double operator()(const CcIn& c) const { return cc_sum_serial(c); }
+};
+
 // One through-thickness point: ElemUpdateG, ElemActDefGrad, ElemCGDefTens and the C33
 // loop for qp `qp`.  ModElemC33 returns at the first qp whose |CC3333| fails the guard
 // (:1674-1681), so in iteration `it` this qp keeps its C33 if it or any earlier qp failed:
@@ -135,8 +246,9 @@ struct QpOut {
   int status;
 };
 
-template <class Skip>
-BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
+template <class Skip, class CcSum>
+BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
+                         bool stop_on_guard = false) {
   QpOut out;
   out.status = 0;
   // Fa_cart and its inverse (:1227-1256, :1291): rebuilt per qp (same values every time)
@@ -264,71 +376,20 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
     const double alpha = in.K * (Je2 - Je) - in.mu;
     const double beta = 0.5 * in.K * Je * (2.0 * Je - 1.0);
     const double b2 = 2.0 * beta;
-    // ---- ElemTotTangMatTens :1603-1643, [2][2][2][2]; loop order g,h,m,n,p,s,w,z ----
-    // Blocks of terms with an exactly zero outer factor (Fi[g][h], G[m][2], Fi[n][m]) are
-    // skipped when every factor is finite and below 1e25 in magnitude: each such term is
-    // then an exact (signed) zero, and CC, which starts at +0 and can never become -0 under
-    // round-to-nearest addition, is unchanged by it -- the sum stays bit for bit the
-    // reference's.  (Flat reference shells have G[a][2] = 0 and Fa_inv.Cov[a][2] =
-    // Fa_inv.Cov[2][a] = 0 at the mid-surface point, theta = 0: 5 of the 81 blocks remain.
-    // The inner factors are left alone: a branch there would split the unrolled block and
-    // cost more than it saves.)
-    const bool zskip = bounded(Fi) && bounded(G) && bounded(Ch) && fabs(alpha) < kZeroSkipMax &&
-                       fabs(beta) < kZeroSkipMax;
-    double CC = 0.0;
-#pragma unroll 1
-    for (int gg = 0; gg < 3; gg++) {
-      double Gr[3], Fgr[3];
-      row3(G, gg, Gr);       // G[g][*]
-      row3(Fi, gg, Fgr);     // Fi[g][*]
-      const double Gg2 = Gr[2];
-#pragma unroll 1
-      for (int h = 0; h < 3; h++) {
-        double Chr[3];
-        row3(Ch, h, Chr);    // Ch[h][*]
-        const double Figh = sel3(h, Fgr[0], Fgr[1], Fgr[2]);
-        if (zskip && Figh == 0.0) continue;
-#pragma unroll 1
-        for (int mm = 0; mm < 3; mm++) {
-          const double Gm2 = sel3(mm, G.m[0][2], G.m[1][2], G.m[2][2]);
-          if (zskip && Gm2 == 0.0) continue;
-          double Fmc[3];
-          col3(Fi, mm, Fmc);  // Fi[*][m]
-#pragma unroll 1
-          for (int n = 0; n < 3; n++) {
-            double Chc[3];
-            col3(Ch, n, Chc);  // Ch[*][n]
-            const double Finm = sel3(n, Fmc[0], Fmc[1], Fmc[2]);
-            if (zskip && Finm == 0.0) continue;
-            const double Ggn = sel3(n, Gr[0], Gr[1], Gr[2]);
-            const double Chhn = sel3(n, Chr[0], Chr[1], Chr[2]);
-            const double quarter = opaque(0.25);
-#pragma unroll
-            for (int pp = 0; pp < 3; pp++) {
-              // CCe(h,n,p,2), CCe(h,2,p,2) (:1578-1581)
-              const double e_np2 = b2 * Chhn * Ch.m[pp][2] - alpha * (Chr[pp] * Chc[2] + Chr[2] * Chc[pp]);
-              const double e_2p2 = b2 * Chr[2] * Ch.m[pp][2] - alpha * (Chr[pp] * Ch.m[2][2] + Chr[2] * Ch.m[pp][2]);
-#pragma unroll
-              for (int s = 0; s < 3; s++) {
-#pragma unroll
-                for (int w = 0; w < 3; w++)
-#pragma unroll
-                  for (int z = 0; z < 3; z++) {
-                    const double e_npz = b2 * Chhn * Ch.m[pp][z] - alpha * (Chr[pp] * Chc[z] + Chr[z] * Chc[pp]);
-                    const double e_2pz = b2 * Chr[2] * Ch.m[pp][z] - alpha * (Chr[pp] * Ch.m[z][2] + Chr[z] * Ch.m[pp][2]);
-                    CC += quarter * Fi.m[w][pp] * Fi.m[z][s] * Figh * Finm * G.m[s][2] * Gm2 *
-                          (Gg2 * (e_npz * G.m[w][2] + e_np2 * G.m[w][z]) +
-                           Ggn * (e_2pz * G.m[w][2] + e_2p2 * G.m[w][z]));
-                  }
-              }
-            }
-          }
-        }
-      }
-    }
+    // ---- ElemTotTangMatTens :1603-1643, [2][2][2][2] ----
+    CcIn cc_in;
+    cc_in.Fi = Fi;
+    cc_in.G = G;
+    cc_in.Ch = Ch;
+    cc_in.alpha = alpha;
+    cc_in.b2 = b2;
+    cc_in.zskip = bounded(Fi) && bounded(G) && bounded(Ch) && fabs(alpha) < kZeroSkipMax &&
+                  fabs(beta) < kZeroSkipMax;
+    const double CC = cc_sum(cc_in);
     // ---- ModElemC33 :1669-1707 ----
     const bool fail = !(fabs(CC) > 1e-14);
     if (fail) out.status |= kStatCC3333;
+    if (fail && stop_on_guard) break;  // the caller redoes the element with coupled qps
     if (!skip(fail, it)) {
       double DeltaC33 = -1.0 * S22 / CC;
       const double C33_old = CCov.m[2][2];
@@ -353,6 +414,11 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
   out.S01 = S01;
   out.c33 = CCov.m[2][2];
   return out;
+}
+
+template <class Skip>
+BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip) {
+  return integrate_qp(in, qp, skip, CcSerial{});
 }
 
 BF_HD void integrate(const ElemIn& in, ElemOut& out) {

@@ -187,13 +187,117 @@ __device__ __forceinline__ void exact_solve(const DevMesh& m, const KParams& kp,
   const int E = kp.E;
   exact::ElemIn in;
   exact_inputs(m, kp, x, y, z, e, in, true);
-  const exact::QpOut r = exact::integrate_qp(in, q, [](bool fail, int) { return fail; });
+  // an element whose guard trips is redone with coupled qps: its solve can stop there
+  const exact::QpOut r = exact::integrate_qp(in, q, [](bool fail, int) { return fail; },
+                                             exact::CcSerial{}, true);
   m.c33[(size_t)q * E + e] = r.c33;
   double* o = m.qsv + 3 * ((size_t)q * E + e);
   o[0] = r.S00;
   o[1] = r.S11;
   o[2] = r.S01;
   m.qst[(size_t)q * E + e] = r.status;
+}
+
+// ---- the hand-over solve with K lanes per (element, qp) ----
+// The hand-over list is short (a few percent of the elements), so its solve is bound by
+// the time of one wave, not by the device's throughput: with one (element, qp) per lane
+// most SIMDs idle while the waves that hold elements add their 6561 terms one by one.
+// Here K lanes share one solve.  In round r lane j forms the 81 terms of block r K + j of
+// the CC^{2222} sum (all lanes at once) and stores them in LDS; the group's first lane adds
+// its own block's terms as they are formed, then the other K - 1 blocks' terms from LDS,
+// in block order -- the same additions in the same order as cc_sum_serial, so the sum is
+// bit for bit the serial one.  Everything else of the iteration is formed by all K lanes
+// alike.  A wave holds kSplitLanes / K groups; its term buffer is [81][groups][K] doubles,
+// kSplitLanes x 648 bytes, so four one-wave blocks fit a CU's 160 KB of LDS.
+constexpr int kSplitLanes = 60;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int K>
+struct CcSplit {
+  double* buf;  // the wave's term buffer
+  int g, j;     // group within the wave, lane within the group
+  __device__ double operator()(const exact::CcIn& c) const {
+    constexpr int NG = kSplitLanes / K;
+    const int lead = (threadIdx.x & 63) - j;
+    double CC = 0.0;  // the lead's is the sum; the other lanes' is not used
+#pragma unroll 1
+    for (int b0 = 0; b0 < 81; b0 += K) {
+      const int b = b0 + j;
+      exact::CcBlock k;
+      bool act = false;
+      if (b < 81) {
+        k = exact::cc_block_at(c, b);
+        act = exact::cc_block_active(c, k);
+      }
+      const unsigned long long bal = __ballot(act);
+      if (bal == 0ull) continue;
+      if (act)
+        exact::cc_block_terms(c, k, [&](int i, double t) {
+          CC += t;
+          buf[(i * NG + g) * K + j] = t;
+        });
+      wave_lds_sync();
+      const unsigned grp = (unsigned)(bal >> lead) & ((1u << K) - 1u);
+      // only the leads read: a quarter (K = 4) of the LDS traffic of a full wave
+#pragma unroll 1
+      for (int sl = 1; sl < K; sl++) {
+        if (j != 0 || !((grp >> sl) & 1u)) continue;
+        const double* src = buf + g * K + sl;
+        // all 81 loads in flight before the first add: the LDS latency is paid once per
+        // block, and the dependent FP64 adds (about 16 cycles each) set the pace
+        double v[81];
+#pragma unroll
+        for (int i = 0; i < 81; i++) v[i] = src[i * NG * K];
+#pragma unroll
+        for (int i = 0; i < 81; i++) CC += v[i];
+        __builtin_amdgcn_sched_group_barrier(0x100, 81, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 81, 0);
+      }
+      wave_lds_sync();
+    }
+    return __shfl(CC, lead);
+  }
+};
+
+// Split factor for a hand-over list of N elements: the largest K whose waves for the
+// expensive qps (all but the mid-surface point of an odd rule) fit one round of the grid.
+__device__ __forceinline__ int split_factor(int N, int P, int n_waves) {
+  const int pe = (P > 1 && (P & 1)) ? P - 1 : P;
+  for (int K = 8; K >= 2; K >>= 1) {
+    const int ng = kSplitLanes / K;
+    if ((long long)((N + ng - 1) / ng) * pe <= n_waves) return K;
+  }
+  return 1;
+}
+
+// Solve of qp q of element e by lane j of a K-lane group (K = 1: one lane, serial sum).
+template <int K>
+__device__ __forceinline__ void exact_solve_k(const DevMesh& m, const KParams& kp,
+                                              const double* __restrict__ x,
+                                              const double* __restrict__ y,
+                                              const double* __restrict__ z, int e, int q,
+                                              double* buf, int g, int j) {
+  if constexpr (K == 1) {
+    exact_solve(m, kp, x, y, z, e, q);
+  } else {
+    const int E = kp.E;
+    exact::ElemIn in;
+    exact_inputs(m, kp, x, y, z, e, in, true);
+    const exact::QpOut r = exact::integrate_qp(in, q, [](bool fail, int) { return fail; },
+                                               CcSplit<K>{buf, g, j}, true);
+    if (j != 0) return;
+    m.c33[(size_t)q * E + e] = r.c33;
+    double* o = m.qsv + 3 * ((size_t)q * E + e);
+    o[0] = r.S00;
+    o[1] = r.S11;
+    o[2] = r.S01;
+    m.qst[(size_t)q * E + e] = r.status;
+  }
 }
 
 // 2. finish: element e from the solve's qp results; guard trips go to the redo list.
@@ -320,19 +424,37 @@ __device__ __forceinline__ int next_item(int* counter) {
   return __shfl(i, 0) + gridDim.x * (blockDim.x >> 6);
 }
 
-// Solve over the hand-over list m.fallback[0 .. *count), items taken from a counter.
-__global__ void __launch_bounds__(kExactBlock) bf_exact_solve_queue_kernel(
-    DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
-    int* __restrict__ counter) {
-  const int N = *count;
+// Work items of the hand-over solve at split factor K: (qp rank i / chunks, chunk of
+// `groups` list entries i % chunks), wave w first item w, then from the counter.
+template <int K>
+__device__ void solve_list_k(const DevMesh& m, const KParams& kp, const double* __restrict__ x,
+                             const double* __restrict__ y, const double* __restrict__ z,
+                             const int* __restrict__ list, int N, int* counter, double* buf) {
+  constexpr int NG = K == 1 ? 64 : kSplitLanes / K;
   const int P = kp.n_qp;
-  const int chunks = (N + 63) >> 6;
+  const int chunks = (N + NG - 1) / NG;
   const int items = chunks * P;
   const int lane = threadIdx.x & 63;
+  const int g = lane / K, j = lane % K;
   for (int i = wave_id(); i < items; i = next_item(counter)) {
-    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * 64 + lane;
-    if (idx < N) exact_solve(m, kp, x, y, z, list[idx], q);
+    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * NG + g;
+    if (g < NG && idx < N) exact_solve_k<K>(m, kp, x, y, z, list[idx], q, buf, g, j);
+  }
+}
+
+// Solve over the hand-over list m.fallback[0 .. *count): one-wave blocks, persistent grid
+// of n_waves waves, split factor from the list length.
+__global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
+    DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
+    int* __restrict__ counter, int n_waves) {
+  __shared__ double buf[kSplitLanes * 81];
+  const int N = *count;
+  switch (split_factor(N, kp.n_qp, n_waves)) {
+    case 8: solve_list_k<8>(m, kp, x, y, z, list, N, counter, buf); break;
+    case 4: solve_list_k<4>(m, kp, x, y, z, list, N, counter, buf); break;
+    case 2: solve_list_k<2>(m, kp, x, y, z, list, N, counter, buf); break;
+    default: solve_list_k<1>(m, kp, x, y, z, list, N, counter, buf); break;
   }
 }
 
@@ -391,20 +513,33 @@ __global__ void __launch_bounds__(kExactBlock) bf_batch_exact_solve_kernel(Batch
   }
 }
 
-// Solve over the compacted hand-over list: item i = (qp rank i / chunks, chunk of 64
-// entries i % chunks), taken from a counter.
-__global__ void __launch_bounds__(kExactBlock) bf_batch_exact_solve_list_kernel(BatchDev b) {
-  const int N = *b.htotal;
+// Solve over the compacted hand-over list b.hlist[0 .. *b.htotal) as the single-context
+// form (work counter b.rtotal[1]).
+template <int K>
+__device__ void batch_solve_list_k(const BatchDev& b, int N, double* buf) {
+  constexpr int NG = K == 1 ? 64 : kSplitLanes / K;
   const int P = b.P;
-  const int chunks = (N + 63) >> 6;
+  const int chunks = (N + NG - 1) / NG;
   const int items = chunks * P;
   const int lane = threadIdx.x & 63;
+  const int g = lane / K, j = lane % K;
   for (int i = wave_id(); i < items; i = next_item(b.rtotal + 1)) {
-    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * 64 + lane;
-    if (idx >= N) continue;
+    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * NG + g;
+    if (g >= NG || idx >= N) continue;
     const int2 v = b.hlist[idx];
     const CaseDev& cd = b.cases[v.x];
-    if (q < cd.kp.n_qp) exact_solve(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, q);
+    if (q < cd.kp.n_qp) exact_solve_k<K>(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, q, buf, g, j);
+  }
+}
+
+__global__ void __launch_bounds__(64) bf_batch_exact_solve_list_kernel(BatchDev b, int n_waves) {
+  __shared__ double buf[kSplitLanes * 81];
+  const int N = *b.htotal;
+  switch (split_factor(N, b.P, n_waves)) {
+    case 8: batch_solve_list_k<8>(b, N, buf); break;
+    case 4: batch_solve_list_k<4>(b, N, buf); break;
+    case 2: batch_solve_list_k<2>(b, N, buf); break;
+    default: batch_solve_list_k<1>(b, N, buf); break;
   }
 }
 
@@ -474,7 +609,8 @@ hipError_t launch_batch_exact(const BatchDev& b, int n_cu, hipStream_t s) {
 hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, hipStream_t s) {
   if (b.n_etiles <= 0) return hipSuccess;
   // persistent: one wave per SIMD at the solve's occupancy of one
-  hipLaunchKernelGGL(bf_batch_exact_solve_list_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b);
+  hipLaunchKernelGGL(bf_batch_exact_solve_list_kernel, dim3(4 * n_cu), dim3(64), 0, s, b,
+                     4 * n_cu);
   hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(2 * n_cu), dim3(256), 0, s, b, 1,
                      (unsigned char)kStatFallback);
   hipLaunchKernelGGL(bf_batch_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b,
@@ -569,8 +705,10 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
                                          hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
   // persistent grids (the list length is known on the device only)
-  hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp,
-                     x, y, z, (const int*)m.fallback, (const int*)&m.status[4], &m.status[6]);
+  // persistent: one wave per SIMD (the solve's occupancy)
+  hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
+                     z, (const int*)m.fallback, (const int*)&m.status[4], &m.status[6],
+                     4 * n_cu);
   hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(2 * n_cu), dim3(256), 0, s, m, kp, x, y, z,
                      (const int*)m.fallback, (const int*)&m.status[4], 0,
                      (unsigned char)kStatFallback);

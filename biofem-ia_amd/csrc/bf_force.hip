@@ -35,17 +35,18 @@ using namespace bf;
 // every lane of the group forms the element's current midsurface geometry (FP64), lane q
 // runs through-thickness point q of the staged constitutive chain in R, the group sums the
 // qp resultants in qp order by cross-lane reads, and each lane writes the nodal forces of
-// patch nodes q, q + n_qp, ...  Elements outside the well-conditioned regime are pushed to
-// the sink for the reference-order kernel and write nothing here.
-template <class R, class Sink>
-__device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& kp,
+// patch nodes q, q + n_qp, ...  An element outside the well-conditioned regime writes
+// nothing here; the return value (true in its group's lane q = 0) asks the caller to push
+// it to the hand-over list.
+template <class R>
+__device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& kp,
                                                const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z, int e, int q,
-                                               int base, const Sink& sink) {
+                                               int base) {
   const int E = kp.E;
   const int n_qp = kp.n_qp;
-  if (q >= n_qp) return;
+  if (q >= n_qp) return false;
   // Every per-element load that does not depend on another one is issued here, before the
   // first branch, so that they share one memory round trip: kind, the first 12 patch slots,
   // the geom0 cache, activation data and the area.  The coordinate gathers form the second.
@@ -135,15 +136,15 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
       o.S11 = __shfl(s.S11, base + j);
       o.S01 = __shfl(s.S01, base + j);
       add_resultants(r, o, kp.w[j], kp.theta[j]);
-      st |= __shfl(s.status | c33_range_bit(s.c33), base + j);
+      st |= __shfl(s.status | c33_range_bit(s.c33, sizeof(R) == 4), base + j);
     }
   }
   const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) && isfinite(r.M0) &&
                       isfinite(r.M1) && isfinite(r.M2);
-  // outside the well-conditioned regime: the reference-order kernel recomputes it
+  // outside the well-conditioned regime: the reference-order kernel recomputes it (or, on
+  // the FP32-mixed path, first the FP64 staged chain of the second tier)
   const bool handed = kp.fallback && needs_exact_bits(st, finite);
-  sink.push(handed && q == 0, e);
-  if (handed) return;
+  if (handed) return q == 0;
   m.c33[(size_t)q * E + e] = s.c33;
 
   // ---- nodal forces (ElemUpdFint, active_strain.c:1762-1995) ----
@@ -174,13 +175,14 @@ __device__ __forceinline__ void staged_element(const DevMesh& m, const KParams& 
     }
   }
   // ---- status (group leader) ----
-  if (q != 0) return;
+  if (q != 0) return false;
   if (!finite) atomicMin(&m.status[0], e);
   m.flags[e] = (unsigned char)((st & (kStatCC3333 | kStatClamp)) | (finite ? 0 : kStatNonFinite));
   if (st & kStatCC3333) {
     atomicMin(&m.status[2], e);
     atomicAdd(&m.status[3], 1);
   }
+  return false;
 }
 
 constexpr int kStagedBlock = 256;
@@ -196,7 +198,32 @@ __global__ void __launch_bounds__(kStagedBlock) bf_element_kernel(DevMesh m, KPa
   const int wave = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
   const int e = wave * per_wave + slot;
   if (slot >= per_wave || e >= kp.E) return;
-  staged_element<R>(m, kp, x, y, z, e, q, base, WaveSink{m.wlist, m.wcount, per_wave, wave});
+  const bool take = staged_element<R>(m, kp, x, y, z, e, q, base);
+  WaveSink{m.wlist, m.wcount, per_wave, wave}.push(take, e);
+}
+
+// Second tier of the FP32-mixed path: wave w re-evaluates the elements the FP32 kernel's
+// wave w handed over (its segment of m.wlist) with the FP64 staged chain and keeps in the
+// segment only those that chain cannot hold either, for the reference-order kernel.  Most
+// of them have C33 between the FP32 path's range and the FP64 path's; the hand-over set of
+// the mixed path is then the FP64 path's.  Same grid as the element kernel; a wave whose
+// segment is empty leaves at once.
+__global__ void __launch_bounds__(kStagedBlock) bf_tier2_kernel(DevMesh m, KParams kp,
+                                                                const double* __restrict__ x,
+                                                                const double* __restrict__ y,
+                                                                const double* __restrict__ z) {
+  const int P = kp.n_qp, per_wave = 64 / P;
+  int slot, q, base;
+  lane_group(P, threadIdx.x & 63, slot, q, base);
+  const int wave = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
+  if ((long long)wave * per_wave >= kp.E) return;
+  const int n = m.wcount[wave];
+  if (n == 0) return;
+  const bool mine = slot < n;
+  const int e = mine ? m.wlist[(size_t)wave * per_wave + slot] : 0;
+  bool take = false;
+  if (mine) take = staged_element<double>(m, kp, x, y, z, e, q, base);
+  WaveSink{m.wlist, m.wcount, per_wave, wave}.push(take, e);
 }
 
 // Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
@@ -278,7 +305,8 @@ __global__ void __launch_bounds__(256) bf_batch_prologue(BatchDev b, int ghosts)
 // them in passes of floor(64 / P) element groups.  Every tile belongs to one case, so the
 // case's mesh and options are wave-uniform.
 template <class R>
-__global__ void __launch_bounds__(kStagedBlock) bf_batch_element_kernel(BatchDev b) {
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(2)))
+bf_batch_element_kernel(BatchDev b) {
   const int P = b.P, per_wave = 64 / P;
   int slot, q, base;
   lane_group(P, threadIdx.x & 63, slot, q, base);
@@ -293,9 +321,40 @@ __global__ void __launch_bounds__(kStagedBlock) bf_batch_element_kernel(BatchDev
     int pushed = 0;
     for (int k0 = 0; k0 < n; k0 += per_wave) {
       const int i = k0 + slot;
-      if (slot < per_wave && i < n)
-        staged_element<R>(cd.m, cd.kp, cd.x, cd.y, cd.z, e0 + i, q, base,
-                          TileSink{b.tlist + (size_t)t * b.tile_cap, &pushed});
+      if (slot < per_wave && i < n) {
+        const bool take = staged_element<R>(cd.m, cd.kp, cd.x, cd.y, cd.z, e0 + i, q, base);
+        TileSink{b.tlist + (size_t)t * b.tile_cap, &pushed}.push(take, e0 + i);
+      }
+    }
+    if ((threadIdx.x & 63) == 0) b.tcount[t] = pushed;
+  }
+}
+
+// Second tier of the FP32-mixed batch path (as bf_tier2_kernel): the wave of tile t
+// re-evaluates the tile's hand-over segment with the FP64 staged chain in passes and
+// rewrites the segment in place with what remains for the reference-order kernel (a pass
+// reads its entries before it appends, and appends never pass the read position).
+__global__ void __launch_bounds__(kStagedBlock) bf_batch_tier2_kernel(BatchDev b) {
+  const int P = b.P, per_wave = 64 / P;
+  int slot, q, base;
+  lane_group(P, threadIdx.x & 63, slot, q, base);
+  int t0, t1, step;
+  batch_tile_range(b, t0, t1, step);
+  for (int t = t0; t < t1; t += step) {
+    const int n = b.tcount[t];
+    if (n == 0) continue;
+    const Tile tl = b.etiles[t];
+    const int c = __builtin_amdgcn_readfirstlane(tl.c);
+    const CaseDev& cd = b.cases[c];
+    int* seg = b.tlist + (size_t)t * b.tile_cap;
+    int pushed = 0;
+    for (int k0 = 0; k0 < n; k0 += per_wave) {
+      const int i = k0 + slot;
+      if (slot < per_wave && i < n) {
+        const int e = seg[i];
+        const bool take = staged_element<double>(cd.m, cd.kp, cd.x, cd.y, cd.z, e, q, base);
+        TileSink{seg, &pushed}.push(take, e);
+      }
     }
     if ((threadIdx.x & 63) == 0) b.tcount[t] = pushed;
   }
@@ -347,6 +406,16 @@ __global__ void __launch_bounds__(256) bf_ghost_kernel(const int* __restrict__ r
   x[n_v + g] = x[b] + x[a] - x[c];
   y[n_v + g] = y[b] + y[a] - y[c];
   z[n_v + g] = z[b] + z[a] - z[c];
+}
+
+// Status words of every case of a batch: first non-finite element and first CC3333 trip.
+__global__ void __launch_bounds__(256) bf_batch_case_status_kernel(const CaseDev* __restrict__ cases,
+                                                                   int n, int* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int* st = cases[c].m.status;
+  out[2 * c] = st[0];
+  out[2 * c + 1] = st[2];
 }
 
 __global__ void bf_status_reset(int* st) {
@@ -846,6 +915,11 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   if (c->E > 0 && precision != BF_PREC_FP64 && kp.fallback) {
     const int per_wave = 64 / kp.n_qp;
     const int n_waves = (c->E + per_wave - 1) / per_wave;
+    if (precision == BF_PREC_MIXED) {
+      const int per_block = (kStagedBlock / 64) * per_wave;
+      hipLaunchKernelGGL(bf_tier2_kernel, dim3((c->E + per_block - 1) / per_block),
+                         dim3(kStagedBlock), 0, s, dm, kp, x, y, z);
+    }
     hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, c->d_wcount,
                        c->d_wlist, n_waves, per_wave, c->d_fallback, c->d_status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, s));
@@ -1172,6 +1246,7 @@ struct bf_batch {
   Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr, *d_xtiles = nullptr;
   int *d_tlist = nullptr, *d_tcount = nullptr, *d_htotal = nullptr, *d_rtotal = nullptr;
   int2 *d_hlist = nullptr, *d_rlist = nullptr;
+  int* d_cstat = nullptr;  // [2 n_cases]: per-case status words read by bf_batch_case_status
   int n_etiles = 0, n_ntiles = 0, n_gtiles = 0, n_xtiles = 0, tile_cap = 0;
   long long E_total = 0;
   int* h_total = nullptr;  // pinned
@@ -1187,7 +1262,7 @@ void batch_free_lists(bf_batch* b) {
   dfree(b->d_cases); dfree(b->d_etiles); dfree(b->d_ntiles); dfree(b->d_gtiles);
   dfree(b->d_xtiles);
   dfree(b->d_tlist); dfree(b->d_tcount); dfree(b->d_hlist); dfree(b->d_htotal);
-  dfree(b->d_rlist); dfree(b->d_rtotal);
+  dfree(b->d_rlist); dfree(b->d_rtotal); dfree(b->d_cstat);
   b->n_etiles = b->n_ntiles = b->n_gtiles = b->n_xtiles = 0;
 }
 
@@ -1258,6 +1333,7 @@ int batch_build(bf_batch* b) {
   BALLOC(b->d_htotal, 1);
   BALLOC(b->d_rlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_rtotal, 2);
+  BALLOC(b->d_cstat, 2 * cases.size());
 #undef BALLOC
   hipStream_t s = b->stream;
   BF_HIP(hipMemcpyAsync(b->d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, s));
@@ -1309,10 +1385,12 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
     // the staged kernels hold two waves per SIMD: 8 per CU
     int blocks = b->blocks > 0 ? b->blocks : (b->n_cu * 8 + wpb - 1) / wpb;
     if (!b->persistent) blocks = (b->n_etiles + wpb - 1) / wpb;
-    if (precision == BF_PREC_FP64_FAST)
+    if (precision == BF_PREC_FP64_FAST) {
       hipLaunchKernelGGL(bf_batch_element_kernel<double>, dim3(blocks), dim3(b->block), 0, s, d);
-    else
+    } else {
       hipLaunchKernelGGL(bf_batch_element_kernel<float>, dim3(blocks), dim3(b->block), 0, s, d);
+      hipLaunchKernelGGL(bf_batch_tier2_kernel, dim3(blocks), dim3(b->block), 0, s, d);
+    }
     hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d);
     BF_HIP(launch_batch_exact_list(d, b->n_cu, s));
   }
@@ -1472,6 +1550,26 @@ int bf_batch_last_fallback_count(bf_batch* b, int* n) {
   BF_HIP(hipMemcpyAsync(b->h_total, b->d_htotal, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   BF_HIP(hipStreamSynchronize(b->stream));
   *n = *b->h_total;
+  return BF_OK;
+}
+
+int bf_batch_case_status(bf_batch* b, int* first_nonfinite, int* first_cc3333) {
+  if (!b) return BF_ERR_ARG;
+  const int n = (int)b->cases.size();
+  if (n == 0) return BF_OK;
+  if (!b->d_cstat) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(b->device));
+  std::vector<int> h((size_t)2 * n);
+  hipLaunchKernelGGL(bf_batch_case_status_kernel, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                     (const CaseDev*)b->d_cases, n, b->d_cstat);
+  BF_HIP(hipGetLastError());
+  BF_HIP(hipMemcpyAsync(h.data(), b->d_cstat, sizeof(int) * 2 * n, hipMemcpyDeviceToHost,
+                        b->stream));
+  BF_HIP(hipStreamSynchronize(b->stream));
+  for (int c = 0; c < n; c++) {
+    if (first_nonfinite) first_nonfinite[c] = h[2 * c] == INT_MAX ? -1 : h[2 * c];
+    if (first_cc3333) first_cc3333[c] = h[2 * c + 1] == INT_MAX ? -1 : h[2 * c + 1];
+  }
   return BF_OK;
 }
 

@@ -240,6 +240,11 @@ int bf_batch_sync(bf_batch *batch);
 int bf_batch_last_ms(bf_batch *batch, float *ms);
 /* Elements of all cases the last staged step recomputed in the reference's order. */
 int bf_batch_last_fallback_count(bf_batch *batch, int *n);
+/* Per case of the batch (n_cases entries each, either array may be NULL): the first element
+ * whose force was not finite and the first element whose CC3333 guard tripped in the last
+ * step, -1 for none -- what bf_force_eval reports per context as BF_ERR_ELEMENT and the
+ * warning count.  Lets a request loop retire a case whose state diverged.  Synchronises. */
+int bf_batch_case_status(bf_batch *batch, int *first_nonfinite, int *first_cc3333);
 int bf_batch_info(bf_batch *batch, int *n_cases, long long *n_elmt, int *n_tiles, int *tile_cap);
 /* Launch shape of the element phase: `passes` element groups per work item (a tile holds
  * passes * floor(64 / n_qp) elements), threads per block (64, 128 or 256), blocks

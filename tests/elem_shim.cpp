@@ -80,7 +80,14 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
         cc = rf.c33; stt = rf.status; fin = std::isfinite(rf.N0) && std::isfinite(rf.M0);
         N0 = rf.N0; N1 = rf.N1; N2 = rf.N2; M0 = rf.M0; M1 = rf.M1; M2 = rf.M2;
       }
-      if (needs_exact(cc, n_qp, stt, fin)) {
+      bool hand = needs_exact(cc, n_qp, stt, fin, precision == 5);
+      if (hand && precision == 5) {  // second tier of the mixed path: the FP64 staged chain
+        rd = integrate_element<double>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp, iters, theta, wq);
+        cc = rd.c33; stt = rd.status; fin = std::isfinite(rd.N0) && std::isfinite(rd.M0);
+        N0 = rd.N0; N1 = rd.N1; N2 = rd.N2; M0 = rd.M0; M1 = rd.M1; M2 = rd.M2;
+        hand = needs_exact(cc, n_qp, stt, fin, false);
+      }
+      if (hand) {
         do_exact = true;
         warn++;
       } else {

@@ -135,3 +135,34 @@ def test_batch_tuner_and_launch_shapes():
         b.close()
         for fe in fes:
             fe.close()
+
+
+@pytest.mark.parametrize("prec", [L.BF_PREC_FP64, L.BF_PREC_MIXED])
+def test_batch_case_status_flags_diverged_case(prec):
+    """A case whose coordinates hold a NaN reports its first non-finite element through
+    bf_batch_case_status; the other cases report none, and the flags clear once the case's
+    coordinates are finite again."""
+    cases = _cases(4)
+    print(f"case batch status, prec {prec}", flush=True)
+    fes = _resident(cases)
+    m, p, (x, y, z) = cases[2]
+    bad = x.copy()
+    bad[5] = np.nan
+    b = Batch(fes)
+    try:
+        upload_coords(fes[2], bad, y, z)
+        b.eval(prec, ghosts=False, graph=False)
+        b.sync()
+        nf, cc = b.case_status()
+        assert nf[2] >= 0 and (nf[[0, 1, 3]] == -1).all(), nf
+        flags = fes[2].elem_flags()
+        assert flags[nf[2]] & L.FLAG_NONFINITE
+        upload_coords(fes[2], x, y, z)
+        b.eval(prec, ghosts=False, graph=False)
+        b.sync()
+        nf, cc = b.case_status()
+        assert (nf == -1).all() and (cc == -1).all(), (nf, cc)
+    finally:
+        b.close()
+        for fe in fes:
+            fe.close()

@@ -37,8 +37,15 @@ def check_all(m, p, x, y, z, tol_exact=0.0, tol_fast=1e-12, tol_mixed=1e-5, omp=
             assert np.array_equal(fe.c33(), c33)
         fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
         nf = fe.fallback_count()
+        # handed-over elements run in the reference order: bit for bit the oracle's C33
+        handed = (fe.elem_flags() & L.FLAG_FALLBACK) != 0
+        assert int(handed.sum()) == nf
+        assert np.array_equal(fe.c33()[handed], c33[handed])
         mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
         nm = fe.fallback_count()
+        handed = (fe.elem_flags() & L.FLAG_FALLBACK) != 0
+        assert int(handed.sum()) == nm
+        assert np.array_equal(fe.c33()[handed], c33[handed])
         fe.set_fallback(False)
         fast0 = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
         mixed0 = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
@@ -69,6 +76,20 @@ def test_membrane_large_staged_paths():
     m, p, xyz = configs.membrane_large()
     print("case membrane 256x192 staged paths", flush=True)
     check_all(m, p, *xyz, omp=16)
+
+
+@pytest.mark.parametrize("sigma", [7e-4, 1.2e-3, 1.5e-3])
+def test_membrane_handover_sizes(sigma):
+    """Hand-over lists of about 1k, 12k and 19k of the 98 304 elements (current z noise
+    N(0, sigma), seed 11): the hand-over solve picks 8, 2 and 1 lanes per (element, qp)
+    for them on a 256-CU device; every handed-over element stays bit for bit the oracle's."""
+    m, p, (x, y, z) = configs.membrane_large()
+    rng = np.random.default_rng(11)
+    z = m.z0.copy()
+    z[: m.n_v] += rng.normal(0.0, sigma, m.n_v)
+    m.apply_ghosts(x, y, z)
+    print(f"case membrane 256x192 z noise {sigma}", flush=True)
+    check_all(m, p, x, y, z, omp=16)
 
 
 @pytest.mark.parametrize("mode", ["isotropic", "manufactured", "ramp3_delay", "ramp1"])

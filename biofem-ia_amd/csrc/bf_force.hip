@@ -43,7 +43,7 @@ __device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& 
                                                const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z, int e, int q,
-                                               int base) {
+                                               int base, double noise_max = kC33NoiseMax) {
   const int E = kp.E;
   const int n_qp = kp.n_qp;
   if (q >= n_qp) return false;
@@ -123,7 +123,7 @@ __device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& 
                         theta_q, own, kp.fallback != 0);
   else
     s = staged_qp<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters,
-                     theta_q, own, kp.fallback != 0);
+                     theta_q, own, kp.fallback != 0, noise_max);
   // ---- group: resultants in qp order, status bits ----
   Resultants<R> r;
   r.N0 = r.N1 = r.N2 = r.M0 = r.M1 = r.M2 = 0.0;
@@ -204,10 +204,11 @@ __global__ void __launch_bounds__(kStagedBlock) bf_element_kernel(DevMesh m, KPa
 
 // Second tier of the FP32-mixed path: wave w re-evaluates the elements the FP32 kernel's
 // wave w handed over (its segment of m.wlist) with the FP64 staged chain and keeps in the
-// segment only those that chain cannot hold either, for the reference-order kernel.  Most
-// of them have C33 between the FP32 path's range and the FP64 path's; the hand-over set of
-// the mixed path is then the FP64 path's.  Same grid as the element kernel; a wave whose
-// segment is empty leaves at once.
+// segment only those that chain cannot hold within the mixed path's budget either, for the
+// reference-order kernel: the FP32 chain keeps the well-conditioned elements (rounding
+// bound 1e-11, C33 in [0.4, 2.5]), the FP64 chain the moderately conditioned ones (bound
+// 1e-8, C33 in [0.1, 10]).  Same grid as the element kernel; a wave whose segment is empty
+// leaves at once.
 __global__ void __launch_bounds__(kStagedBlock) bf_tier2_kernel(DevMesh m, KParams kp,
                                                                 const double* __restrict__ x,
                                                                 const double* __restrict__ y,
@@ -222,7 +223,7 @@ __global__ void __launch_bounds__(kStagedBlock) bf_tier2_kernel(DevMesh m, KPara
   const bool mine = slot < n;
   const int e = mine ? m.wlist[(size_t)wave * per_wave + slot] : 0;
   bool take = false;
-  if (mine) take = staged_element<double>(m, kp, x, y, z, e, q, base);
+  if (mine) take = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
   WaveSink{m.wlist, m.wcount, per_wave, wave}.push(take, e);
 }
 
@@ -352,7 +353,8 @@ __global__ void __launch_bounds__(kStagedBlock) bf_batch_tier2_kernel(BatchDev b
       const int i = k0 + slot;
       if (slot < per_wave && i < n) {
         const int e = seg[i];
-        const bool take = staged_element<double>(cd.m, cd.kp, cd.x, cd.y, cd.z, e, q, base);
+        const bool take = staged_element<double>(cd.m, cd.kp, cd.x, cd.y, cd.z, e, q, base,
+                                                 kC33NoiseMaxTier2);
         TileSink{seg, &pushed}.push(take, e);
       }
     }

@@ -82,7 +82,8 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
       }
       bool hand = needs_exact(cc, n_qp, stt, fin, precision == 5);
       if (hand && precision == 5) {  // second tier of the mixed path: the FP64 staged chain
-        rd = integrate_element<double>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp, iters, theta, wq);
+        rd = integrate_element<double>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp,
+                                       iters, theta, wq, kC33NoiseMaxTier2);
         cc = rd.c33; stt = rd.status; fin = std::isfinite(rd.N0) && std::isfinite(rd.M0);
         N0 = rd.N0; N1 = rd.N1; N2 = rd.N2; M0 = rd.M0; M1 = rd.M1; M2 = rd.M2;
         hand = needs_exact(cc, n_qp, stt, fin, false);

@@ -80,9 +80,10 @@ def test_membrane_large_staged_paths():
 
 @pytest.mark.parametrize("sigma", [7e-4, 1.2e-3, 1.5e-3])
 def test_membrane_handover_sizes(sigma):
-    """Hand-over lists of about 1k, 12k and 19k of the 98 304 elements (current z noise
-    N(0, sigma), seed 11): the hand-over solve picks 8, 2 and 1 lanes per (element, qp)
-    for them on a 256-CU device; every handed-over element stays bit for bit the oracle's."""
+    """FP64 hand-over lists of about 1k, 11k and 19k of the 98 304 elements (current z noise
+    N(0, sigma), seed 11; the mixed path's are shorter): the hand-over solve picks 8, 2 and 1
+    lanes per (element, qp) for them on a 256-CU device; every handed-over element stays bit
+    for bit the oracle's."""
     m, p, (x, y, z) = configs.membrane_large()
     rng = np.random.default_rng(11)
     z = m.z0.copy()

@@ -20,6 +20,7 @@ BF_ERR_NOMEM = -3
 BF_ERR_ELEMENT = -4
 BF_ERR_STATE = -5
 BF_ERR_UNSUPPORTED = -6
+BF_ERR_IO = -7
 ERR_NAMES = {
     BF_ERR_ARG: "BF_ERR_ARG",
     BF_ERR_DEVICE: "BF_ERR_DEVICE",
@@ -27,6 +28,7 @@ ERR_NAMES = {
     BF_ERR_ELEMENT: "BF_ERR_ELEMENT",
     BF_ERR_STATE: "BF_ERR_STATE",
     BF_ERR_UNSUPPORTED: "BF_ERR_UNSUPPORTED",
+    BF_ERR_IO: "BF_ERR_IO",
 }
 BF_ELEM_CC3333 = 6
 BF_ELEM_NONFINITE = 7
@@ -68,7 +70,18 @@ class BfParams(C.Structure):
     ]
 
 
-# name -> (restype, argtypes); every symbol include/biofem_hip.h declares
+class BfVtkFrame(C.Structure):
+    _fields_ = [
+        ("n_v", C.c_int), ("n_nodes", C.c_int), ("n_elmt", C.c_int),
+        ("x", _dp), ("y", _dp), ("z", _dp), ("x0", _dp), ("y0", _dp), ("z0", _dp),
+        ("nv1", _ip), ("nv2", _ip), ("nv3", _ip), ("contact", _ip), ("m", _dp), ("IE", _dp),
+        ("Fint", _dp), ("Fext", _dp), ("Fdyn", _dp), ("Fcnt", _dp), ("u", _dp), ("xdd", _dp),
+        ("dA", _dp), ("dA0", _dp), ("n_fib", _dp), ("gamma", _dp),
+        ("muscle_activation", C.c_int), ("manufactured", C.c_int), ("lv_geom_process", C.c_int),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/biofem_hip.h and biofem_io.h declare
 SIGNATURES = {
     "bf_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "bf_ctx_destroy": (C.c_int, [_vp]),
@@ -104,6 +117,16 @@ SIGNATURES = {
     "bf_batch_last_fallback_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "bf_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bf_batch_case_status": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bf_io_dimension": (C.c_int, [C.c_char_p, C.c_int, _ip, _ip, _ip, _ip, _ip]),
+    "bf_io_input": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _ip,
+                              _ip, _ip, _ip]),
+    "bf_io_write_lists": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _ip,
+                                    _ip, _ip, C.c_int, _ip, _ip, C.c_int]),
+    "bf_io_write_vtk": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(BfVtkFrame)]),
+    "bf_io_write_state": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                    C.c_int]),
+    "bf_io_read_state": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                   _ip]),
     "bf_batch_set_launch": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "bf_batch_get_launch": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bf_batch_tune": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),

@@ -31,6 +31,7 @@ extern "C" {
 #define BF_ERR_ELEMENT (-4)     /* PETSC_ERR_PLIB  : per-element failure, see bf_last_error */
 #define BF_ERR_STATE (-5)       /* call order (e.g. eval before mesh upload) */
 #define BF_ERR_UNSUPPORTED (-6) /* PETSC_ERR_SUP   : e.g. patch wider than 18 nodes */
+#define BF_ERR_IO (-7)          /* PETSC_ERR_FILE_* : missing, unwritable or malformed file */
 
 /* per-element codes reported by bf_last_error() */
 #define BF_ELEM_STENCIL 2   /* patch slot empty (active_strain.c:498-500, :577-579) */

@@ -17,14 +17,14 @@ def declared(header):
 
 def test_every_declared_symbol_is_exported():
     lib = L.lib()
-    names = declared("biofem_hip.h") + declared("biofem_fe.h")
+    names = declared("biofem_hip.h") + declared("biofem_fe.h") + declared("biofem_io.h")
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
 
 
 def test_python_binding_covers_the_abi():
-    names = set(declared("biofem_hip.h"))
+    names = set(declared("biofem_hip.h")) | set(declared("biofem_io.h"))
     assert names <= set(L.SIGNATURES), names - set(L.SIGNATURES)
 
 

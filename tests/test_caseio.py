@@ -1,0 +1,200 @@
+"""The reference's on-disk formats around the force path (include/biofem_io.h, host C):
+a case directory in `biofem sweep`'s layout (tests/golden/case_rect3x2, written by
+tests/golden/make_case_fixture.py) parsed as Dimension / Input read it, Input's neighbour
+search against its O(E^2) restatement, write/read round trips, the VTK frame against an
+independent formatting of Output's printf sequence, and the checkpoint bytes against
+PETSc's big-endian binary layout."""
+import importlib.util
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from biofem_amd import _lib as L
+from biofem_amd import caseio
+from biofem_amd.mesh import area0, flat_membrane
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASE = os.path.join(HERE, "golden", "case_rect3x2", "input")
+
+
+def _fixture():
+    spec = importlib.util.spec_from_file_location(
+        "make_case_fixture", os.path.join(HERE, "golden", "make_case_fixture.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.mesh()
+
+
+def _neighbours_bruteforce(tri):
+    """Input's search (src/io.c:476-519) as written: for every i, every j in increasing
+    order; mn counts i's corners found in j, cn adds 3.5 / 2.5 / 1.5 for corners 1 / 2 / 3;
+    mn == 2 writes j's other corner to nv4 (cn 4), nv5 (cn 5) or nv6."""
+    E = len(tri)
+    out = np.full((E, 3), 1000000, np.int64)
+    for i in range(E):
+        n1e, n2e, n3e = tri[i]
+        for j in range(E):
+            n1pe, n2pe, n3pe = tri[j]
+            mn, cn = 0, 0.0
+            if n1e in (n1pe, n2pe, n3pe):
+                mn += 1; cn += 3.5
+            if n2e in (n1pe, n2pe, n3pe):
+                mn += 1; cn += 2.5
+            if n3e in (n1pe, n2pe, n3pe):
+                mn += 1; cn += 1.5
+            if mn == 2:
+                if n1pe not in (n1e, n2e, n3e):
+                    npe = n1pe
+                elif n2pe not in (n1e, n2e, n3e):
+                    npe = n2pe
+                else:
+                    npe = n3pe
+                out[i, 0 if cn == 4.0 else (1 if cn == 5.0 else 2)] = npe
+    return out
+
+
+def _write_lists(d, x, y, z, tri, rings=()):
+    lib = L.lib()
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    tri = np.asarray(tri)
+    nb = i32([len(r) for r in rings] or [0])
+    bn = i32([v for r in rings for v in r] or [0])
+    L.check("bf_io_write_lists", lib.bf_io_write_lists(
+        os.fsencode(d), 0, len(x), L.dp(f64(x)), L.dp(f64(y)), L.dp(f64(z)), len(tri),
+        L.ip(i32(tri[:, 0])), L.ip(i32(tri[:, 1])), L.ip(i32(tri[:, 2])), len(rings), L.ip(nb),
+        L.ip(bn), int(sum(len(r) for r in rings))))
+
+
+def test_sweep_case_directory_parses_as_input_reads_it():
+    pts, tri, sides = _fixture()
+    d = caseio.dimension(CASE)
+    assert d == dict(n_v=12, n_elmt=12, n_edge=4, n_ghosts=10, sum_n_bnodes=14)
+    r = caseio.read_lists(CASE)
+    assert np.array_equal(r["x"], [float(f"{p[0]:.6f}") for p in pts])
+    assert np.array_equal(r["y"], [float(f"{p[1]:.6f}") for p in pts])
+    assert np.array_equal(r["z"], np.zeros(12))
+    assert np.array_equal(r["tri"], np.array(tri))
+    assert np.array_equal(r["n_bnodes"], [len(s) for s in sides])
+    assert np.array_equal(r["bnodes"], [v for s in sides for v in s])
+    assert np.array_equal(r["nv456"], _neighbours_bruteforce(tri))
+
+
+def test_char_length_divides_the_coordinates():
+    pts, _, _ = _fixture()
+    r = caseio.read_lists(CASE, char_length=(2.0, 4.0, 8.0))
+    assert np.array_equal(r["x"], [float(f"{p[0]:.6f}") / 2.0 + 0.0 for p in pts])
+    assert np.array_equal(r["y"], [float(f"{p[1]:.6f}") / 4.0 + 0.0 for p in pts])
+
+
+def test_neighbour_search_last_match_wins(tmp_path):
+    """A random lattice plus two extra triangles on one interior edge: the edge table
+    reproduces the search, including its highest-numbered-neighbour rule."""
+    m = flat_membrane(6, 5, jitter=0.05, seed=4)
+    tri = np.asarray(m.tri[: m.n_elmt]).copy()
+    a, b = tri[7][0], tri[7][2]            # an interior diagonal
+    c1, c2 = [v for v in (20, 25, 40) if v not in (a, b)][:2]
+    extra = np.array([[b, a, c1], [a, b, c2]])
+    tri = np.concatenate([tri, extra])
+    _write_lists(str(tmp_path), m.x0[: m.n_v], m.y0[: m.n_v], m.z0[: m.n_v], tri)
+    r = caseio.read_lists(str(tmp_path))
+    assert np.array_equal(r["tri"], tri)
+    assert np.array_equal(r["nv456"], _neighbours_bruteforce(tri))
+
+
+def test_read_case_runs_the_reference_setup():
+    m = caseio.read_case(CASE)
+    assert m.extra["rings"] == [[0, 1, 2, 3, 7, 11, 10, 9, 8, 4]]  # four open sides joined
+    assert m.n_nodes == 12 + 10 and m.tri.shape == (12 + 20, 3)
+    x, y, z = m.x0, m.y0, m.z0
+    for g, (b, a, c) in enumerate(m.ghost_rules):  # GlobalGhost mirror rule
+        assert x[12 + g] == x[b] + x[a] - x[c] and y[12 + g] == y[b] + y[a] - y[c]
+    assert np.array_equal(m.dA0, area0(m.tri[:12], x, y, z))
+    assert (m.patch[:, :3] < m.n_nodes).all()
+
+
+def test_write_case_round_trip(tmp_path):
+    m = flat_membrane(5, 4, jitter=0.03, seed=1)
+    caseio.write_case(str(tmp_path), m)
+    r = caseio.read_lists(str(tmp_path))
+    assert np.array_equal(r["x"], m.x0[: m.n_v]) and np.array_equal(r["y"], m.y0[: m.n_v])
+    assert np.array_equal(r["tri"], m.tri[: m.n_elmt])
+    rings = caseio.boundary_rings(m.tri[: m.n_elmt])
+    assert len(rings) == 1 and len(rings[0]) == 2 * (5 + 4)
+    assert np.array_equal(r["bnodes"], rings[0])
+    # the written case loads through the reference's setup
+    mc = caseio.read_case(str(tmp_path))
+    assert mc.n_nodes == m.n_v + 18
+
+
+def _expected_vtk(m, x, y, z, fint, u, mass, gamma, dA):
+    """Output's sequence (src/io.c:612-800) formatted independently of the C writer."""
+    n, E = m.n_v, m.n_elmt
+    T = m.tri[:E]
+    z3 = np.zeros(3 * m.n_nodes)
+    out = ["# vtk DataFile Version 2.0", "Surface Grid", "ASCII", "DATASET UNSTRUCTURED_GRID",
+           f"POINTS  {n} float"]
+    out += ["%f %f %f" % (x[i], y[i], z[i]) for i in range(n)]
+    out += [f"CELLS {E} {4 * E}"] + ["%d  %d %d %d" % (3, *T[e]) for e in range(E)]
+    out += [f"CELL_TYPES {E}"] + ["5"] * E
+    out += [f"POINT_DATA {n}", "SCALARS contact integer", "LOOKUP_TABLE default"] + ["0"] * n
+    out += ["SCALARS m float", "LOOKUP_TABLE default"] + ["%f " % mass[i] for i in range(n)]
+    out += ["SCALARS IE float", "LOOKUP_TABLE default"] + ["%f " % 0.0] * n
+    out += ["VECTORS disp float"] + ["%f %f %f" % (x[i] - m.x0[i], y[i] - m.y0[i], z[i] - m.z0[i])
+                                     for i in range(n)]
+    vec = lambda lab, v: [f"VECTORS {lab} float"] + ["%f %f %f" % tuple(v[3 * i:3 * i + 3]) for i in range(n)]
+    out += vec("Fint", fint) + vec("Fext", z3) + vec("Fdyn", z3) + vec("Fcnt", z3) + vec("u", u)
+    out += vec("Res", fint)
+    out += [f"CELL_DATA {E}", "SCALARS dA float", "LOOKUP_TABLE default"] + ["%f " % a for a in dA]
+    out += ["SCALARS dA0 float", "LOOKUP_TABLE default"] + ["%f " % a for a in m.dA0]
+    out += ["SCALARS gamma float", "LOOKUP_TABLE default"] + ["%f " % g for g in gamma]
+    return "\n".join(out) + "\n"
+
+
+def test_vtk_frame_follows_output(tmp_path):
+    m = flat_membrane(4, 3, jitter=0.05, seed=2)
+    rng = np.random.default_rng(3)
+    x, y, z = m.x0 + 0.01 * rng.standard_normal(m.n_nodes), m.y0.copy(), m.z0 + 0.02
+    fint = rng.standard_normal(3 * m.n_nodes)
+    u = rng.standard_normal(3 * m.n_nodes)
+    mass = rng.uniform(0, 1, m.n_v)
+    gamma = rng.uniform(0, 0.2, m.n_elmt)
+    path = caseio.write_vtk(str(tmp_path), m, 7, x, y, z, fint=fint, u=u, mass=mass,
+                            gamma=gamma, muscle_activation=True)
+    assert os.path.basename(path) == "surface00_00007.vtk"
+    dA = area0(m.tri[: m.n_elmt], x, y, z)
+    assert open(path).read() == _expected_vtk(m, x, y, z, fint, u, mass, gamma, dA)
+
+
+def test_state_checkpoints_use_petsc_binary_layouts(tmp_path):
+    n_total = 7
+    rng = np.random.default_rng(5)
+    v = {t: rng.standard_normal(3 * n_total) for t in ("x", "xn", "xd", "xdd")}
+    d = str(tmp_path)
+    caseio.write_state(d, 12, n_total, v["x"], v["xn"], v["xd"], v["xdd"], ibi=1,
+                       layout=caseio.STATE_RAW)
+    raw = open(os.path.join(d, "xn1_00012.dat"), "rb").read()
+    assert raw == np.asarray(v["xn"], ">f8").tobytes()
+    got, lay = caseio.read_state(d, 12, n_total, ibi=1)
+    assert lay == caseio.STATE_RAW and all(np.array_equal(got[t], v[t]) for t in v)
+    caseio.write_state(d, 13, n_total, x=v["x"], layout=caseio.STATE_VECVIEW)
+    vv = open(os.path.join(d, "x0_00013.dat"), "rb").read()
+    assert vv == struct.pack(">ii", 1211214, 3 * n_total) + np.asarray(v["x"], ">f8").tobytes()
+    got, lay = caseio.read_state(d, 13, n_total, tags=("x",))
+    assert lay == caseio.STATE_VECVIEW and np.array_equal(got["x"], v["x"])
+    # a VecView file of a 64-bit-integer PETSc build
+    with open(os.path.join(d, "xd0_00014.dat"), "wb") as f:
+        f.write(struct.pack(">qq", 1211214, 3 * n_total) + np.asarray(v["xd"], ">f8").tobytes())
+    got, lay = caseio.read_state(d, 14, n_total, tags=("xd",))
+    assert lay == caseio.STATE_VECVIEW and np.array_equal(got["xd"], v["xd"])
+    # a checkpoint of another mesh size is refused
+    with pytest.raises(L.BfError):
+        caseio.read_state(d, 12, n_total + 1, ibi=1, tags=("x",))
+
+
+def test_missing_case_reports_io_error(tmp_path):
+    with pytest.raises(L.BfError) as e:
+        caseio.dimension(str(tmp_path))
+    assert "BF_ERR_IO" in str(e.value)

@@ -117,7 +117,9 @@ int bf_force_eval(bf_ctx *ctx, const double *x, const double *y, const double *z
                   double *fint, int accumulate, int precision);
 
 /* Same on device-resident arrays, asynchronous on `stream` (a hipStream_t, NULL = the
- * context's stream).  Status is collected by bf_last_error(). */
+ * context's stream, which is non-blocking: a caller whose work runs on the legacy NULL
+ * stream orders its own stream against the launches with events).  Status is collected
+ * by bf_last_error(). */
 int bf_force_eval_device(bf_ctx *ctx, const double *d_x, const double *d_y, const double *d_z,
                          double *d_fint, int accumulate, int precision, void *stream);
 

@@ -234,10 +234,15 @@ BF_HD double cc_sum_serial(const CcIn& c) {
 }
 struct CcSerial {
   BF_HD double operator()(const CcIn& c) const { return cc_sum_serial(c); }
+  // the qp's other sums: every entry is formed by this lane
+  BF_HD bool mine(int) const { return true; }
+  BF_HD double share(double v, int) const { return v; }
 };
 
 // One through-thickness point: ElemUpdateG, ElemActDefGrad, ElemCGDefTens and the C33
-// loop for qp `qp`.  ModElemC33 returns at the first qp whose |CC3333| fails the guard
+// loop for qp `qp`.  cc_sum forms CC^{2222}; lanes that solve the same qp together also
+// divide the nine Ce_inv entries among themselves (cc_sum.mine(entry): this lane forms it,
+// cc_sum.share(value, entry): the owner's value), each entry still one sum in order.  ModElemC33 returns at the first qp whose |CC3333| fails the guard
 // (:1674-1681), so in iteration `it` this qp keeps its C33 if it or any earlier qp failed:
 // skip(fail, it) receives this qp's guard result and answers that question (sequentially
 // from a bit mask, or across the lanes that hold the element's other qps).
@@ -307,12 +312,15 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
   for (int it = 0; it < in.iters; it++) {
     // ---- ElemElasCGDefTens :1389-1430: sum over m, p, q, l per entry ----
     Mat Ch, CeiCov, CeCont, CeCov;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) Ch.m[i][j] = 0.0;
 #pragma unroll 1
     for (int i = 0; i < 3; i++) {
       double Fr[3];
       row3(FaCont, i, Fr);
 #pragma unroll 1
       for (int j = 0; j < 3; j++) {
+        if (!cc_sum.mine(3 * i + j)) continue;
         double Fcj[3];
         col3(FaCont, j, Fcj);
         const double Fo[3] = {opaque(Fr[0]), opaque(Fr[1]), opaque(Fr[2])};
@@ -335,6 +343,10 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
         }
       }
     }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) Ch.m[i][j] = cc_sum.share(Ch.m[i][j], 3 * i + j);
     lower2(gm0, Ch, CeiCov);
     inv(CeiCov, CeCont);
     lower2(gm0, CeCont, CeCov);
@@ -351,7 +363,8 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
         const double term2 = in.K * Ch.m[i][j] * (Je * Je - Je);
         Se.m[i][j] = term1 + term2;
       }
-    // ---- ElemTotStress :1506-1528, the four entries later steps read ----
+    // ---- ElemTotStress :1506-1528, the four entries later steps read: S^22 in every
+    // iteration, the in-plane ones only in the last, whose values are the ones kept ----
     auto tot = [&](int i, int j) {
       const double half = opaque(0.5);
       double acc = 0.0;
@@ -367,9 +380,11 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
                      (Se.m[pp][z] * G.m[w][i] + Se.m[pp][i] * G.m[w][z]);
       return acc;
     };
-    S00 = tot(0, 0);
-    S11 = tot(1, 1);
-    S01 = tot(0, 1);
+    if (it == in.iters - 1) {
+      S00 = tot(0, 0);
+      S11 = tot(1, 1);
+      S01 = tot(0, 1);
+    }
     const double S22 = tot(2, 2);
     // ---- ElemElsTangMatTens :1548-1585 (alpha, beta; entries formed on demand) ----
     const double Je2 = Je * Je;
@@ -389,7 +404,8 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
     // ---- ModElemC33 :1669-1707 ----
     const bool fail = !(fabs(CC) > 1e-14);
     if (fail) out.status |= kStatCC3333;
-    if (fail && stop_on_guard) break;  // the caller redoes the element with coupled qps
+    // the caller redoes the element with coupled qps and drops this qp's stresses
+    if (fail && stop_on_guard) break;
     if (!skip(fail, it)) {
       double DeltaC33 = -1.0 * S22 / CC;
       const double C33_old = CCov.m[2][2];

@@ -262,6 +262,11 @@ struct CcSplit {
     }
     return __shfl(CC, lead);
   }
+  // Ce_inv entry k is formed by lane k % K of the group
+  __device__ bool mine(int k) const { return k % K == j; }
+  __device__ double share(double v, int k) const {
+    return __shfl(v, (int)(threadIdx.x & 63) - j + k % K);
+  }
 };
 
 // Split factor for a hand-over list of N elements: the largest K whose waves for the
@@ -301,12 +306,18 @@ __device__ __forceinline__ void exact_solve_k(const DevMesh& m, const KParams& k
 }
 
 // 2. finish: element e from the solve's qp results; guard trips go to the redo list.
+// kFinishLanes consecutive lanes share an element (the same e, so the same branches): lane
+// `slice` writes nodes slice, slice + kFinishLanes, ...; the lanes combine their finite
+// flags and slice 0 does the element's bookkeeping.
+constexpr int kFinishLanes = 4;
+
 template <class Redo>
 __device__ __forceinline__ void exact_finish(const DevMesh& m, const KParams& kp,
                                              const double* __restrict__ x,
                                              const double* __restrict__ y,
                                              const double* __restrict__ z, int e,
-                                             unsigned char extra_flags, const Redo& redo) {
+                                             unsigned char extra_flags, const Redo& redo,
+                                             int slice) {
   const int E = kp.E;
   int status = 0;
   double Sv[kMaxQp][3];
@@ -323,13 +334,15 @@ __device__ __forceinline__ void exact_finish(const DevMesh& m, const KParams& kp
     }
   }
   if (status & kStatCC3333) {  // the cross-qp guard rule matters: redo with coupled qps
-    redo(e);
+    if (slice == 0) redo(e);
     return;
   }
   exact::ElemIn in;
   const ExactRows r = exact_inputs(m, kp, x, y, z, e, in, false);
-  const bool finite = exact_forces(m, kp, e, r, in.cur, Sv, 0, 1);
-  exact_status(m, e, status, finite, extra_flags);
+  int bad = exact_forces(m, kp, e, r, in.cur, Sv, slice, kFinishLanes) ? 0 : 1;
+#pragma unroll
+  for (int k = 1; k < kFinishLanes; k <<= 1) bad |= __shfl_xor(bad, k, kFinishLanes);
+  if (slice == 0) exact_status(m, e, status, bad == 0, extra_flags);
 }
 
 // 3. redo: one lane group per element, lane q runs qp q; the guard of iteration it skips
@@ -472,10 +485,14 @@ __global__ void __launch_bounds__(256) bf_exact_finish_kernel(DevMesh m, KParams
                                                               const int* __restrict__ count,
                                                               int n_static,
                                                               unsigned char extra_flags) {
-  const int N = count ? *count : n_static;
+  const long long N = (count ? *count : n_static) * (long long)kFinishLanes;
   const RedoSink redo{m.redo, &m.status[5]};
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += gridDim.x * blockDim.x)
-    exact_finish(m, kp, x, y, z, list ? list[idx] : idx, extra_flags, redo);
+  // the stride is a multiple of kFinishLanes: an element's lanes stay together
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int idx = (int)(t / kFinishLanes), slice = (int)(t % kFinishLanes);
+    exact_finish(m, kp, x, y, z, list ? list[idx] : idx, extra_flags, redo, slice);
+  }
 }
 
 // Lane groups over the redo list (count read on the device; normally empty).
@@ -553,25 +570,30 @@ struct BatchRedoSink {
 // Finish over the element tiles (list = 0) or the hand-over list (list = 1).
 __global__ void __launch_bounds__(256) bf_batch_exact_finish_kernel(BatchDev b, int from_list,
                                                                     unsigned char extra_flags) {
-  const int stride = gridDim.x * blockDim.x;
+  // the stride is a multiple of kFinishLanes: an element's lanes stay together
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (from_list) {
-    const int N = *b.htotal;
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += stride) {
+    const long long N = *b.htotal * (long long)kFinishLanes;
+    for (long long t = t0; t < N; t += stride) {
+      const int idx = (int)(t / kFinishLanes), slice = (int)(t % kFinishLanes);
       const int2 v = b.hlist[idx];
       const CaseDev& cd = b.cases[v.x];
       exact_finish(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, extra_flags,
-                   BatchRedoSink{b.rlist, b.rtotal, v.x});
+                   BatchRedoSink{b.rlist, b.rtotal, v.x}, slice);
     }
     return;
   }
-  const long long total = (long long)b.n_xtiles * 64;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += stride) {
-    const int t = (int)(i >> 6), k = (int)(i & 63);
+  const long long total = (long long)b.n_xtiles * 64 * kFinishLanes;
+  for (long long i = t0; i < total; i += stride) {
+    const long long entry = i / kFinishLanes;
+    const int slice = (int)(i % kFinishLanes);
+    const int t = (int)(entry >> 6), k = (int)(entry & 63);
     const Tile tl = b.xtiles[t];
     if (k >= tl.n) continue;
     const CaseDev& cd = b.cases[tl.c];
     exact_finish(cd.m, cd.kp, cd.x, cd.y, cd.z, tl.b + k, extra_flags,
-                 BatchRedoSink{b.rlist, b.rtotal, tl.c});
+                 BatchRedoSink{b.rlist, b.rtotal, tl.c}, slice);
   }
 }
 
@@ -597,7 +619,7 @@ hipError_t launch_batch_exact(const BatchDev& b, int n_cu, hipStream_t s) {
   if (b.n_xtiles <= 0) return hipSuccess;
   const int items = b.n_xtiles * b.P;
   hipLaunchKernelGGL(bf_batch_exact_solve_kernel, dim3((items + 1) / 2), dim3(kExactBlock), 0, s, b);
-  const long long total = (long long)b.n_xtiles * 64;
+  const long long total = (long long)b.n_xtiles * 64 * kFinishLanes;
   const int fblocks = (int)std::min<long long>((total + 255) / 256, 8LL * n_cu);
   hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(fblocks), dim3(256), 0, s, b, 0,
                      (unsigned char)0);
@@ -611,7 +633,8 @@ hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, hipStream_t s) {
   // persistent: one wave per SIMD at the solve's occupancy of one
   hipLaunchKernelGGL(bf_batch_exact_solve_list_kernel, dim3(4 * n_cu), dim3(64), 0, s, b,
                      4 * n_cu);
-  hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(2 * n_cu), dim3(256), 0, s, b, 1,
+  // 64-lane blocks: a short hand-over list still spreads over the CUs
+  hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, b, 1,
                      (unsigned char)kStatFallback);
   hipLaunchKernelGGL(bf_batch_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b,
                      (unsigned char)kStatFallback);
@@ -709,7 +732,8 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
   hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
                      z, (const int*)m.fallback, (const int*)&m.status[4], &m.status[6],
                      4 * n_cu);
-  hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(2 * n_cu), dim3(256), 0, s, m, kp, x, y, z,
+  // 64-lane blocks: a short hand-over list still spreads over the CUs
+  hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x, y, z,
                      (const int*)m.fallback, (const int*)&m.status[4], 0,
                      (unsigned char)kStatFallback);
   hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,
@@ -723,8 +747,10 @@ hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const doubl
   const int items = ((kp.E + 63) / 64) * kp.n_qp;
   hipLaunchKernelGGL(bf_exact_solve_kernel, dim3((items + 1) / 2), dim3(kExactBlock), 0, s, m,
                      kp, x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E);
-  hipLaunchKernelGGL(bf_exact_finish_kernel, dim3((kp.E + 255) / 256), dim3(256), 0, s, m, kp,
-                     x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E, (unsigned char)0);
+  hipLaunchKernelGGL(bf_exact_finish_kernel,
+                     dim3((unsigned)(((long long)kp.E * kFinishLanes + 255) / 256)), dim3(256), 0,
+                     s, m, kp, x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E,
+                     (unsigned char)0);
   hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,
                      z, (unsigned char)0);
   return hipGetLastError();

@@ -190,6 +190,97 @@ def run_served(args, L, log):
     return out
 
 
+def run_implicit(args, L, log):
+    """Implicit Newmark steps of the 32 x 24 membrane (config of the 100-step case: x = 0
+    edge clamped, rho 1, dt 1e-2, gamma 1/2): each step a matrix-free Newton-Krylov solve
+    whose residuals evaluate the library's force on device (FP64 staged path).  With --cpu,
+    steps of the same solver over the oracle's force on the host cores are timed beside it."""
+    import torch
+    from biofem_amd import configs
+    from biofem_amd.fem import ActStrainFE
+    from biofem_amd.implicit import NewmarkJFNK, for_fe, lumped_masses
+    m, p, _ = configs.membrane_small()
+    rho, dt = 1.0, 1e-2
+    x0 = np.stack([m.x0, m.y0, m.z0], 1).reshape(-1)
+    fixed = np.zeros(3 * m.n_nodes, bool)
+    clamp = np.where(np.isclose(m.x0[: m.n_v], 0.0))[0]
+    for d in range(3):
+        fixed[3 * clamp + d] = True
+    mass = lumped_masses(m, rho, p.h0)
+    n_eval = [0]
+
+    def counted(f):
+        def g(U):
+            n_eval[0] += 1
+            return f(U)
+        return g
+
+    def stepper(force, device):
+        return NewmarkJFNK(counted(force), torch.as_tensor(x0, device=device),
+                           torch.as_tensor(mass, device=device), dt, n_v=m.n_v,
+                           fixed=torch.as_tensor(fixed))
+
+    def summary(infos, wall):
+        n = len(infos)
+        return dict(steps=n, ms_per_step=1e3 * wall / n,
+                    converged_steps=sum(i.converged for i in infos),
+                    newton_iterations_per_step=sum(i.iterations for i in infos) / n,
+                    gmres_iterations_per_step=sum(i.linear_iterations for i in infos) / n,
+                    residual_evals_per_step=n_eval[0] / n,
+                    max_final_rel_residual=max(i.fnorms[-1] / i.fnorms[0] for i in infos))
+
+    fe = ActStrainFE(m, p)
+    force = for_fe(fe, L.BF_PREC_FP64_FAST)
+    stepper(force, "cuda").step()          # warm-up on a throwaway state
+    st = stepper(force, "cuda")
+    n_eval[0] = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos, x_at = [], {}
+    for k in range(args.implicit_steps):
+        infos.append(st.step())
+        if k + 1 == args.implicit_cpu_steps:
+            x_at[k + 1] = st.x      # a new tensor each step: no copy inside the timed loop
+    torch.cuda.synchronize()
+    gpu = summary(infos, time.perf_counter() - t0)
+    fe.close()
+    log(f"implicit: {gpu}")
+    out = dict(workload="flat active-strain membrane 32x24 (1536 triangles), x = 0 edge clamped, "
+                        "implicit Newmark (gamma 1/2, beta 1/4), dt 1e-2, rho 1, Newton + GMRES(30) "
+                        "on finite-difference Jacobian products, PETSc's default tolerances",
+               gpu=gpu)
+    if args.cpu:
+        from oracle import bf_oracle as O
+        nth = args.cpu_threads or min(os.cpu_count() or 1, 64)
+        cached = O.Cached(m, p, omp=True)
+
+        def cpu_force(U):
+            V = U.numpy().reshape(-1, 3)
+            x, y, z = (np.ascontiguousarray(V[:, k]) for k in range(3))
+            m.apply_ghosts(x, y, z)
+            f = np.zeros(3 * m.n_nodes)
+            cached.eval(x, y, z, f, nthreads=nth)
+            f[3 * m.n_v:] = 0.0
+            return torch.from_numpy(f)
+
+        sc = stepper(cpu_force, "cpu")
+        n_eval[0] = 0
+        t0 = time.perf_counter()
+        cinfos = [sc.step() for _ in range(args.implicit_cpu_steps)]
+        cpu = summary(cinfos, time.perf_counter() - t0)
+        cached.close()
+        cpu.update(threads=nth, kind="port",
+                   sample=f"the first {len(cinfos)} steps of the same case, oracle force (OpenMP)")
+        if len(cinfos) in x_at:   # the device's state against the host's after as many steps
+            xg = x_at[len(cinfos)].cpu().numpy()
+            xc, xr = sc.x.numpy(), x0
+            cpu["rel_diff_displacement_vs_gpu"] = float(
+                np.linalg.norm(xg - xc) / max(np.linalg.norm(xc - xr), 1e-300))
+        out["cpu"] = cpu
+        log(f"implicit cpu: {cpu}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +299,9 @@ def main():
     ap.add_argument("--serve-cases", type=int, default=400,
                     help="cases of the served sweep (2000 = the full sweep, ~100 s of arrivals)")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
+    ap.add_argument("--implicit-steps", type=int, default=20,
+                    help="implicit Newmark steps of the 32x24 membrane (0: skip)")
+    ap.add_argument("--implicit-cpu-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -300,6 +394,7 @@ def main():
         log(f"calibration copy: 3 x {args.copy_mib} MiB")
     batched = run_batched(args, L, log) if args.batch else None
     served = run_served(args, L, log) if args.serve_cases > 0 else None
+    implicit = run_implicit(args, L, log) if args.implicit_steps > 0 else None
     if "fp64" not in res:
         print(json.dumps({k: v for k, v in res.items()}), flush=True)
         fe.close()
@@ -378,6 +473,7 @@ def main():
         "parity": parity,
         "batched": batched,
         "served_sweep": served,
+        "implicit_steps": implicit,
     }
     fe.close()
     print(json.dumps(out), flush=True)

@@ -52,12 +52,12 @@ def gmres(matvec, b: torch.Tensor, rtol: float = 1e-5, atol: float = 1e-50,
         k_done = 0
         for k in range(m):
             w = matvec(V[k])
-            # classical Gram-Schmidt in two matrix products
+            # classical Gram-Schmidt in two matrix products; one transfer to the host
             h = V[: k + 1] @ w
             w = w - h @ V[: k + 1]
-            hn = float(torch.linalg.vector_norm(w))
-            H[: k + 1, k] = h.double().cpu().numpy()
-            H[k + 1, k] = hn
+            hv = torch.cat([h, torch.linalg.vector_norm(w).view(1)]).double().cpu().numpy()
+            hn = float(hv[-1])
+            H[: k + 2, k] = hv
             for i in range(k):  # earlier Givens rotations
                 t = cs[i] * H[i, k] + sn[i] * H[i + 1, k]
                 H[i + 1, k] = -sn[i] * H[i, k] + cs[i] * H[i + 1, k]

@@ -204,8 +204,10 @@ __device__ __forceinline__ void gather_node(const int* __restrict__ ptr,
     ay = fint[3 * (size_t)n + 1];
     az = fint[3 * (size_t)n + 2];
   }
-  // entries are read kGatherChunk at a time (their loads overlap), then added in order
-  constexpr int kGatherChunk = 8;
+  // entries are read kGatherChunk at a time (their loads overlap), then added in order; 16
+  // holds every entry of a regular interior node (12) and of valence-7 nodes in one round
+  // trip (measured: the 64-membrane batch gather 222 -> 124 us against 8)
+  constexpr int kGatherChunk = 16;
   const int b = ptr[n], en = ptr[n + 1];
   for (int j0 = b; j0 < en; j0 += kGatherChunk) {
     double vx[kGatherChunk], vy[kGatherChunk], vz[kGatherChunk];

@@ -184,7 +184,8 @@ def run_served(args, L, log):
     out = rep.summary()
     out.update(workload=f"{len(specs)} cases, Poisson 20/s, flat or ellipsoid 2k-400k triangles "
                         "(log-uniform, snapped to the cached mesh sizes), 50-500 explicit steps, "
-                        "FP32-mixed, continuous batching, 10 steps per round",
+                        "FP32-mixed, continuous batching, 10 steps per round; mesh cache and "
+                        "kernels warmed before the first arrival (warmup_s)",
                hbm_budget_gib=args.serve_budget_gib)
     log(f"served sweep: {out}")
     return out

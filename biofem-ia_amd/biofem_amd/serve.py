@@ -157,6 +157,7 @@ class SweepReport:
     gpu_ms: float = 0.0
     max_active: int = 0
     peak_footprint: int = 0
+    warmup: float = 0.0    # seconds of Server.warm before the first arrival
 
     def summary(self) -> dict:
         ok = [c for c in self.cases if c.failed is None]
@@ -174,7 +175,7 @@ class SweepReport:
             queue_wait_p50_s=float(np.percentile(wait, 50)) if len(wait) else None,
             queue_wait_p99_s=float(np.percentile(wait, 99)) if len(wait) else None,
             rounds=self.rounds, gpu_busy_s=self.gpu_ms * 1e-3, max_active=self.max_active,
-            peak_footprint_gib=self.peak_footprint / 2**30)
+            peak_footprint_gib=self.peak_footprint / 2**30, warmup_s=self.warmup)
 
 
 class Server:
@@ -182,7 +183,7 @@ class Server:
 
     def __init__(self, hbm_budget: int, max_active: int = 256, steps_per_round: int = 10,
                  precision: int = L.BF_PREC_MIXED, graph: bool = True, device: int = 0,
-                 keep_results: bool = False, log=None):
+                 keep_results: bool = False, log=None, warm_up: bool = True):
         self.budget = hbm_budget
         self.max_active = max_active
         self.k = steps_per_round
@@ -192,12 +193,41 @@ class Server:
         self.keep = keep_results
         self.cache = MeshCache()
         self.log = log or (lambda s: None)
+        self.warm_up = warm_up
+
+    def warm(self, specs) -> float:
+        """Before the first arrival, as a server does before it takes traffic: build the
+        reference mesh of every resolution the specs ask for, and step a short copy of the
+        smallest case through the batch path (kernel code loaded, graph captured).  Returns
+        the seconds spent.  Per-case work (fibres, context, upload) stays in each request's
+        latency."""
+        t = time.perf_counter()
+        for s in specs:
+            self.cache.get(s)
+        if specs:
+            small = min(specs, key=lambda s: self.cache.get(s).n_elmt)
+            case = build_case(dataclasses.replace(small, steps=2), self.cache, self.device)
+            batch = Batch(device=self.device)
+            try:
+                batch.set_cases([case.fe])
+                for _ in range(2):
+                    L.check("bf_batch_step", batch.lib.bf_batch_step(batch.h, self.precision, 1,
+                                                                     int(self.graph)))
+                batch.sync()
+                batch.case_status()
+            finally:
+                batch.close()
+                case.fe.close()
+        return time.perf_counter() - t
 
     def run(self, specs, time_scale: float = 1.0) -> SweepReport:
         """Serve `specs` in arrival order; arrivals are replayed in wall-clock time divided
         by time_scale."""
         pending = sorted(specs, key=lambda s: s.arrival)
         rep = SweepReport()
+        if self.warm_up:
+            rep.warmup = self.warm(pending)
+            self.log(f"warm-up {rep.warmup:.2f} s: {len(self.cache.cache)} mesh resolutions")
         batch = Batch(device=self.device)
         active: list = []
         used = 0

@@ -112,11 +112,12 @@ def jfnk(residual, U0: torch.Tensor, rtol: float = 1e-8, atol: float = 1e-50,
     for it in range(1, max_it + 1):
         unorm = float(torch.linalg.vector_norm(U))
 
-        def jv(a, U=U, F=F, unorm=unorm):
-            an = float(torch.linalg.vector_norm(a))
-            if an == 0.0:
-                return torch.zeros_like(a)
-            h = mffd_err * math.sqrt(1.0 + unorm) / an
+        c = mffd_err * math.sqrt(1.0 + unorm)
+
+        def jv(a, U=U, F=F, c=c):
+            # h stays on the device (no host round trip per product); a = 0 gives
+            # R(U) - F = 0 and a zero product, as a skipped one would
+            h = c / torch.linalg.vector_norm(a).clamp_min(1e-300)
             return (residual(U + h * a) - F) / h
 
         d, kits, _, _ = gmres(jv, -F, rtol=ksp_rtol, restart=ksp_restart, max_it=ksp_max_it)
@@ -180,6 +181,12 @@ class NewmarkJFNK:
         self.ghost = torch.zeros(n3, dtype=torch.bool, device=x0.device)
         self.ghost[3 * self.n_v:] = True
         self.fext = torch.zeros_like(x0) if fext is None else fext.to(x0)
+        # free rows: R = Fint + a U - b with a = M M1 + C C1 and, per step,
+        # b = Fext + M (M1 xn + M2 xd + M3 xdd) + C (C1 xn + C2 xd + C3 xdd); ghost rows of a
+        # and b are zero, so R's ghost rows are Fint's (zero); fixed rows: U - x0
+        self.a = torch.where(self.ghost, 0.0, self.M * self.M1 + self.C * self.C1)
+        self.fixed_rows = self.fixed & ~self.ghost
+        self.b = torch.zeros_like(x0)
         self.x = x0.clone()
         self.xn = x0.clone()
         self.xnm1 = x0.clone()
@@ -195,11 +202,14 @@ class NewmarkJFNK:
         return self.M * self.M1 * U + self.C * self.C1 * U - facc - fvel
 
     def residual(self, U: torch.Tensor) -> torch.Tensor:
-        R = self.fint(U) - self.fext + self.fdyn(U)
-        R = torch.where(self.fixed, U - self.x0, R)
-        return torch.where(self.ghost, torch.zeros_like(R), R)
+        """FormFunctionFEM's residual at U (the b of the current step)."""
+        R = torch.addcmul(self.fint(U), self.a, U).sub_(self.b)
+        return torch.where(self.fixed_rows, U - self.x0, R)
 
     def step(self) -> SolveInfo:
+        facc = self.M * (self.M1 * self.xn + self.M2 * self.xd + self.M3 * self.xdd)
+        fvel = self.C * (self.C1 * self.xn + self.C2 * self.xd + self.C3 * self.xdd)
+        self.b = torch.where(self.ghost, 0.0, self.fext + facc + fvel)
         U, info = jfnk(self.residual, self.x, **self.solver)
         if info.converged:
             self.x = U
@@ -230,7 +240,8 @@ def for_fe(fe, precision: int = L.BF_PREC_FP64_FAST, device: str = "cuda"):
     reports as a diverged step."""
     m = fe.mesh
     n = m.n_nodes
-    bx, by, bz = (torch.zeros(n, dtype=torch.float64, device=device) for _ in range(3))
+    bxyz = torch.zeros((3, n), dtype=torch.float64, device=device)  # SoA rows x, y, z
+    bx, by, bz = bxyz[0], bxyz[1], bxyz[2]
     f = torch.zeros(3 * n, dtype=torch.float64, device=device)
     lib = fe.lib
     vp = C.c_void_p
@@ -242,10 +253,7 @@ def for_fe(fe, precision: int = L.BF_PREC_FP64_FAST, device: str = "cuda"):
 
     def internal_force(U: torch.Tensor) -> torch.Tensor:
         cur = torch.cuda.current_stream(device)
-        V = U.view(-1, 3)
-        bx.copy_(V[:, 0])
-        by.copy_(V[:, 1])
-        bz.copy_(V[:, 2])
+        bxyz.copy_(U.view(-1, 3).t())
         side.wait_stream(cur)
         s = vp(side.cuda_stream)
         L.check("bf_ghost_update_device", lib.bf_ghost_update_device(
@@ -254,8 +262,6 @@ def for_fe(fe, precision: int = L.BF_PREC_FP64_FAST, device: str = "cuda"):
                                       vp(bz.data_ptr()), vp(f.data_ptr()), 0, precision, s)
         L.check("bf_force_eval_device", rc)
         cur.wait_stream(side)
-        out = f.clone()
-        out[3 * m.n_v:] = 0.0
-        return out
+        return f.clone()      # accumulate = 0: ghost rows are written as zeros
 
     return internal_force

@@ -229,8 +229,74 @@ __global__ void __launch_bounds__(kStagedBlock) bf_tier2_kernel(DevMesh m, KPara
 
 // Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
 // m.fallback and stores the total in m.status[4].  One block: a chunk of waves per thread,
-// a block-wide scan of the chunk sums, then each thread copies its chunk.
+// a block-wide scan of the chunk sums, then each thread copies its chunk.  The counts and
+// segments were just written by waves on every XCD, so each load is a round trip to
+// memory: a thread reads its counts kCompactRegs at a time with the loads in flight
+// together, and loads the first two entries of every segment of a group before it stores
+// any (most segments hold none or one).
 constexpr int kCompactBlock = 1024;
+constexpr int kCompactRegs = 8;
+
+// Block-wide exclusive offset of this thread's `sum` (inclusive Hillis-Steele scan in s_sum);
+// the block's total is left in s_sum[kCompactBlock - 1].
+__device__ __forceinline__ int compact_offset(int* s_sum, int sum) {
+  const int t = threadIdx.x;
+  s_sum[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < kCompactBlock; d <<= 1) {
+    const int v = (t >= d) ? s_sum[t - d] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  return s_sum[t] - sum;
+}
+
+// Counts of segments g0 .. g0 + kCompactRegs - 1 (zero at and past w1).
+__device__ __forceinline__ void compact_counts(const int* __restrict__ count, int g0, int w1,
+                                               int (&cnt)[kCompactRegs]) {
+#pragma unroll
+  for (int r = 0; r < kCompactRegs; r++) cnt[r] = (g0 + r < w1) ? count[g0 + r] : 0;
+}
+
+// Thread's segments [w0, w1) of `list` (per_wave entries apart) to out[pos ..): entry k of
+// segment w as emit(position, w, value).
+template <class Emit>
+__device__ __forceinline__ void compact_copy(const int* __restrict__ count,
+                                             const int* __restrict__ list, int per_wave, int w0,
+                                             int w1, int pos, Emit&& emit) {
+  for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
+    int cnt[kCompactRegs], v0[kCompactRegs], v1[kCompactRegs];
+    compact_counts(count, g0, w1, cnt);
+#pragma unroll
+    for (int r = 0; r < kCompactRegs; r++) {
+      const size_t base = (size_t)(g0 + r) * per_wave;
+      v0[r] = cnt[r] > 0 ? list[base] : 0;
+      v1[r] = cnt[r] > 1 ? list[base + 1] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kCompactRegs; r++) {
+      const int w = g0 + r;
+      if (cnt[r] > 0) emit(pos, w, v0[r]);
+      if (cnt[r] > 1) emit(pos + 1, w, v1[r]);
+      for (int k = 2; k < cnt[r]; k++) emit(pos + k, w, list[(size_t)w * per_wave + k]);
+      pos += cnt[r];
+    }
+  }
+}
+
+// Sum of the thread's counts [w0, w1).
+__device__ __forceinline__ int compact_sum(const int* __restrict__ count, int w0, int w1) {
+  int sum = 0;
+  for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
+    int cnt[kCompactRegs];
+    compact_counts(count, g0, w1, cnt);
+#pragma unroll
+    for (int r = 0; r < kCompactRegs; r++) sum += cnt[r];
+  }
+  return sum;
+}
+
 __global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(const int* __restrict__ wcount,
                                                                    const int* __restrict__ wlist,
                                                                    int n_waves, int per_wave,
@@ -239,24 +305,11 @@ __global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(const int* __
   __shared__ int s_sum[kCompactBlock];
   const int t = threadIdx.x;
   const int chunk = (n_waves + kCompactBlock - 1) / kCompactBlock;
-  const int w0 = t * chunk, w1 = min(n_waves, w0 + chunk);
-  int sum = 0;
-  for (int w = w0; w < w1; w++) sum += wcount[w];
-  s_sum[t] = sum;
-  __syncthreads();
-  // inclusive Hillis-Steele scan over the block
-  for (int d = 1; d < kCompactBlock; d <<= 1) {
-    const int v = (t >= d) ? s_sum[t - d] : 0;
-    __syncthreads();
-    s_sum[t] += v;
-    __syncthreads();
-  }
-  int pos = s_sum[t] - sum;
-  for (int w = w0; w < w1; w++) {
-    const int c = wcount[w];
-    for (int k = 0; k < c; k++) out[pos + k] = wlist[(size_t)w * per_wave + k];
-    pos += c;
-  }
+  const int w0 = min(n_waves, t * chunk), w1 = min(n_waves, w0 + chunk);
+  const int sum = compact_sum(wcount, w0, w1);
+  const int pos = compact_offset(s_sum, sum);
+  if (sum > 0)  // most chunks hold no hand-over
+    compact_copy(wcount, wlist, per_wave, w0, w1, pos, [&](int at, int, int v) { out[at] = v; });
   if (t == kCompactBlock - 1) status[4] = s_sum[t];
 }
 
@@ -367,24 +420,12 @@ __global__ void __launch_bounds__(kCompactBlock) bf_batch_compact_kernel(BatchDe
   __shared__ int s_sum[kCompactBlock];
   const int t = threadIdx.x;
   const int chunk = (b.n_etiles + kCompactBlock - 1) / kCompactBlock;
-  const int w0 = t * chunk, w1 = min(b.n_etiles, w0 + chunk);
-  int sum = 0;
-  for (int w = w0; w < w1; w++) sum += b.tcount[w];
-  s_sum[t] = sum;
-  __syncthreads();
-  for (int d = 1; d < kCompactBlock; d <<= 1) {
-    const int v = (t >= d) ? s_sum[t - d] : 0;
-    __syncthreads();
-    s_sum[t] += v;
-    __syncthreads();
-  }
-  int pos = s_sum[t] - sum;
-  for (int w = w0; w < w1; w++) {
-    const int cnt = b.tcount[w];
-    const int c = b.etiles[w].c;
-    for (int k = 0; k < cnt; k++) b.hlist[pos + k] = make_int2(c, b.tlist[(size_t)w * b.tile_cap + k]);
-    pos += cnt;
-  }
+  const int w0 = min(b.n_etiles, t * chunk), w1 = min(b.n_etiles, w0 + chunk);
+  const int sum = compact_sum(b.tcount, w0, w1);
+  const int pos = compact_offset(s_sum, sum);
+  if (sum > 0)  // most chunks hold no hand-over
+    compact_copy(b.tcount, b.tlist, b.tile_cap, w0, w1, pos,
+                 [&](int at, int w, int v) { b.hlist[at] = make_int2(b.etiles[w].c, v); });
   if (t == kCompactBlock - 1) *b.htotal = s_sum[t];
 }
 

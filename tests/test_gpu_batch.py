@@ -61,6 +61,36 @@ def test_batch_equals_single(prec, mixed_qp):
             fe.close()
 
 
+@pytest.mark.parametrize("prec", [L.BF_PREC_FP64_FAST, L.BF_PREC_MIXED])
+def test_batch_handover_lists_equal_single(prec):
+    """Rough current states (z noise of one mesh spacing h) make every case hand about a
+    tenth of its elements to the reference order: the batch's concatenated hand-over list
+    gives every case's forces bit for bit as its own evaluation."""
+    rng = np.random.default_rng(9)
+    cases = []
+    for m, p, _ in _cases(7):
+        h = 1.0 / np.sqrt(m.n_elmt / 1.5)
+        x, y, z = m.x0.copy(), m.y0.copy(), m.z0.copy()
+        z[: m.n_v] += rng.normal(0.0, h, m.n_v)
+        m.apply_ghosts(x, y, z)
+        cases.append((m, p, (x, y, z)))
+    print(f"case batch of {len(cases)} rough states prec {prec}", flush=True)
+    ref = _single(cases, prec)
+    fes = _resident(cases)
+    b = Batch(fes)
+    try:
+        for graph in (False, True):
+            b.eval(prec, ghosts=False, graph=graph)
+            b.sync()
+            assert b.fallback_count() > 0
+            for i in range(len(fes)):
+                assert np.array_equal(b.forces(i), ref[i]), (graph, i)
+    finally:
+        b.close()
+        for fe in fes:
+            fe.close()
+
+
 def test_batch_plain_matches_oracle():
     cases = _cases(3)
     print("case batch plain vs oracle", flush=True)

@@ -6,6 +6,7 @@
 #include "../../include/biofem_io.h"
 
 #include <errno.h>
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -34,7 +35,8 @@ int bf_io_dimension(const char *in_dir, int ibi, int *n_v, int *n_elmt, int *n_e
                     int *n_ghosts, int *sum_n_bnodes) {
   if (!in_dir) return BF_ERR_ARG;
   char fn[4096];
-  int nv = 0, ne = 0, nedge = 0, ng = 0, sum = 0;
+  int nv = 0, ne = 0, nedge = 0, ng = 0;
+  long long sum = 0;
   list_path(fn, sizeof fn, in_dir, "nlist", ibi);
   FILE *f = fopen(fn, "r");
   if (!f) return BF_ERR_IO;
@@ -55,6 +57,7 @@ int bf_io_dimension(const char *in_dir, int ibi, int *n_v, int *n_elmt, int *n_e
     int c = 0;
     ok = fscanf(f, "%d", &c) == 1 && c >= 0;
     sum += c;
+    ok = ok && sum <= INT_MAX;  /* the list lengths size the caller's arrays */
   }
   fclose(f);
   if (!ok) return BF_ERR_IO;
@@ -62,7 +65,7 @@ int bf_io_dimension(const char *in_dir, int ibi, int *n_v, int *n_elmt, int *n_e
   if (n_elmt) *n_elmt = ne;
   if (n_edge) *n_edge = nedge;
   if (n_ghosts) *n_ghosts = ng;
-  if (sum_n_bnodes) *sum_n_bnodes = sum;
+  if (sum_n_bnodes) *sum_n_bnodes = (int)sum;
   return BF_OK;
 }
 

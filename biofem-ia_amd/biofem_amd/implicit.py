@@ -158,13 +158,19 @@ class NewmarkJFNK:
 
     internal_force(U) -> Fint [3 n_nodes] with ghost rows zero; `for_fe` builds it from an
     ActStrainFE (GlobalGhost + the library's force on device).  mass [n_nodes]: assembled
-    lumped masses (lumped_masses); fixed [3 n_nodes] bool: degrees of freedom held at x0."""
+    lumped masses (lumped_masses); fixed [3 n_nodes] bool: degrees of freedom held at x0.
+    element_thickness() -> [n_elmt]: UpdateElementThickness of the last force evaluation,
+    called once per converged step as the reference does (src/main.c:497-503) and kept in
+    `thickness` (default: internal_force.thickness when the callable has one)."""
 
     def __init__(self, internal_force, x0: torch.Tensor, mass: torch.Tensor, dt: float,
                  n_v: int | None = None, fixed: torch.Tensor | None = None,
                  fext: torch.Tensor | None = None, newmark_gamma: float = 0.5,
-                 dampfactor: float = 0.0, **solver):
+                 dampfactor: float = 0.0, element_thickness=None, **solver):
         self.fint = internal_force
+        self.element_thickness = (element_thickness if element_thickness is not None
+                                  else getattr(internal_force, "thickness", None))
+        self.thickness = None
         self.x0 = x0.clone()
         n3 = x0.numel()
         self.n_v = n3 // 3 if n_v is None else n_v
@@ -213,6 +219,9 @@ class NewmarkJFNK:
         U, info = jfnk(self.residual, self.x, **self.solver)
         if info.converged:
             self.x = U
+            # the last residual was evaluated at U (the accepted line-search point)
+            if self.element_thickness is not None:
+                self.thickness = self.element_thickness()
         # xAccVel (src/main.c:1367-1420), then the shift of src/main.c:563-564
         xdd_old = self.xdd
         self.xdd = self.M1 * (self.x - self.xn) - self.M2 * self.xd - self.M3 * self.xdd
@@ -264,4 +273,9 @@ def for_fe(fe, precision: int = L.BF_PREC_FP64_FAST, device: str = "cuda"):
         cur.wait_stream(side)
         return f.clone()      # accumulate = 0: ghost rows are written as zeros
 
+    def thickness() -> np.ndarray:
+        side.synchronize()    # the library reads its C33 state on its own stream
+        return fe.thickness()
+
+    internal_force.thickness = thickness
     return internal_force

@@ -83,6 +83,12 @@ def test_implicit_steps_solve_the_reference_residual(dampfactor):
             v_new = xd + (1 - gam) * dt * xdd + gam * dt * a_new
             assert np.allclose(st.xdd.cpu().numpy(), a_new, rtol=1e-12, atol=1e-9)
             assert np.allclose(st.xd.cpu().numpy(), v_new, rtol=1e-12, atol=1e-11)
+            # UpdateElementThickness after the converged step (src/active_strain.c:2198-2224)
+            x, y, z = U[0::3].copy(), U[1::3].copy(), U[2::3].copy()
+            m.apply_ghosts(x, y, z)
+            _, _, c33 = O.finternal_act(m, p, x, y, z, want_c33=True)
+            th_ref = p.h0 * np.sqrt(c33.mean(axis=1))
+            assert np.abs(st.thickness - th_ref).max() <= 1e-11 * p.h0
         assert np.abs(U[fixed] - x0[fixed]).max() <= 1e-10
         assert np.abs(U - x0).max() > 1e-6      # the active membrane moved
     finally:

@@ -108,3 +108,33 @@ def test_damping_and_fixed_dofs():
         energies.append(0.5 * v @ (mass * v) + 0.5 * u @ K @ u)
         assert abs(st.x[0].item()) <= 1e-10       # held by its residual row, to the tolerance
     assert all(e2 <= e1 * (1 + 1e-12) for e1, e2 in zip(energies[1:], energies[2:]))
+
+
+def test_thickness_hook_runs_once_per_converged_step():
+    """UpdateElementThickness is called after every converged step, never per Newton
+    iteration (src/main.c:497-503); the hook's value is kept on the stepper."""
+    n, dt = 6, 0.02
+    K, mass = _chain(n)
+    last = {}
+
+    def fint(U):
+        last["x"] = U.numpy()[0::3].copy()
+        out = np.zeros(3 * n)
+        out[0::3] = K @ U.numpy()[0::3]
+        return T(out)
+
+    calls = []
+
+    def thickness():
+        calls.append(last["x"].copy())
+        return np.full(n - 1, float(len(calls)))
+
+    st = NewmarkJFNK(fint, T(np.zeros(3 * n)), T(mass), dt, element_thickness=thickness,
+                     rtol=1e-12)
+    st.x[3::3] = 0.02
+    st.xn = st.x.clone()
+    for k in range(3):
+        assert st.step().converged
+        assert len(calls) == k + 1 and st.thickness[0] == k + 1
+        assert np.array_equal(calls[-1], st.x.numpy()[0::3])   # evaluated at the accepted state
+

@@ -269,13 +269,19 @@ struct CcSplit {
   }
 };
 
-// Split factor for a hand-over list of N elements: the largest K whose waves for the
-// expensive qps (all but the mid-surface point of an odd rule) fit one round of the grid.
+// Split factor for a hand-over list of N elements: the largest K (8, 6, 4, 2; more
+// instantiations cost the kernel's register allocation more than they save) whose items fit
+// one round of the grid, the expensive qps' (all but the mid-surface point of an odd rule)
+// counted whole and the mid-surface point's at 0.4 of one (measured on flat reference
+// shells, where its zero blocks are skipped): the waves left over by the expensive items
+// must absorb the cheap ones, or a wave that takes a cheap item after an expensive one sets
+// the kernel's time.
 __device__ __forceinline__ int split_factor(int N, int P, int n_waves) {
-  const int pe = (P > 1 && (P & 1)) ? P - 1 : P;
-  for (int K = 8; K >= 2; K >>= 1) {
-    const int ng = kSplitLanes / K;
-    if ((long long)((N + ng - 1) / ng) * pe <= n_waves) return K;
+  const bool mid = P > 1 && (P & 1);
+  const int pe = mid ? P - 1 : P;
+  for (int K = 8; K >= 2; K -= 2) {
+    const long long chunks = (N + kSplitLanes / K - 1) / (kSplitLanes / K);
+    if (chunks * (10 * pe + (mid ? 4 : 0)) <= 10LL * n_waves) return K;
   }
   return 1;
 }
@@ -465,6 +471,7 @@ __global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
   const int N = *count;
   switch (split_factor(N, kp.n_qp, n_waves)) {
     case 8: solve_list_k<8>(m, kp, x, y, z, list, N, counter, buf); break;
+    case 6: solve_list_k<6>(m, kp, x, y, z, list, N, counter, buf); break;
     case 4: solve_list_k<4>(m, kp, x, y, z, list, N, counter, buf); break;
     case 2: solve_list_k<2>(m, kp, x, y, z, list, N, counter, buf); break;
     default: solve_list_k<1>(m, kp, x, y, z, list, N, counter, buf); break;
@@ -554,6 +561,7 @@ __global__ void __launch_bounds__(64) bf_batch_exact_solve_list_kernel(BatchDev 
   const int N = *b.htotal;
   switch (split_factor(N, b.P, n_waves)) {
     case 8: batch_solve_list_k<8>(b, N, buf); break;
+    case 6: batch_solve_list_k<6>(b, N, buf); break;
     case 4: batch_solve_list_k<4>(b, N, buf); break;
     case 2: batch_solve_list_k<2>(b, N, buf); break;
     default: batch_solve_list_k<1>(b, N, buf); break;

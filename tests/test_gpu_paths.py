@@ -78,12 +78,12 @@ def test_membrane_large_staged_paths():
     check_all(m, p, *xyz, omp=16)
 
 
-@pytest.mark.parametrize("sigma", [7e-4, 1.2e-3, 1.5e-3])
+@pytest.mark.parametrize("sigma", [7e-4, 1e-3, 1.2e-3, 1.5e-3])
 def test_membrane_handover_sizes(sigma):
-    """FP64 hand-over lists of about 1k, 11k and 19k of the 98 304 elements (current z noise
-    N(0, sigma), seed 11; the mixed path's are shorter): the hand-over solve picks 8, 2 and 1
-    lanes per (element, qp) for them on a 256-CU device; every handed-over element stays bit
-    for bit the oracle's."""
+    """FP64 hand-over lists of about 0.9k, 6.0k, 11k and 19k of the 98 304 elements (current
+    z noise N(0, sigma), seed 11), the mixed path's of 0.3k, 3.4k, 7.1k and 14k: on a 256-CU
+    device the hand-over solve splits each (element, qp) over 8, 4, 2, 1 lanes and 8, 6, 2, 1
+    lanes for them; every handed-over element stays bit for bit the oracle's."""
     m, p, (x, y, z) = configs.membrane_large()
     rng = np.random.default_rng(11)
     z = m.z0.copy()

@@ -117,6 +117,9 @@ SIGNATURES = {
     "bf_batch_last_fallback_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "bf_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bf_batch_case_status": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bf_batch_step_async": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_longlong]),
+    "bf_batch_step_wait": (C.c_int, [_vp, C.c_longlong, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bf_io_dimension": (C.c_int, [C.c_char_p, C.c_int, _ip, _ip, _ip, _ip, _ip]),
     "bf_io_input": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _ip,
                               _ip, _ip, _ip]),

@@ -45,6 +45,22 @@ class Batch:
         L.check("bf_batch_last_ms", self.lib.bf_batch_last_ms(self.h, C.byref(ms)))
         return ms.value
 
+    def step_async(self, precision, steps: int, round_no: int, graph: bool = True) -> None:
+        """Enqueue round `round_no` (bf_batch_step + its per-case status) and return."""
+        L.check("bf_batch_step_async", self.lib.bf_batch_step_async(
+            self.h, int(precision), int(steps), int(graph), int(round_no)))
+
+    def step_wait(self, round_no: int, n_cases: int):
+        """Wait for round `round_no`: (GPU ms, first non-finite element per case, first
+        CC3333 trip per case), -1 for none, in the case order the round ran with."""
+        ms = C.c_float()
+        nf = np.full(max(n_cases, 1), -1, np.int32)
+        cc = np.full(max(n_cases, 1), -1, np.int32)
+        ip = C.POINTER(C.c_int)
+        L.check("bf_batch_step_wait", self.lib.bf_batch_step_wait(
+            self.h, int(round_no), C.byref(ms), nf.ctypes.data_as(ip), cc.ctypes.data_as(ip)))
+        return ms.value, nf[:n_cases], cc[:n_cases]
+
     def case_status(self):
         """Per case of the batch: (first non-finite element, first CC3333 trip) of the last
         step, -1 for none."""

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import dataclasses
 import time
+from collections import deque
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -222,7 +223,10 @@ class Server:
 
     def run(self, specs, time_scale: float = 1.0) -> SweepReport:
         """Serve `specs` in arrival order; arrivals are replayed in wall-clock time divided
-        by time_scale."""
+        by time_scale.  Rounds are enqueued one ahead of the one the host is waiting for
+        (bf_batch_step_async), so the GPU does not idle while the host reads a round's
+        status; the case set changes (an arrival admitted, a case finished or retired) only
+        after the rounds in flight have been drained."""
         pending = sorted(specs, key=lambda s: s.arrival)
         rep = SweepReport()
         if self.warm_up:
@@ -230,37 +234,75 @@ class Server:
             self.log(f"warm-up {rep.warmup:.2f} s: {len(self.cache.cache)} mesh resolutions")
         batch = Batch(device=self.device)
         active: list = []
+        inflight: deque = deque()   # (round, steps, [(case, steps done before the round)])
         used = 0
+        rnd = 0
         t0 = time.perf_counter()
         t_log = 0.0
         i = 0
         changed = False
+
+        def settle(entry):
+            """Wait for a round; a case whose forces went non-finite in it is marked for
+            retirement with the round's step window."""
+            r, k, cases = entry
+            ms, nonfinite, _ = batch.step_wait(r, len(cases))
+            rep.gpu_ms += ms
+            rep.rounds += 1
+            for (c, start), e in zip(cases, nonfinite):
+                if e >= 0 and c.failed is None:
+                    c.failed = (f"non-finite force at element {int(e)} within steps "
+                                f"{start + 1}..{start + k}")
+
+        def fits(spec) -> bool:
+            need = self.cache.get(spec).footprint_bytes(state=True)
+            if need > self.budget:
+                raise RuntimeError(f"case {spec.cid} needs {need} B, budget {self.budget} B")
+            return used + need <= self.budget and len(active) < self.max_active
+
         try:
-            while i < len(pending) or active:
+            while i < len(pending) or active or inflight:
                 now = time.perf_counter() - t0
-                # admission: arrived requests in order, while memory and slots allow
-                while i < len(pending) and pending[i].arrival / time_scale <= now:
-                    spec = pending[i]
-                    need = self.cache.get(spec).footprint_bytes(state=True)
-                    if need > self.budget:
-                        raise RuntimeError(f"case {spec.cid} needs {need} B, budget {self.budget} B")
-                    if used + need > self.budget or len(active) >= self.max_active:
-                        break
-                    case = build_case(spec, self.cache, self.device)
-                    case.admitted = time.perf_counter() - t0
-                    active.append(case)
-                    used += case.footprint
-                    changed = True
-                    i += 1
-                rep.max_active = max(rep.max_active, len(active))
-                rep.peak_footprint = max(rep.peak_footprint, used)
-                if not active:
-                    nxt = pending[i].arrival / time_scale
-                    time.sleep(max(0.0, nxt - (time.perf_counter() - t0)))
-                    continue
-                if changed:
-                    batch.set_cases([c.fe for c in active])
-                    changed = False
+                arrived = i < len(pending) and pending[i].arrival / time_scale <= now
+                if (not active or arrived and fits(pending[i])
+                        or any(c.remaining <= 0 or c.failed for c in active)):
+                    while inflight:
+                        settle(inflight.popleft())
+                    done = [c for c in active if c.remaining <= 0 or c.failed]
+                    if done:
+                        tnow = time.perf_counter() - t0
+                        for c in done:
+                            if self.keep and c.failed is None:
+                                c.result = case_result(c)
+                            if c.failed:
+                                self.log(f"case {c.spec.cid} retired: {c.failed}")
+                            c.finished = tnow
+                            used -= c.footprint
+                            c.fe.close()
+                            rep.cases.append(c)
+                        active = [c for c in active if c.remaining > 0 and not c.failed]
+                        changed = True
+                    # admission: arrived requests in order, while memory and slots allow
+                    now = time.perf_counter() - t0
+                    while (i < len(pending) and pending[i].arrival / time_scale <= now
+                           and fits(pending[i])):
+                        case = build_case(pending[i], self.cache, self.device)
+                        case.admitted = time.perf_counter() - t0
+                        active.append(case)
+                        used += case.footprint
+                        changed = True
+                        i += 1
+                    rep.max_active = max(rep.max_active, len(active))
+                    rep.peak_footprint = max(rep.peak_footprint, used)
+                    if not active:
+                        if i >= len(pending):
+                            break
+                        nxt = pending[i].arrival / time_scale
+                        time.sleep(max(0.0, nxt - (time.perf_counter() - t0)))
+                        continue
+                    if changed:
+                        batch.set_cases([c.fe for c in active])
+                        changed = False
                 if now - t_log >= 5.0:
                     t_log = now
                     waiting = sum(1 for sp in pending[i:] if sp.arrival / time_scale <= now)
@@ -268,39 +310,22 @@ class Server:
                              f"{waiting} waiting, {len(pending) - i - waiting} yet to arrive, "
                              f"{used / 2**30:.2f} GiB resident")
                 k = min(self.k, min(c.remaining for c in active))
-                L.check("bf_batch_step", batch.lib.bf_batch_step(batch.h, self.precision, k,
-                                                                 int(self.graph)))
-                batch.sync()
-                rep.gpu_ms += batch.last_ms()
-                rep.rounds += 1
-                # a case whose state diverged (non-finite forces) is retired with its reason
-                # instead of holding the GPU for the rest of its steps
-                nonfinite, _ = batch.case_status()
-                for c, e in zip(active, nonfinite):
-                    if e >= 0:
-                        c.failed = (f"non-finite force at element {int(e)} within steps "
-                                    f"{c.spec.steps - c.remaining + 1}..{c.spec.steps - c.remaining + k}")
-                done = [c for c in active if c.remaining - k <= 0 or c.failed]
+                batch.step_async(self.precision, k, rnd, self.graph)
+                inflight.append((rnd, k, [(c, c.spec.steps - c.remaining) for c in active]))
                 for c in active:
                     c.remaining -= k
-                if done:
-                    tnow = time.perf_counter() - t0
-                    for c in done:
-                        if self.keep and c.failed is None:
-                            c.result = case_result(c)
-                        if c.failed:
-                            self.log(f"case {c.spec.cid} retired: {c.failed}")
-                        c.finished = tnow
-                        used -= c.footprint
-                        c.fe.close()
-                        rep.cases.append(c)
-                    active = [c for c in active if c.remaining > 0 and not c.failed]
-                    changed = True
+                rnd += 1
+                if len(inflight) >= 2:   # one round queued behind the one that runs
+                    settle(inflight.popleft())
             rep.wall = time.perf_counter() - t0
         finally:
-            batch.close()
-            for c in active:
-                c.fe.close()
+            try:
+                while inflight:
+                    settle(inflight.popleft())
+            finally:
+                batch.close()
+                for c in active:
+                    c.fe.close()
         return rep
 
 

@@ -1297,6 +1297,14 @@ struct bf_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int g_prec = -1, g_ghosts = -1, g_step = -1;
+  // served rounds in flight (bf_batch_step_async): slot r & 1 holds round r's per-case
+  // status words on the device and in pinned host memory, and its start / end events
+  int* d_rstat[2] = {nullptr, nullptr};
+  int* h_rstat[2] = {nullptr, nullptr};
+  int rstat_cap = 0;
+  hipEvent_t rev0[2] = {nullptr, nullptr}, rev1[2] = {nullptr, nullptr};
+  long long r_round[2] = {-1, -1};
+  int r_cases[2] = {0, 0};
 };
 
 namespace {
@@ -1477,6 +1485,12 @@ int bf_batch_destroy(bf_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_drop_graph(b);
   batch_free_lists(b);
+  for (int k = 0; k < 2; k++) {
+    dfree(b->d_rstat[k]);
+    if (b->h_rstat[k]) (void)hipHostFree(b->h_rstat[k]);
+    if (b->rev0[k]) (void)hipEventDestroy(b->rev0[k]);
+    if (b->rev1[k]) (void)hipEventDestroy(b->rev1[k]);
+  }
   if (b->h_total) (void)hipHostFree(b->h_total);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1492,6 +1506,7 @@ int bf_batch_set_cases(bf_batch* b, bf_ctx* const* ctxs, int n) {
     if (!c || c->device != b->device) return BF_ERR_ARG;
     if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
   }
+  if (b->r_round[0] >= 0 || b->r_round[1] >= 0) return BF_ERR_STATE;  // rounds in flight
   BF_HIP(hipSetDevice(b->device));
   BF_HIP(hipStreamSynchronize(b->stream));
   b->cases.assign(ctxs, ctxs + n);
@@ -1613,6 +1628,67 @@ int bf_batch_case_status(bf_batch* b, int* first_nonfinite, int* first_cc3333) {
     if (first_nonfinite) first_nonfinite[c] = h[2 * c] == INT_MAX ? -1 : h[2 * c];
     if (first_cc3333) first_cc3333[c] = h[2 * c + 1] == INT_MAX ? -1 : h[2 * c + 1];
   }
+  return BF_OK;
+}
+
+int bf_batch_step_async(bf_batch* b, int precision, int n_steps, int use_graph, long long round) {
+  if (!b || round < 0 || n_steps < 0) return BF_ERR_ARG;
+  const int slot = (int)(round & 1);
+  if (b->r_round[slot] >= 0) return BF_ERR_STATE;  // round - 2 not waited for
+  const int n = (int)b->cases.size();
+  BF_HIP(hipSetDevice(b->device));
+  if (n > b->rstat_cap) {  // grow both slots; only with no round in flight
+    if (b->r_round[slot ^ 1] >= 0) return BF_ERR_STATE;
+    for (int k = 0; k < 2; k++) {
+      dfree(b->d_rstat[k]);
+      if (b->h_rstat[k]) (void)hipHostFree(b->h_rstat[k]);
+      b->h_rstat[k] = nullptr;
+    }
+    b->rstat_cap = 0;
+    for (int k = 0; k < 2; k++) {
+      int rc = dalloc(&b->d_rstat[k], (size_t)2 * n);
+      if (rc != BF_OK) return rc;
+      BF_HIP(hipHostMalloc((void**)&b->h_rstat[k], sizeof(int) * 2 * n, hipHostMallocDefault));
+    }
+    b->rstat_cap = n;
+  }
+  for (int k = 0; k < 2; k++) {
+    if (!b->rev0[k]) BF_HIP(hipEventCreate(&b->rev0[k]));
+    if (!b->rev1[k]) BF_HIP(hipEventCreate(&b->rev1[k]));
+  }
+  BF_HIP(hipEventRecord(b->rev0[slot], b->stream));
+  int rc = batch_run(b, precision, 1, true, n_steps, use_graph);
+  if (rc != BF_OK) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(bf_batch_case_status_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                       b->stream, (const CaseDev*)b->d_cases, n, b->d_rstat[slot]);
+    BF_HIP(hipGetLastError());
+    BF_HIP(hipMemcpyAsync(b->h_rstat[slot], b->d_rstat[slot], sizeof(int) * 2 * n,
+                          hipMemcpyDeviceToHost, b->stream));
+  }
+  BF_HIP(hipEventRecord(b->rev1[slot], b->stream));
+  b->r_round[slot] = round;
+  b->r_cases[slot] = n;
+  return BF_OK;
+}
+
+int bf_batch_step_wait(bf_batch* b, long long round, float* ms, int* first_nonfinite,
+                       int* first_cc3333) {
+  if (!b || round < 0) return BF_ERR_ARG;
+  const int slot = (int)(round & 1);
+  if (b->r_round[slot] != round) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(b->device));
+  BF_HIP(hipEventSynchronize(b->rev1[slot]));
+  float t = 0.f;
+  BF_HIP(hipEventElapsedTime(&t, b->rev0[slot], b->rev1[slot]));
+  if (ms) *ms = t;
+  const int n = b->r_cases[slot];
+  const int* h = b->h_rstat[slot];
+  for (int c = 0; c < n; c++) {
+    if (first_nonfinite) first_nonfinite[c] = h[2 * c] == INT_MAX ? -1 : h[2 * c];
+    if (first_cc3333) first_cc3333[c] = h[2 * c + 1] == INT_MAX ? -1 : h[2 * c + 1];
+  }
+  b->r_round[slot] = -1;
   return BF_OK;
 }
 

@@ -50,8 +50,11 @@ extern "C" {
 #define BF_PREC_FP64_FAST 2 /* FP64 on the staged contractions (closed-form CC^{2222}, 3x3
                                products instead of the 6- and 8-deep index loops) */
 /* Both staged modes hand every element whose local C33 iteration leaves its
- * well-conditioned range (bisection clamp, or C33 outside [1/2, 2] at a qp) to the plain
- * kernel, which recomputes it in the reference's operation order (bf_set_fallback). */
+ * well-conditioned range (ModElemC33's bisection clamp, the CC3333 guard, a non-finite
+ * value, C33 outside [0.1, 10] -- [0.4, 2.5] for the FP32 chain -- at a qp, or an update
+ * whose rounding bound exceeds the path's budget) to the plain kernel, which recomputes it
+ * in the reference's operation order (bf_set_fallback); the FP32-mixed mode first retries
+ * such elements with the FP64 staged chain at the mixed budget. */
 
 #define BF_PATCH_SENTINEL 1000000 /* empty patch slot (src/io.c:306) */
 #define BF_MAX_NEN 18             /* widest patch (valence 12) */
@@ -248,6 +251,18 @@ int bf_batch_last_fallback_count(bf_batch *batch, int *n);
  * step, -1 for none -- what bf_force_eval reports per context as BF_ERR_ELEMENT and the
  * warning count.  Lets a request loop retire a case whose state diverged.  Synchronises. */
 int bf_batch_case_status(bf_batch *batch, int *first_nonfinite, int *first_cc3333);
+
+/* A served round without a host round trip in between: bf_batch_step_async enqueues
+ * bf_batch_step (ghosts on) followed by the per-case status of its last step into round
+ * slot (round & 1) and returns; bf_batch_step_wait waits for that round and returns its GPU
+ * time and per-case status as bf_batch_case_status does, in the case order the round ran
+ * with.  At most two rounds in flight (round r's slot must be waited for before round
+ * r + 2 is enqueued); bf_batch_set_cases refuses (BF_ERR_STATE) while a round is in flight.
+ * No reference counterpart (the serving loop replaces `biofem sweep`). */
+int bf_batch_step_async(bf_batch *batch, int precision, int n_steps, int use_graph,
+                        long long round);
+int bf_batch_step_wait(bf_batch *batch, long long round, float *ms, int *first_nonfinite,
+                       int *first_cc3333);
 int bf_batch_info(bf_batch *batch, int *n_cases, long long *n_elmt, int *n_tiles, int *tile_cap);
 /* Launch shape of the element phase: `passes` element groups per work item (a tile holds
  * passes * floor(64 / n_qp) elements), threads per block (64, 128 or 256), blocks

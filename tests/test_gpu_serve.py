@@ -32,3 +32,41 @@ def test_sweep_overlap_admission_and_identity():
         n = by_id[spec.cid].mesh.n_v
         assert np.isfinite(mixed).all()
         assert np.array_equal(alone[:n], mixed[:n]), spec.cid
+
+
+def test_async_rounds_contract_and_identity():
+    """bf_batch_step_async / bf_batch_step_wait: two rounds in flight at most, the case set
+    frozen while one is, and rounds enqueued ahead give the synchronous steps' state bit for
+    bit."""
+    from biofem_amd.batch import Batch
+    from biofem_amd.serve import build_case, case_result
+    specs = draw_sweep(n_cases=2, rate=100.0, seed=3, elmt_lo=2000, elmt_hi=6000,
+                       steps_lo=20, steps_hi=20)
+    cache = MeshCache()
+    print("case async rounds of", [s.cid for s in specs], flush=True)
+    cases = [build_case(s, cache) for s in specs]
+    b = Batch([c.fe for c in cases])
+    try:
+        b.step_async(L.BF_PREC_MIXED, 4, 0)
+        b.step_async(L.BF_PREC_MIXED, 4, 1)
+        with pytest.raises(L.BfError):        # round 0 not waited for
+            b.step_async(L.BF_PREC_MIXED, 4, 2)
+        with pytest.raises(L.BfError):        # rounds in flight
+            b.set_cases([c.fe for c in cases])
+        for r in (0, 1):
+            ms, nonfinite, _ = b.step_wait(r, len(cases))
+            assert ms > 0.0 and (nonfinite == -1).all()
+        with pytest.raises(L.BfError):        # already waited for
+            b.step_wait(1, len(cases))
+        for r in range(2, 5):                 # 20 steps in all, one round kept ahead
+            b.step_async(L.BF_PREC_MIXED, 4, r)
+            b.step_wait(r - 1, len(cases)) if r > 2 else None
+        b.step_wait(4, len(cases))
+        got = [case_result(c) for c in cases]
+    finally:
+        b.close()
+        for c in cases:
+            c.fe.close()
+    for spec, g in zip(specs, got):
+        alone = run_alone(spec, steps_per_round=4)
+        assert np.array_equal(alone, g), spec.cid

@@ -1281,6 +1281,7 @@ struct bf_batch {
   int P = 1;
   // launch shape of the element phase
   int passes = 4;       // element groups per work item: tile_cap = passes * floor(64 / P)
+  int launch_set = 0;   // 0: passes follows the case set (batch_build), 1: set or tuned
   int block = 256;      // threads per block of the staged kernel
   int blocks = 0;       // 0: persistent default (waves for every SIMD at the kernel's occupancy)
   int persistent = 1;
@@ -1332,8 +1333,19 @@ int batch_build(bf_batch* b) {
   const int nc = (int)b->cases.size();
   if (nc == 0) return BF_OK;
   int P = 1;
-  for (bf_ctx* c : b->cases) P = std::max(P, c->kp.n_qp);
+  long long E_all = 0;
+  for (bf_ctx* c : b->cases) {
+    P = std::max(P, c->kp.n_qp);
+    E_all += c->E;
+  }
   b->P = P;
+  if (!b->launch_set) {
+    // untuned: as many element passes per work item (1 .. 4) as keep every wave slot of
+    // the staged kernels (two per SIMD) busy -- a small case set gets one pass per wave
+    // instead of four passes on a quarter of the waves
+    const long long slots = (long long)b->n_cu * 8 * (64 / P);
+    b->passes = (int)std::max(1LL, std::min(4LL, E_all / std::max(1LL, slots)));
+  }
   b->tile_cap = b->passes * (64 / P);
   std::vector<CaseDev> cases((size_t)nc);
   std::vector<Tile> et, nt, gt, xt;
@@ -1529,6 +1541,7 @@ int bf_batch_set_launch(bf_batch* b, int passes, int block, int blocks, int pers
   b->block = block;
   b->blocks = blocks;
   b->persistent = persistent ? 1 : 0;
+  b->launch_set = 1;
   return batch_build(b);
 }
 
@@ -1719,6 +1732,7 @@ int bf_batch_tune(bf_batch* b, int precision, int reps, int* passes, int* block,
     }
   float best = 1e30f;
   Cand bc{b->passes, b->block, b->blocks, b->persistent};
+  b->launch_set = 1;
   for (const Cand& c : cands) {
     b->passes = c.passes;
     b->block = c.block;

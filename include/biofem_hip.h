@@ -267,7 +267,8 @@ int bf_batch_info(bf_batch *batch, int *n_cases, long long *n_elmt, int *n_tiles
 /* Launch shape of the element phase: `passes` element groups per work item (a tile holds
  * passes * floor(64 / n_qp) elements), threads per block (64, 128 or 256), blocks
  * (0 = one wave per SIMD slot of the kernel's occupancy), persistent (1: waves stride over
- * the work list, 0: one work item per wave). */
+ * the work list, 0: one work item per wave).  Until it is set or tuned, passes follows the
+ * case set: 1 to 4, as many as keep every wave slot of the element phase busy. */
 int bf_batch_set_launch(bf_batch *batch, int passes, int block, int blocks, int persistent);
 int bf_batch_get_launch(bf_batch *batch, int *passes, int *block, int *blocks, int *persistent);
 /* Autotuner: times every launch shape of a fixed candidate set on the batch's own cases

@@ -191,6 +191,51 @@ def run_served(args, L, log):
     return out
 
 
+def run_explicit(args, L, log):
+    """The 32 x 24 membrane's 100 explicit damped steps on device-resident state (the step
+    the served sweep runs; x = 0 edge clamped, lumped masses, damping 0.995, the stable dt):
+    device ms per step for each path.  With --cpu the oracle-driven host restatement of the
+    same steps is timed over its first 10 steps."""
+    import ctypes as C_
+    from biofem_amd import configs
+    from biofem_amd.batch import upload_coords
+    from biofem_amd.fem import ActStrainFE
+    from biofem_amd.mesh import lumped_mass
+    from biofem_amd.serve import stable_dt
+    m, p, _ = configs.membrane_small()
+    mass = lumped_mass(m, 1.0, p.h0)
+    fixed = np.zeros(m.n_nodes, np.uint8)
+    fixed[: m.n_v][np.isclose(m.x0[: m.n_v], 0.0)] = 1
+    dt, damp, n = stable_dt(m, p), 0.995, 100
+    out = dict(workload="flat active-strain membrane 32x24 (1536 triangles), 100 explicit damped "
+                        "steps on device-resident state, x = 0 edge clamped", steps=n)
+    for name, prec in (("fp64", L.BF_PREC_FP64_FAST), ("fp64_ref_order", L.BF_PREC_FP64),
+                       ("mixed", L.BF_PREC_MIXED)):
+        fe = ActStrainFE(m, p)
+        try:
+            upload_coords(fe, m.x0, m.y0, m.z0)
+            ub = C_.POINTER(C_.c_ubyte)
+            L.check("bf_state_init", fe.lib.bf_state_init(fe.ctx, L.dp(mass),
+                                                          fixed.ctypes.data_as(ub), None, dt, damp))
+            L.check("bf_step", fe.lib.bf_step(fe.ctx, prec, 10))      # warm-up
+            L.check("bf_sync", fe.lib.bf_sync(fe.ctx))
+            t0 = time.perf_counter()
+            L.check("bf_step", fe.lib.bf_step(fe.ctx, prec, n))
+            L.check("bf_sync", fe.lib.bf_sync(fe.ctx))
+            out[name] = {"ms_per_step": 1e3 * (time.perf_counter() - t0) / n}
+        finally:
+            fe.close()
+    if args.cpu:
+        from oracle import bf_oracle as O
+        k = 10
+        t0 = time.perf_counter()
+        O.explicit_steps(m, p, m.x0, m.y0, m.z0, mass, fixed, dt, damp, k, nthreads=1)
+        out["cpu"] = {"ms_per_step": 1e3 * (time.perf_counter() - t0) / k, "cores": 1,
+                      "kind": "port", "sample": f"the first {k} of the same steps, oracle force"}
+    log(f"explicit steps: {out}")
+    return out
+
+
 def run_implicit(args, L, log):
     """Implicit Newmark steps of the 32 x 24 membrane (config of the 100-step case: x = 0
     edge clamped, rho 1, dt 1e-2, gamma 1/2): each step a matrix-free Newton-Krylov solve
@@ -303,6 +348,8 @@ def main():
     ap.add_argument("--implicit-steps", type=int, default=20,
                     help="implicit Newmark steps of the 32x24 membrane (0: skip)")
     ap.add_argument("--implicit-cpu-steps", type=int, default=2)
+    ap.add_argument("--explicit-steps", type=int, default=1,
+                    help="100 explicit steps of the 32x24 membrane (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -396,6 +443,7 @@ def main():
     batched = run_batched(args, L, log) if args.batch else None
     served = run_served(args, L, log) if args.serve_cases > 0 else None
     implicit = run_implicit(args, L, log) if args.implicit_steps > 0 else None
+    explicit = run_explicit(args, L, log) if args.explicit_steps else None
     if "fp64" not in res:
         print(json.dumps({k: v for k, v in res.items()}), flush=True)
         fe.close()
@@ -475,6 +523,7 @@ def main():
         "batched": batched,
         "served_sweep": served,
         "implicit_steps": implicit,
+        "explicit_steps": explicit,
     }
     fe.close()
     print(json.dumps(out), flush=True)

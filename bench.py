@@ -191,6 +191,45 @@ def run_served(args, L, log):
     return out
 
 
+def run_ellipsoid(args, L, log):
+    """The ellipsoid shell (184 320 triangles, valence-5/7 irregular patches, helix fibres,
+    Gaussian-pulse activation; config of the curved workload): one force evaluation per
+    step on device-resident coordinates in each path, timed as the headline, with the error
+    of each against the oracle (OpenMP build, bit-identical to the serial one) when --cpu."""
+    from biofem_amd import configs
+    from biofem_amd.fem import ActStrainFE
+    lib = L.lib()
+    m, p, (x, y, z) = configs.ellipsoid_shell()
+    E, q = m.n_elmt, p.n_qp
+    fe = ActStrainFE(m, p)
+    out = dict(workload=f"truncated prolate ellipsoid shell ({E} triangles, {m.n_nodes} nodes incl. "
+                        "ghosts, irregular valence-5/7 patches), helix fibres -60..60 deg, active "
+                        "strain 0.12 x Gaussian pulse, n_qp=3, C33 iters=3", elements=E)
+    try:
+        forces = {}
+        for name, prec in (("fp64", L.BF_PREC_FP64_FAST), ("mixed", L.BF_PREC_MIXED),
+                           ("fp64_ref_order", L.BF_PREC_FP64)):
+            forces[name] = fe.finternal_act(x, y, z, precision=prec)
+            handed = fe.fallback_count() if prec != L.BF_PREC_FP64 else 0
+            steps = args.steps if name != "fp64_ref_order" else max(3, args.steps // 10)
+            wall, _, t_elem, t_hand, t_gather = run_gpu(fe, lib, L, prec, steps, args.warmup)
+            out[name] = dict(steps=steps, ms_per_step=1e3 * wall / steps,
+                             elem_qp_per_s=E * q * steps / wall, kernel_ms_element=t_elem,
+                             kernel_ms_handover=t_hand, kernel_ms_gather=t_gather,
+                             handed_to_ref_order=handed)
+        if args.cpu:
+            from oracle import bf_oracle as O
+            nth = args.cpu_threads or min(os.cpu_count() or 1, 64)
+            ref, _, _ = O.finternal_act(m, p, x, y, z, omp_threads=nth)
+            nrm = float(np.linalg.norm(ref))
+            out["parity"] = {k: float(np.linalg.norm(f - ref) / nrm) for k, f in forces.items()}
+            out["parity"]["fp64_ref_order_bitwise_equal"] = bool(np.array_equal(forces["fp64_ref_order"], ref))
+    finally:
+        fe.close()
+    log(f"ellipsoid: {out}")
+    return out
+
+
 def run_explicit(args, L, log):
     """The 32 x 24 membrane's 100 explicit damped steps on device-resident state (the step
     the served sweep runs; x = 0 edge clamped, lumped masses, damping 0.995, the stable dt):
@@ -348,6 +387,8 @@ def main():
     ap.add_argument("--implicit-steps", type=int, default=20,
                     help="implicit Newmark steps of the 32x24 membrane (0: skip)")
     ap.add_argument("--implicit-cpu-steps", type=int, default=2)
+    ap.add_argument("--ellipsoid", type=int, default=1,
+                    help="also time the 184k-triangle ellipsoid shell (0: skip)")
     ap.add_argument("--explicit-steps", type=int, default=1,
                     help="100 explicit steps of the 32x24 membrane (0: skip)")
     args = ap.parse_args()
@@ -444,6 +485,7 @@ def main():
     served = run_served(args, L, log) if args.serve_cases > 0 else None
     implicit = run_implicit(args, L, log) if args.implicit_steps > 0 else None
     explicit = run_explicit(args, L, log) if args.explicit_steps else None
+    ellipsoid = run_ellipsoid(args, L, log) if args.ellipsoid else None
     if "fp64" not in res:
         print(json.dumps({k: v for k, v in res.items()}), flush=True)
         fe.close()
@@ -524,6 +566,7 @@ def main():
         "served_sweep": served,
         "implicit_steps": implicit,
         "explicit_steps": explicit,
+        "ellipsoid": ellipsoid,
     }
     fe.close()
     print(json.dumps(out), flush=True)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "bf_exact.h"
 #include "bf_kernels.h"
@@ -207,9 +208,9 @@ __device__ __forceinline__ void exact_solve(const DevMesh& m, const KParams& kp,
 // its own block's terms as they are formed, then the other K - 1 blocks' terms from LDS,
 // in block order -- the same additions in the same order as cc_sum_serial, so the sum is
 // bit for bit the serial one.  Everything else of the iteration is formed by all K lanes
-// alike.  A wave holds kSplitLanes / K groups; its term buffer is [81][groups][K] doubles,
-// kSplitLanes x 648 bytes, so four one-wave blocks fit a CU's 160 KB of LDS.
-constexpr int kSplitLanes = 60;
+// alike.  A wave holds kSplitLanes / K groups (kSplitLanes = 60); its term buffer is
+// [81][groups][K] doubles, kSplitLanes x 648 bytes, so four one-wave blocks fit a CU's
+// 160 KB of LDS.
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -242,7 +243,9 @@ struct CcSplit {
           buf[(i * NG + g) * K + j] = t;
         });
       wave_lds_sync();
-      const unsigned grp = (unsigned)(bal >> lead) & ((1u << K) - 1u);
+      // the group's active blocks (64-bit only for groups wider than 31 lanes)
+      using Mask = typename std::conditional<(K > 31), unsigned long long, unsigned>::type;
+      const Mask grp = (Mask)(bal >> lead) & (Mask)(((Mask)1 << K) - (Mask)1);
       // only the leads read: a quarter (K = 4) of the LDS traffic of a full wave
 #pragma unroll 1
       for (int sl = 1; sl < K; sl++) {
@@ -269,20 +272,22 @@ struct CcSplit {
   }
 };
 
-// Split factor for a hand-over list of N elements: the largest K (8, 6, 4, 2; more
-// instantiations cost the kernel's register allocation more than they save) whose items fit
-// one round of the grid, the expensive qps' (all but the mid-surface point of an odd rule)
-// counted whole and the mid-surface point's at 0.4 of one (measured on flat reference
-// shells, where its zero blocks are skipped): the waves left over by the expensive items
-// must absorb the cheap ones, or a wave that takes a cheap item after an expensive one sets
-// the kernel's time.
-__device__ __forceinline__ int split_factor(int N, int P, int n_waves) {
-  const bool mid = P > 1 && (P & 1);
-  const int pe = mid ? P - 1 : P;
-  for (int K = 8; K >= 2; K -= 2) {
-    const long long chunks = (N + kSplitLanes / K - 1) / (kSplitLanes / K);
-    if (chunks * (10 * pe + (mid ? 4 : 0)) <= 10LL * n_waves) return K;
+// Split factor for a hand-over list of N elements: the largest K that fits (split_fits).
+// The fit weights the mid-surface qp because the waves left over by the expensive items must
+// absorb the cheap ones, or a wave that takes a cheap item after an expensive one sets the
+// kernel's time.  K = 60, 30, 15 (short lists, a few hundred elements or fewer) run in
+// bf_exact_solve_wide_kernel, K = 8, 6, 4, 2, 1 in bf_exact_solve_queue_kernel: the two sets
+// compile as separate kernels, so neither's register allocation pays for the other's
+// instantiations (more K in one kernel slowed its common cases).  Without `wide` the narrow
+// set alone (the batch list kernel, and the queue kernel when the wide one is not launched).
+__device__ __forceinline__ int split_factor(int N, int P, int n_waves, bool wide) {
+  if (wide) {
+    if (split_fits(N, 60, P, n_waves)) return 60;
+    if (split_fits(N, 30, P, n_waves)) return 30;
+    if (split_fits(N, 15, P, n_waves)) return 15;
   }
+  for (int K = 8; K >= 2; K -= 2)
+    if (split_fits(N, K, P, n_waves)) return K;
   return 1;
 }
 
@@ -469,12 +474,32 @@ __global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
     int* __restrict__ counter, int n_waves) {
   __shared__ double buf[kSplitLanes * 81];
   const int N = *count;
-  switch (split_factor(N, kp.n_qp, n_waves)) {
+  switch (split_factor(N, kp.n_qp, n_waves, false)) {
     case 8: solve_list_k<8>(m, kp, x, y, z, list, N, counter, buf); break;
     case 6: solve_list_k<6>(m, kp, x, y, z, list, N, counter, buf); break;
     case 4: solve_list_k<4>(m, kp, x, y, z, list, N, counter, buf); break;
     case 2: solve_list_k<2>(m, kp, x, y, z, list, N, counter, buf); break;
     default: solve_list_k<1>(m, kp, x, y, z, list, N, counter, buf); break;
+  }
+}
+
+// The short lists: 15, 30 or 60 lanes per (element, qp), so a round forms 15 to 60 of the
+// sum's 81 blocks at once.  Runs before bf_exact_solve_queue_kernel and hands it its count:
+// 0 when this kernel takes the list, the list length when the list is too long for a wide
+// split (then this kernel does nothing else).
+__global__ void __launch_bounds__(64) bf_exact_solve_wide_kernel(
+    DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
+    int* __restrict__ counter, int n_waves, int* __restrict__ narrow_count) {
+  __shared__ double buf[kSplitLanes * 81];
+  const int N = *count;
+  const int K = N > 0 ? split_factor(N, kp.n_qp, n_waves, true) : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_count = K >= kWideSplitMin ? 0 : N;
+  switch (K) {
+    case 60: solve_list_k<60>(m, kp, x, y, z, list, N, counter, buf); break;
+    case 30: solve_list_k<30>(m, kp, x, y, z, list, N, counter, buf); break;
+    case 15: solve_list_k<15>(m, kp, x, y, z, list, N, counter, buf); break;
+    default: break;  // the queue kernel's
   }
 }
 
@@ -559,7 +584,7 @@ __device__ void batch_solve_list_k(const BatchDev& b, int N, double* buf) {
 __global__ void __launch_bounds__(64) bf_batch_exact_solve_list_kernel(BatchDev b, int n_waves) {
   __shared__ double buf[kSplitLanes * 81];
   const int N = *b.htotal;
-  switch (split_factor(N, b.P, n_waves)) {
+  switch (split_factor(N, b.P, n_waves, false)) {
     case 8: batch_solve_list_k<8>(b, N, buf); break;
     case 6: batch_solve_list_k<6>(b, N, buf); break;
     case 4: batch_solve_list_k<4>(b, N, buf); break;
@@ -732,14 +757,20 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 }
 
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
-                                         const double* y, const double* z, int n_cu,
+                                         const double* y, const double* z, int n_cu, bool wide,
                                          hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
-  // persistent grids (the list length is known on the device only)
-  // persistent: one wave per SIMD (the solve's occupancy)
+  // persistent grids of one wave per SIMD (the solve's occupancy; the list length is known
+  // on the device only).  With `wide` the wide solve kernel runs first and leaves the
+  // narrow one its count in status[7] (0 if it took the list); without it the narrow kernel
+  // reads the list length itself.
+  if (wide)
+    hipLaunchKernelGGL(bf_exact_solve_wide_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
+                       z, (const int*)m.fallback, (const int*)&m.status[4], &m.status[6],
+                       4 * n_cu, &m.status[7]);
   hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
-                     z, (const int*)m.fallback, (const int*)&m.status[4], &m.status[6],
-                     4 * n_cu);
+                     z, (const int*)m.fallback, (const int*)&m.status[wide ? 7 : 4],
+                     &m.status[6], 4 * n_cu);
   // 64-lane blocks: a short hand-over list still spreads over the CUs
   hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x, y, z,
                      (const int*)m.fallback, (const int*)&m.status[4], 0,

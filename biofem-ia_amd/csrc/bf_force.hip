@@ -516,6 +516,12 @@ struct bf_ctx {
   unsigned char* d_fixed = nullptr;
   double dt = 0.0, damp = 0.0;
   bool has_state = false;
+  // Launch of the hand-over solve per staged precision (index BF_PREC_*): whether the list
+  // length last read back for it fitted the wide split (a short list), so the wide solve
+  // kernel runs beside the narrow one.  False until a status read follows an evaluation of
+  // that precision; the choice changes the time, never the result.
+  bool wide_hint[3] = {false, false, false};
+  int fb_prec = -1;  // precision of the last evaluation if it ran the hand-over, else -1
   int* d_ptr = nullptr;
   unsigned* d_ent = nullptr;
   int* d_rules = nullptr;
@@ -955,6 +961,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
       hipLaunchKernelGGL(bf_element_kernel<float>, grid, block, 0, s, dm, kp, x, y, z);
   }
   BF_HIP(hipEventRecord(e1, s));
+  c->fb_prec = -1;
   if (c->E > 0 && precision != BF_PREC_FP64 && kp.fallback) {
     const int per_wave = 64 / kp.n_qp;
     const int n_waves = (c->E + per_wave - 1) / per_wave;
@@ -965,7 +972,9 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     }
     hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, c->d_wcount,
                        c->d_wlist, n_waves, per_wave, c->d_fallback, c->d_status);
-    BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, s));
+    BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
+                                         s));
+    c->fb_prec = precision;
   }
   BF_HIP(hipEventRecord(e2, s));
   if (step)
@@ -984,6 +993,9 @@ static int collect_status(bf_ctx* c, int* elem, int* code, int* n_warn) {
   BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   const int* st = c->h_status;
+  if (c->fb_prec >= 0)
+    c->wide_hint[c->fb_prec] =
+        st[4] > 0 && split_fits(st[4], kWideSplitMin, c->kp.n_qp, 4 * c->n_cu);
   int el = -1, cd = 0;
   if (st[0] != INT_MAX) { el = st[0]; cd = BF_ELEM_NONFINITE; }
   else if (st[2] != INT_MAX) { el = st[2]; cd = BF_ELEM_CC3333; }

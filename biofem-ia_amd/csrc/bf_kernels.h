@@ -46,6 +46,7 @@ struct DevMesh {
 };
 // status[5]: number of entries in redo
 // status[6]: work counter of the hand-over solve
+// status[7]: the narrow hand-over solve's list length, when the wide solve runs first
 
 // GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
 __device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) {
@@ -301,11 +302,28 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 // guard tripped with the qps coupled.  status[5] must be zero on entry.
 hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
                                 const double* y, const double* z, int n_cu, hipStream_t s);
+// The hand-over solve splits each (element, qp) of a list over K lanes of a wave, K from
+// the list length (largest first): 60, 30, 15 (the wide solve kernel), 8, 6, 4, 2, 1.  A
+// split fits when its items fill at most one round of the n_waves persistent waves, the
+// expensive qps (all but the mid-surface point of an odd rule) counted whole and the
+// mid-surface point's at 0.4 of one (measured on flat reference shells, where its zero blocks
+// are skipped).  kSplitLanes lanes of each wave take part.
+constexpr int kSplitLanes = 60;
+constexpr int kWideSplitMin = 15;
+__host__ __device__ inline bool split_fits(int N, int K, int P, int n_waves) {
+  const bool mid = P > 1 && (P & 1);
+  const int pe = mid ? P - 1 : P;
+  const long long chunks = (N + kSplitLanes / K - 1) / (kSplitLanes / K);
+  return chunks * (10 * pe + (mid ? 4 : 0)) <= 10LL * n_waves;
+}
+
 // Re-evaluates, in the reference's operation order, the elements a staged kernel listed in
-// m.fallback[0 .. m.status[4]) (persistent grids, count read on the device; the solve takes
-// its work items from status[6], which must be zero on entry).
+// m.fallback[0 .. m.status[4]) (persistent grids of 4 n_cu waves, count read on the device;
+// the solve takes its work items from status[6], which must be zero on entry).  `wide` also
+// launches the solve kernel for short lists (split of kWideSplitMin lanes or more); without
+// it the narrow kernel takes every list -- the sums, and so the results, are the same.
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
-                                         const double* y, const double* z, int n_cu,
+                                         const double* y, const double* z, int n_cu, bool wide,
                                          hipStream_t s);
 // Batch forms: the reference-order path over the element tiles (plain FP64 batches) and
 // over the compacted hand-over list b.hlist[0 .. *b.htotal).  b.rtotal[0..1] must be zero

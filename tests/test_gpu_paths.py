@@ -6,6 +6,7 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import torch
 
 from biofem_amd import _lib as L
 from biofem_amd import configs
@@ -35,23 +36,33 @@ def check_all(m, p, x, y, z, tol_exact=0.0, tol_fast=1e-12, tol_mixed=1e-5, omp=
         assert e <= tol_exact
         if tol_exact == 0.0:
             assert np.array_equal(fe.c33(), c33)
+        # Each staged path twice: the first evaluation runs the narrow hand-over solve, the
+        # second the solve the list length read back after the first picks (the wide kernel
+        # for short lists); both add the CC terms in the reference's order, so the two
+        # results are identical.
         fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
         nf = fe.fallback_count()
+        assert np.array_equal(fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST), fast)
+        assert fe.fallback_count() == nf
         # handed-over elements run in the reference order: bit for bit the oracle's C33
         handed = (fe.elem_flags() & L.FLAG_FALLBACK) != 0
         assert int(handed.sum()) == nf
         assert np.array_equal(fe.c33()[handed], c33[handed])
         mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
         nm = fe.fallback_count()
+        assert np.array_equal(fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED), mixed)
+        assert fe.fallback_count() == nm
         handed = (fe.elem_flags() & L.FLAG_FALLBACK) != 0
         assert int(handed.sum()) == nm
         assert np.array_equal(fe.c33()[handed], c33[handed])
         fe.set_fallback(False)
         fast0 = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
         mixed0 = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
-        print(f"  fast {rel(fast, ref):.3e} ({nf} handed over) mixed {rel(mixed, ref):.3e} "
-              f"({nm} handed over); without hand-over: fast {rel(fast0, ref):.3e} "
-              f"mixed {rel(mixed0, ref):.3e}", flush=True)
+        waves = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+        print(f"  fast {rel(fast, ref):.3e} ({nf} handed over, split "
+              f"{split_lanes(nf, p.n_qp, waves)}) mixed {rel(mixed, ref):.3e} ({nm} handed "
+              f"over, split {split_lanes(nm, p.n_qp, waves)}); without hand-over: fast "
+              f"{rel(fast0, ref):.3e} mixed {rel(mixed0, ref):.3e}", flush=True)
         assert rel(fast, ref) <= tol_fast
         assert rel(mixed, ref) <= tol_mixed
         return th, c33
@@ -78,12 +89,25 @@ def test_membrane_large_staged_paths():
     check_all(m, p, *xyz, omp=16)
 
 
-@pytest.mark.parametrize("sigma", [7e-4, 1e-3, 1.2e-3, 1.5e-3])
+def split_lanes(n, n_qp, n_waves):
+    """Lanes per (element, qp) the hand-over solve picks for a list of n elements: the
+    largest of 60, 30, 15 (the wide kernel), 8, 6, 4, 2 whose items fit one round of the
+    persistent grid, the mid-surface qp of an odd rule weighted 0.4; else 1."""
+    mid = n_qp > 1 and n_qp % 2 == 1
+    pe = n_qp - 1 if mid else n_qp
+    for k in (60, 30, 15, 8, 6, 4, 2):
+        chunks = -(-n // (60 // k))
+        if chunks * (10 * pe + (4 if mid else 0)) <= 10 * n_waves:
+            return k
+    return 1
+
+
+@pytest.mark.parametrize("sigma", [4e-4, 5.5e-4, 6.5e-4, 7e-4, 1e-3, 1.2e-3, 1.5e-3])
 def test_membrane_handover_sizes(sigma):
-    """FP64 hand-over lists of about 0.9k, 6.0k, 11k and 19k of the 98 304 elements (current
-    z noise N(0, sigma), seed 11), the mixed path's of 0.3k, 3.4k, 7.1k and 14k: on a 256-CU
-    device the hand-over solve splits each (element, qp) over 8, 4, 2, 1 lanes and 8, 6, 2, 1
-    lanes for them; every handed-over element stays bit for bit the oracle's."""
+    """Hand-over lists from a few dozen to 19k of the 98 304 elements (current z noise
+    N(0, sigma), seed 11), for the FP64 and the mixed path: on a 256-CU device the solve
+    splits each (element, qp) over 60, 30, 15 (the wide kernel), 8, 6, 4, 2 or 1 lanes by the
+    list length, and every handed-over element stays bit for bit the oracle's."""
     m, p, (x, y, z) = configs.membrane_large()
     rng = np.random.default_rng(11)
     z = m.z0.copy()

@@ -148,8 +148,11 @@ int bf_update_thickness(bf_ctx *ctx, double *thickness);
 int bf_last_error(bf_ctx *ctx, int *elem, int *code, int *n_warn);
 
 /* Number of elements the last staged evaluation recomputed in the reference's operation
- * order; bf_set_fallback(ctx, 0) disables that hand-over (default on).  No reference
- * counterpart: the reference has a single arithmetic path. */
+ * order; bf_set_fallback(ctx, 0) disables that hand-over (default on).  The count read
+ * here (and by bf_force_eval, bf_last_error) also steers the next evaluation of that
+ * precision: after a short list the hand-over solve adds a kernel that spreads each
+ * element over more lanes.  The results do not depend on it.  No reference counterpart:
+ * the reference has a single arithmetic path. */
 int bf_last_fallback_count(bf_ctx *ctx, int *n);
 int bf_set_fallback(bf_ctx *ctx, int on);
 

@@ -275,11 +275,11 @@ struct CcSplit {
 // Split factor for a hand-over list of N elements: the largest K that fits (split_fits).
 // The fit weights the mid-surface qp because the waves left over by the expensive items must
 // absorb the cheap ones, or a wave that takes a cheap item after an expensive one sets the
-// kernel's time.  K = 60, 30, 15 (short lists, a few hundred elements or fewer) run in
-// bf_exact_solve_wide_kernel, K = 8, 6, 4, 2, 1 in bf_exact_solve_queue_kernel: the two sets
-// compile as separate kernels, so neither's register allocation pays for the other's
-// instantiations (more K in one kernel slowed its common cases).  Without `wide` the narrow
-// set alone (the batch list kernel, and the queue kernel when the wide one is not launched).
+// kernel's time.  K = 60, 30, 15 (short lists, up to ~1.7k elements at n_qp = 3 on 256
+// CUs) run in bf_exact_solve_wide_kernel (`wide`), K = 8, 6, 4, 2, 1 in
+// bf_exact_solve_queue_kernel and the batch list kernel (not `wide`): the two sets compile
+// as separate kernels, so neither's register allocation pays for the other's
+// instantiations (more K in one kernel slowed its common cases).
 __device__ __forceinline__ int split_factor(int N, int P, int n_waves, bool wide) {
   if (wide) {
     if (split_fits(N, 60, P, n_waves)) return 60;

@@ -159,6 +159,9 @@ class NewmarkJFNK:
     internal_force(U) -> Fint [3 n_nodes] with ghost rows zero; `for_fe` builds it from an
     ActStrainFE (GlobalGhost + the library's force on device).  mass [n_nodes]: assembled
     lumped masses (lumped_masses); fixed [3 n_nodes] bool: degrees of freedom held at x0.
+    damping / dampfactor: -damping and -dampfactor; the mass-proportional damping matrix
+    C = dampfactor M enters the residual only when damping is set, as FDynamic applies Damp
+    only under -damping (src/main.c:1257, :1326), otherwise C = 0.
     element_thickness() -> [n_elmt]: UpdateElementThickness of the last force evaluation,
     called once per converged step as the reference does (src/main.c:497-503) and kept in
     `thickness` (default: internal_force.thickness when the callable has one)."""
@@ -166,7 +169,8 @@ class NewmarkJFNK:
     def __init__(self, internal_force, x0: torch.Tensor, mass: torch.Tensor, dt: float,
                  n_v: int | None = None, fixed: torch.Tensor | None = None,
                  fext: torch.Tensor | None = None, newmark_gamma: float = 0.5,
-                 dampfactor: float = 0.0, element_thickness=None, **solver):
+                 damping: bool = False, dampfactor: float = 0.0, element_thickness=None,
+                 **solver):
         self.fint = internal_force
         self.element_thickness = (element_thickness if element_thickness is not None
                                   else getattr(internal_force, "thickness", None))
@@ -181,7 +185,7 @@ class NewmarkJFNK:
         self.C1, self.C2, self.C3 = g / (b * dt), g / b - 1.0, dt * (g / (2.0 * b) - 1.0)
         self.V2, self.V3 = (1.0 - g) * dt, g * dt          # xAccVel's C2, C3
         self.M = mass.repeat_interleave(3).to(x0)
-        self.C = dampfactor * self.M
+        self.C = (dampfactor if damping else 0.0) * self.M
         self.fixed = (torch.zeros(n3, dtype=torch.bool, device=x0.device) if fixed is None
                       else fixed.to(x0.device))
         self.ghost = torch.zeros(n3, dtype=torch.bool, device=x0.device)

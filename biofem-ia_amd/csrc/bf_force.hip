@@ -989,7 +989,16 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   return kernel_launch_check();
 }
 
+// Orders the context's stream after the last evaluation, which may have run on a caller's
+// stream (bf_force_eval_device): ev[3] is recorded on the evaluation's stream after its last
+// kernel.  Every read of a per-evaluation result on the context's stream goes through here.
+static int after_last_eval(bf_ctx* c) {
+  if (c->timing) BF_HIP(hipStreamWaitEvent(c->stream, c->ev[3], 0));
+  return BF_OK;
+}
+
 static int collect_status(bf_ctx* c, int* elem, int* code, int* n_warn) {
+  if (int rc = after_last_eval(c)) return rc;
   BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   const int* st = c->h_status;
@@ -1066,6 +1075,7 @@ int bf_c33_download(bf_ctx* c, double* c33) {
   BF_HIP(hipSetDevice(c->device));
   const int q = c->kp.n_qp, E = c->E;
   std::vector<double> buf((size_t)q * std::max(E, 1));
+  if (int rc = after_last_eval(c)) return rc;
   BF_HIP(hipMemcpyAsync(buf.data(), c->d_c33, sizeof(double) * (size_t)q * E, hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   for (int e = 0; e < E; e++)
@@ -1077,6 +1087,7 @@ int bf_elem_flags_download(bf_ctx* c, unsigned char* flags) {
   if (!c || !flags) return BF_ERR_ARG;
   if (!c->has_mesh) return BF_ERR_STATE;
   BF_HIP(hipSetDevice(c->device));
+  if (int rc = after_last_eval(c)) return rc;
   BF_HIP(hipMemcpyAsync(flags, c->d_flags, (size_t)c->E, hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   return BF_OK;
@@ -1102,14 +1113,15 @@ int bf_update_thickness(bf_ctx* c, double* thickness) {
 int bf_last_error(bf_ctx* c, int* elem, int* code, int* n_warn) {
   if (!c) return BF_ERR_ARG;
   BF_HIP(hipSetDevice(c->device));
-  collect_status(c, elem, code, n_warn);
-  return BF_OK;
+  const int rc = collect_status(c, elem, code, n_warn);
+  return rc == BF_ERR_ELEMENT ? BF_OK : rc;  // the element failure is reported in *code
 }
 
 int bf_last_fallback_count(bf_ctx* c, int* n) {
   if (!c || !n) return BF_ERR_ARG;
   BF_HIP(hipSetDevice(c->device));
-  collect_status(c, nullptr, nullptr, nullptr);
+  const int rc = collect_status(c, nullptr, nullptr, nullptr);
+  if (rc != BF_OK && rc != BF_ERR_ELEMENT) return rc;
   *n = c->h_status[4];
   return BF_OK;
 }

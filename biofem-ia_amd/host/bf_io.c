@@ -72,7 +72,8 @@ int bf_io_dimension(const char *in_dir, int ibi, int *n_v, int *n_elmt, int *n_e
 /* ---- edge table for the neighbour search ---- */
 typedef struct {
   uint64_t key;  /* (lo + 1) << 32 | (hi + 1); 0 = empty */
-  int hi_e, lo_e; /* the two highest-numbered triangles on the edge (-1: none) */
+  int id;        /* edge number: row of the incident-triangle lists */
+  int last_e;    /* last triangle counted on the edge (a triangle lists an edge once) */
 } EdgeSlot;
 
 static uint64_t edge_key(int a, int b) {
@@ -80,68 +81,111 @@ static uint64_t edge_key(int a, int b) {
   return ((uint64_t)(lo + 1u) << 32) | (uint64_t)(hi + 1u);
 }
 
-static EdgeSlot *edge_find(EdgeSlot *t, size_t cap, uint64_t key, int insert) {
+static EdgeSlot *edge_find(EdgeSlot *t, size_t cap, uint64_t key, int insert, int *n_edges) {
   size_t h = (size_t)((key * 0x9E3779B97F4A7C15ull) >> 17) & (cap - 1);
   for (;;) {
     if (t[h].key == key) return &t[h];
     if (t[h].key == 0) {
       if (!insert) return NULL;
       t[h].key = key;
-      t[h].hi_e = t[h].lo_e = -1;
+      t[h].id = (*n_edges)++;
+      t[h].last_e = -1;
       return &t[h];
     }
     h = (h + 1) & (cap - 1);
   }
 }
 
-/* nv4/nv5/nv6 of Input (src/io.c:476-519): for element i and each other element j that
- * shares exactly two of i's corners, the third corner of j goes to nv4 (shared corners
- * nv2, nv3), nv5 (nv1, nv3) or nv6 (nv1, nv2); j runs in increasing order, so the last
- * match wins.  Equivalent: the highest-numbered other triangle on that edge. */
+/* nv4/nv5/nv6 of Input (src/io.c:476-519): for element i and each element j in increasing
+ * order, mn counts i's corners found among j's; mn == 2 writes j's first corner that is
+ * not one of i's to nv4 (shared corners nv2, nv3), nv5 (nv1, nv3) or nv6 (nv1, nv2), so
+ * the last such j wins.  Every j with mn == 2 holds both shared corners, so it is listed
+ * under that pair in an edge table of all incident triangles (ascending): the search
+ * scans the pair's list from its end for the first j with mn == 2.  A triangle with a
+ * repeated corner (not a valid surface triangle) gets no neighbour across that pair. */
 static int neighbours(int n_elmt, const int *nv1, const int *nv2, const int *nv3, int *nv4,
                       int *nv5, int *nv6) {
   size_t cap = 16;
   while (cap < (size_t)n_elmt * 6 + 16) cap <<= 1;
   EdgeSlot *t = (EdgeSlot *)calloc(cap, sizeof(EdgeSlot));
-  if (!t) return BF_ERR_NOMEM;
+  int *ptr = (int *)calloc((size_t)3 * n_elmt + 2, sizeof(int));
+  int *lst = (int *)malloc(((size_t)3 * n_elmt + 1) * sizeof(int));
+  if (!t || !ptr || !lst) {
+    free(t); free(ptr); free(lst);
+    return BF_ERR_NOMEM;
+  }
+  int n_edges = 0;
+  /* pass 1: edge numbers and the number of triangles per edge */
   for (int j = 0; j < n_elmt; j++) {
     const int c[3] = {nv1[j], nv2[j], nv3[j]};
     for (int k = 0; k < 3; k++) {
       const int a = c[k], b = c[(k + 1) % 3];
       if (a == b) continue;
-      EdgeSlot *s = edge_find(t, cap, edge_key(a, b), 1);
-      if (s->hi_e == j) continue;
-      s->lo_e = s->hi_e;
-      s->hi_e = j;
+      EdgeSlot *s = edge_find(t, cap, edge_key(a, b), 1, &n_edges);
+      if (s->last_e == j) continue;
+      s->last_e = j;
+      ptr[s->id + 1]++;
     }
+  }
+  for (int e = 0; e < n_edges; e++) ptr[e + 1] += ptr[e];
+  /* pass 2: the lists, ascending in j */
+  for (size_t h = 0; h < cap; h++) t[h].last_e = -1;
+  {
+    int *fill = (int *)malloc(((size_t)n_edges + 1) * sizeof(int));
+    if (!fill) {
+      free(t); free(ptr); free(lst);
+      return BF_ERR_NOMEM;
+    }
+    memcpy(fill, ptr, (size_t)n_edges * sizeof(int));
+    for (int j = 0; j < n_elmt; j++) {
+      const int c[3] = {nv1[j], nv2[j], nv3[j]};
+      for (int k = 0; k < 3; k++) {
+        const int a = c[k], b = c[(k + 1) % 3];
+        if (a == b) continue;
+        EdgeSlot *s = edge_find(t, cap, edge_key(a, b), 0, &n_edges);
+        if (s->last_e == j) continue;
+        s->last_e = j;
+        lst[fill[s->id]++] = j;
+      }
+    }
+    free(fill);
   }
   for (int i = 0; i < n_elmt; i++) {
     const int c[3] = {nv1[i], nv2[i], nv3[i]};
-    /* edge opposite corner k: (nv2,nv3) -> nv4, (nv1,nv3) -> nv5, (nv1,nv2) -> nv6 */
+    /* pair without corner k: (nv2,nv3) -> nv4, (nv1,nv3) -> nv5, (nv1,nv2) -> nv6 */
     int out[3] = {BF_NO_NEIGHBOUR, BF_NO_NEIGHBOUR, BF_NO_NEIGHBOUR};
     for (int k = 0; k < 3; k++) {
       const int a = c[(k + 1) % 3], b = c[(k + 2) % 3];
       if (a == b) continue;
-      EdgeSlot *s = edge_find(t, cap, edge_key(a, b), 0);
+      EdgeSlot *s = edge_find(t, cap, edge_key(a, b), 0, &n_edges);
       if (!s) continue;
-      const int j = (s->hi_e != i) ? s->hi_e : s->lo_e;
-      if (j < 0) continue;
-      /* j shares exactly the corners a, b with i: its third corner */
-      const int d[3] = {nv1[j], nv2[j], nv3[j]};
-      int shared = 0, other = -1;
-      for (int q = 0; q < 3; q++) {
-        if (d[q] == c[0] || d[q] == c[1] || d[q] == c[2])
-          shared++;
-        else
-          other = d[q];
+      for (int p = ptr[s->id + 1] - 1; p >= ptr[s->id]; p--) {
+        const int j = lst[p];
+        const int d[3] = {nv1[j], nv2[j], nv3[j]};
+        int mn = 0;
+        int in_j[3];
+        for (int q = 0; q < 3; q++) {
+          in_j[q] = (c[q] == d[0] || c[q] == d[1] || c[q] == d[2]);
+          mn += in_j[q];
+        }
+        /* mn == 2 with a and b shared means corner k is the one missing from j */
+        if (mn != 2 || in_j[k]) continue;
+        int npe = d[2];
+        if (d[0] != c[0] && d[0] != c[1] && d[0] != c[2])
+          npe = d[0];
+        else if (d[1] != c[0] && d[1] != c[1] && d[1] != c[2])
+          npe = d[1];
+        out[k] = npe;
+        break;
       }
-      if (shared == 2 && other >= 0) out[k] = other;
     }
     if (nv4) nv4[i] = out[0];
     if (nv5) nv5[i] = out[1];
     if (nv6) nv6[i] = out[2];
   }
   free(t);
+  free(ptr);
+  free(lst);
   return BF_OK;
 }
 

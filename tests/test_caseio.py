@@ -104,6 +104,24 @@ def test_neighbour_search_last_match_wins(tmp_path):
     assert np.array_equal(r["nv456"], _neighbours_bruteforce(tri))
 
 
+def test_neighbour_search_duplicate_triangle(tmp_path):
+    """A triangle repeated with the same three corners (and one listed twice more on the
+    edge): the duplicate shares all three corners with its twin, so Input's search skips it
+    and takes the highest-numbered triangle sharing exactly two corners."""
+    m = flat_membrane(5, 4, jitter=0.05, seed=6)
+    tri = np.asarray(m.tri[: m.n_elmt]).copy()
+    a, b = tri[9][0], tri[9][2]
+    c = [v for v in (17, 22, 11) if v not in tri[9]][0]
+    extra = np.array([tri[9][[2, 0, 1]], [a, b, c], tri[9]])
+    tri = np.concatenate([tri, extra])
+    _write_lists(str(tmp_path), m.x0[: m.n_v], m.y0[: m.n_v], m.z0[: m.n_v], tri)
+    r = caseio.read_lists(str(tmp_path))
+    ref = _neighbours_bruteforce(tri)
+    assert np.array_equal(r["nv456"], ref)
+    # the element on the shared edge keeps a neighbour (not the empty marker)
+    assert (ref[9] != 1000000).sum() >= (ref[len(tri) - 1] != 1000000).sum() > 0
+
+
 def test_read_case_runs_the_reference_setup():
     m = caseio.read_case(CASE)
     assert m.extra["rings"] == [[0, 1, 2, 3, 7, 11, 10, 9, 8, 4]]  # four open sides joined

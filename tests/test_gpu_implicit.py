@@ -58,7 +58,8 @@ def test_implicit_steps_solve_the_reference_residual(dampfactor):
         dev = torch.device("cuda")
         st = NewmarkJFNK(for_fe(fe, L.BF_PREC_FP64_FAST), torch.as_tensor(x0, device=dev),
                          torch.as_tensor(mass, device=dev), dt, n_v=m.n_v,
-                         fixed=torch.as_tensor(fixed), dampfactor=dampfactor)
+                         fixed=torch.as_tensor(fixed), damping=dampfactor != 0.0,
+                         dampfactor=dampfactor)
         # the library's force at the reference state against the oracle's
         f_dev = st.fint(st.x).cpu().numpy()
         f_ref, _, _ = O.finternal_act(m, p, *(a.copy() for a in (m.x0, m.y0, m.z0)), want_c33=True)

@@ -239,3 +239,42 @@ def test_nonfinite_case_reported():
         assert int((fe.elem_flags() & L.FLAG_CC3333).astype(bool).sum()) == warn
     finally:
         fe.close()
+
+
+def test_status_after_eval_on_caller_stream():
+    """bf_last_error, bf_elem_flags_download and bf_c33_download read the status of an
+    evaluation that bf_force_eval_device ran on a caller's stream, without the caller
+    synchronising that stream first: the reads are ordered after it."""
+    m, p, (x, y, z) = _noisy_reference(128, 3e-3, 0)
+    print("case non-finite 128x96 on a caller stream", flush=True)
+    ref, st, c33 = O.finternal_act(m, p, x, y, z, want_c33=True)
+    fe = ActStrainFE(m, p)
+    try:
+        # a finite evaluation first, on the context's stream: its status has no
+        # non-finite element
+        xs, ys, zs = configs.membrane_large(128, 96)[2]
+        fe.finternal_act(xs, ys, zs, precision=L.BF_PREC_FP64)
+        assert fe.last_error()[1] != L.BF_ELEM_NONFINITE
+        dev = torch.device("cuda:0")
+        bx, by, bz = (torch.as_tensor(a, device=dev) for a in (x, y, z))
+        f = torch.zeros(3 * m.n_nodes, dtype=torch.float64, device=dev)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        vp = C.c_void_p
+        for _ in range(2):
+            with torch.cuda.stream(side):
+                rc = fe.lib.bf_force_eval_device(fe.ctx, vp(bx.data_ptr()), vp(by.data_ptr()),
+                                                 vp(bz.data_ptr()), vp(f.data_ptr()), 0,
+                                                 L.BF_PREC_FP64, vp(side.cuda_stream))
+            assert rc == L.BF_OK
+            e, code, warn = fe.last_error()      # no side.synchronize() before the read
+            assert code == L.BF_ELEM_NONFINITE and e >= 0
+            assert int((fe.elem_flags() & L.FLAG_CC3333).astype(bool).sum()) == warn
+            assert np.array_equal(fe.c33(), c33)
+            # and back to a finite state on the context's stream
+            fe.finternal_act(xs, ys, zs, precision=L.BF_PREC_FP64)
+            assert fe.last_error()[1] != L.BF_ELEM_NONFINITE
+        side.synchronize()
+        assert np.array_equal(f.cpu().numpy(), ref, equal_nan=True)
+    finally:
+        fe.close()

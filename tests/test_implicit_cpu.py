@@ -96,8 +96,8 @@ def test_damping_and_fixed_dofs():
 
     fixed = np.zeros(3 * n, bool)
     fixed[0] = True                               # node 0 held in x
-    st = NewmarkJFNK(fint, T(x0), T(mass), dt, fixed=torch.as_tensor(fixed), dampfactor=2.0,
-                     rtol=1e-12)
+    st = NewmarkJFNK(fint, T(x0), T(mass), dt, fixed=torch.as_tensor(fixed), damping=True,
+                     dampfactor=2.0, rtol=1e-12)
     st.x[3::3] = 0.02
     st.x[0] = 0.01                                # off its reference value: pulled back
     st.xn = st.x.clone()
@@ -108,6 +108,31 @@ def test_damping_and_fixed_dofs():
         energies.append(0.5 * v @ (mass * v) + 0.5 * u @ K @ u)
         assert abs(st.x[0].item()) <= 1e-10       # held by its residual row, to the tolerance
     assert all(e2 <= e1 * (1 + 1e-12) for e1, e2 in zip(energies[1:], energies[2:]))
+
+
+def test_dampfactor_needs_damping_flag():
+    """-dampfactor alone adds no damping: FDynamic applies Damp only under -damping
+    (src/main.c:1257, :1326), so the residual is the undamped one, step for step."""
+    n, dt = 8, 0.02
+    K, mass = _chain(n)
+
+    def fint(U):
+        out = np.zeros(3 * n)
+        out[0::3] = K @ U.numpy()[0::3]
+        return T(out)
+
+    runs = {}
+    for key, kw in (("plain", {}), ("factor_only", dict(dampfactor=3.0)),
+                    ("damped", dict(damping=True, dampfactor=3.0))):
+        st = NewmarkJFNK(fint, T(np.zeros(3 * n)), T(mass), dt, rtol=1e-12, **kw)
+        st.x[3::3] = 0.02
+        st.xn = st.x.clone()
+        for _ in range(5):
+            assert st.step().converged
+        runs[key] = st.x.numpy().copy()
+        assert torch.count_nonzero(st.C).item() == (n * 3 if key == "damped" else 0)
+    assert np.array_equal(runs["plain"], runs["factor_only"])
+    assert not np.allclose(runs["plain"], runs["damped"], rtol=0, atol=1e-9)
 
 
 def test_thickness_hook_runs_once_per_converged_step():

@@ -4,18 +4,32 @@
  * operation rounds as the reference's C source does.  Each function cites the
  * reference lines it restates; arithmetic order follows those lines.
  *
- * Parity pins (tests/test_oracle.py, tests/test_topology.py):
- *  - constants (Na/Nab tables, Gauss rule): literal tables, checked against exact
- *    fractions to 4 decimals and Gauss-Legendre moments.
- *  - regular geometry: closed form on affine patches (a_alpha = A * dX/dxi).
- *  - irregular IN* tables: affine reproduction, partition of unity of the B1 rows,
- *    v = 6 sub-patch consistency.
- *  - constitutive chain: gamma = 0 at x = X gives S = 0 and Fint = 0 up to the
- *    KL normal-derivative term; stress-free active state (x = Fa X on a flat plate);
- *    objectivity under rotations about the origin; uniaxial plane-stress fixed
- *    point of the C33 Newton loop against the closed-form Neo-Hookean root.
- *  - assembly: brute-force per-node sum on a 4x3 grid.
- *  - topology: Patch slots checked against the geometric one-ring on lattices.
+ * Parity pins, per function (tests/test_oracle.py = T, tests/test_oracle_pins.py = P,
+ * tests/test_topology.py = Y); none retypes the formula it checks:
+ *  - Na_c/Nab_c tables (:69-88): derivatives of the 12 Loop box-spline basis functions at
+ *    the centre, symbolic, rounded to 4 decimals (T).
+ *  - bfo_gauss (:174-230): Gauss-Legendre moments and numpy's leggauss (T).
+ *  - bfo_gamma_of_time (:1156-1179): closed-form ramp values of types 0-4 (T).
+ *  - bfo_geom_subdiv regular (:477-558): affine and quadratic reproduction (T); irregular
+ *    (:563-719) and bfo_irregular_tables: one independent Loop subdivision step, affine row
+ *    sums (T); LV cap CST (:448-472): uniform stretch against the textbook CST force (P).
+ *  - a3_alpha (:745-762) and elem_update_g (:896-982) off the mid-surface: prescribed
+ *    polynomial surface, symbolic derivative of the unnormalised normal, closed-form C33
+ *    root and stress per through-thickness point (P).
+ *  - elem_act_def_grad (:1196-1329): stress-free active state x = Fa X for the fibre law (T),
+ *    C33 closed form for the isotropic law (T), stress-free manufactured state
+ *    x = lambda(t) X (P).
+ *  - elem_cg_def_tens .. elem_tot_tang, elem_c33_solve (:1332-1648, :2048-2138): uniaxial
+ *    and isotropic C33 roots and the rate 1/2 of the half-Newton step (T), in-plane
+ *    stress closed form at every qp (P), objectivity, force-free reference state (T).
+ *  - mod_elem_c33 (:1654-1721): bisection clamp against its grid closed form, CC3333 guard
+ *    skip and trip count (P).
+ *  - elem_upd_fint (:1762-1995): membrane and bending rows against finite differences of
+ *    the metric and curvature, regular and irregular patches (P); cap CST (P).
+ *  - assemble / bfo_finternal_act (:2226-2267): brute-force element-order sum, accumulate,
+ *    OpenMP build bitwise (T).
+ *  - bfo_area_normal, bfo_irrver, bfo_patch, bfo_global_ghost_init/bfo_global_ghost:
+ *    lattice areas (T), the hashed native builders and geometric one-rings (Y).
  * The reference binary cannot be built here (PETSc/MPI absent), so no function has
  * golden vectors from the reference itself; bitwise agreement with the reference
  * executable (VecSetValues summation included) is therefore parity unpinned.

@@ -384,7 +384,7 @@ def main():
     ap.add_argument("--serve-cases", type=int, default=400,
                     help="cases of the served sweep (2000 = the full sweep, ~100 s of arrivals)")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
-    ap.add_argument("--implicit-steps", type=int, default=20,
+    ap.add_argument("--implicit-steps", type=int, default=100,
                     help="implicit Newmark steps of the 32x24 membrane (0: skip)")
     ap.add_argument("--implicit-cpu-steps", type=int, default=2)
     ap.add_argument("--ellipsoid", type=int, default=1,

@@ -2,8 +2,11 @@
 // biofem-ia_amd/csrc/bf_elem.h (the same header the gfx950 kernels include).
 //
 // Test-only: it lets the CPU test suite check the staged constitutive algebra and the
-// FP32-mixed error budget against the oracle without a GPU.  It is never linked into
-// or called by the product library.
+// FP32-mixed error budget against the oracle without a GPU, and measure the reference
+// order's own rounding against the staged chain in x87 extended precision (precision 6).
+// It is never linked into or called by the product library.
+// precision: 0 staged FP64, 1 FP32-mixed, 2 reference order, 3 staged FP32,
+//            4 / 5 staged FP64 / FP32-mixed with the hand-over, 6 staged extended.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -135,7 +138,12 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
       }
       continue;
     }
-    if (precision == 0) {
+    if (precision == 6) {  // the staged chain in x87 extended precision (64-bit mantissa)
+      Resultants<long double> r = integrate_element<long double>(
+          Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act, h0, mu, K, n_qp, iters, theta, wq);
+      N0 = r.N0; N1 = r.N1; N2 = r.N2; M0 = r.M0; M1 = r.M1; M2 = r.M2; st = r.status;
+      if (c33_out) for (int q = 0; q < n_qp; q++) c33_out[(size_t)e * n_qp + q] = r.c33[q];
+    } else if (precision == 0) {
       Resultants<double> r = integrate_element<double>(Gc.a1, Gc.a2, Gc.a3, a31c, a32c, R0, act,
                                                        h0, mu, K, n_qp, iters, theta, wq);
       N0 = r.N0; N1 = r.N1; N2 = r.N2; M0 = r.M0; M1 = r.M1; M2 = r.M2; st = r.status;

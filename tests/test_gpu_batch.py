@@ -15,6 +15,10 @@ pytestmark = pytest.mark.gpu
 PRECS = [L.BF_PREC_FP64, L.BF_PREC_FP64_FAST, L.BF_PREC_MIXED]
 
 
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
 def _cases(n, seed=2, nx_hi=40, mixed_qp=False):
     cases = configs.batch_membranes(n=n, seed=seed, nx_lo=8, nx_hi=nx_hi)
     if mixed_qp:  # lane groups sized by the largest n_qp; smaller cases leave lanes idle
@@ -192,6 +196,39 @@ def test_batch_case_status_flags_diverged_case(prec):
         b.sync()
         nf, cc = b.case_status()
         assert (nf == -1).all() and (cc == -1).all(), (nf, cc)
+    finally:
+        b.close()
+        for fe in fes:
+            fe.close()
+
+
+def test_batch_config_full_size_vs_oracle():
+    """The 64-membrane batch at its stated sizes (nx log-uniform in [16, 256], seed 2;
+    1.02 M triangles), autotuned launch shape, hipGraph replays: every case's forces
+    against the OpenMP oracle -- plain FP64 bit for bit, staged FP64 within 1e-12 and
+    FP32-mixed within 1e-5 (2-norm relative, per case)."""
+    cases = configs.batch_membranes()
+    n_el = sum(m.n_elmt for m, _, _ in cases)
+    print(f"case batch of {len(cases)} membranes, {n_el} triangles: oracle", flush=True)
+    refs = [O.finternal_act(m, p, x, y, z, omp_threads=16)[0] for m, p, (x, y, z) in cases]
+    fes = _resident(cases)
+    b = Batch(fes)
+    try:
+        best = b.tune(L.BF_PREC_MIXED, reps=2)
+        print("  tuned", best, flush=True)
+        worst = {}
+        for prec in PRECS:
+            for rep in range(2):           # capture, then a replay
+                b.eval(prec, ghosts=True, graph=True)
+                b.sync()
+                errs = [rel(b.forces(i), refs[i]) for i in range(len(cases))]
+                if prec == L.BF_PREC_FP64:
+                    assert all(np.array_equal(b.forces(i), refs[i]) for i in range(len(cases)))
+                worst[prec] = max(errs)
+            print(f"  precision {prec}: worst case rel err {worst[prec]:.3e}, handed over "
+                  f"{b.fallback_count()}", flush=True)
+        assert worst[L.BF_PREC_FP64_FAST] <= 1e-12
+        assert worst[L.BF_PREC_MIXED] <= 1e-5
     finally:
         b.close()
         for fe in fes:

@@ -94,3 +94,44 @@ def test_implicit_steps_solve_the_reference_residual(dampfactor):
         assert np.abs(U - x0).max() > 1e-6      # the active membrane moved
     finally:
         fe.close()
+
+
+def test_implicit_100_steps_config1():
+    """The 32 x 24 active membrane (x = 0 edge clamped, rho 1, dt 1e-2, no damping) for
+    100 implicit Newmark steps, as the reference's main loop runs them (src/main.c:420-575):
+    every step converges, and after every step FormFunctionFEM's residual restated on the
+    oracle's force is at the Newton tolerance relative to the step's starting residual."""
+    m, p, _ = configs.membrane_small()
+    print("case membrane 32x24, 100 implicit Newmark steps", flush=True)
+    rho, dt = 1.0, 1e-2
+    fe = ActStrainFE(m, p)
+    try:
+        x0 = np.stack([m.x0, m.y0, m.z0], 1).reshape(-1)
+        fixed = np.zeros(3 * m.n_nodes, bool)
+        clamp = np.where(np.isclose(m.x0[: m.n_v], 0.0))[0]
+        for d in range(3):
+            fixed[3 * clamp + d] = True
+        mass = lumped_masses(m, rho, p.h0)
+        dev = torch.device("cuda")
+        st = NewmarkJFNK(for_fe(fe, L.BF_PREC_FP64_FAST), torch.as_tensor(x0, device=dev),
+                         torch.as_tensor(mass, device=dev), dt, n_v=m.n_v,
+                         fixed=torch.as_tensor(fixed))
+        worst = 0.0
+        for s in range(100):
+            xn, xd, xdd = (a.cpu().numpy().copy() for a in (st.xn, st.xd, st.xdd))
+            r0 = _oracle_residual(m, p, st.x.cpu().numpy(), xn, xd, xdd, x0, fixed, rho, dt)
+            info = st.step()
+            assert info.converged, (s, info)
+            r = _oracle_residual(m, p, st.x.cpu().numpy(), xn, xd, xdd, x0, fixed, rho, dt)
+            ratio = np.linalg.norm(r) / np.linalg.norm(r0)
+            worst = max(worst, ratio)
+            assert ratio <= 1e-6, (s, ratio)
+            if s % 20 == 19:
+                print(f"  step {s + 1}: {info.iterations} Newton / {info.linear_iterations} "
+                      f"GMRES, oracle residual ratio {ratio:.2e} (worst {worst:.2e})",
+                      flush=True)
+        U = st.x.cpu().numpy()
+        assert np.abs(U[fixed] - x0[fixed]).max() <= 1e-10
+        assert np.isfinite(U).all() and np.abs(U - x0).max() > 1e-6
+    finally:
+        fe.close()

@@ -117,7 +117,31 @@ def test_membrane_handover_sizes(sigma):
     check_all(m, p, x, y, z, omp=16)
 
 
-@pytest.mark.parametrize("mode", ["isotropic", "manufactured", "ramp3_delay", "ramp1"])
+@pytest.mark.parametrize("mesh", ["membrane", "ellipsoid"])
+@pytest.mark.parametrize("n_qp", [1, 2, 4, 5])
+def test_quadrature_rules(n_qp, mesh):
+    """SetGaussianQuadrature's rules other than the 3-point one (active_strain.c:184-222):
+    the device's table and the lane groups of n_qp lanes (64 / n_qp elements per wave)
+    against the oracle, on a displaced flat membrane and on the irregular patches of the
+    small ellipsoid."""
+    # On the shell the reference order's own rounding is ~1e-12 of the force (the staged
+    # chain is within 1e-13 of an extended-precision evaluation, the oracle 1.4e-12 away:
+    # test_staged_cpu.py::test_reference_rounding_on_ellipsoid), so the staged FP64 path's
+    # stated tolerance there is 5e-12.
+    if mesh == "membrane":
+        m, p, _ = configs.membrane_small()
+        x, y, z = configs.smooth_displacement(m, 0.01)
+        tol_fast = 1e-12
+    else:
+        m, p, (x, y, z) = configs.ellipsoid_shell(levels=3)
+        tol_fast = 5e-12
+    p.n_qp = n_qp
+    print(f"case {mesh} n_qp {n_qp}", flush=True)
+    check_all(m, p, x, y, z, tol_fast=tol_fast)
+
+
+@pytest.mark.parametrize("mode", ["isotropic", "manufactured", "ramp3_delay", "ramp1",
+                                  "ramp2", "ramp2_delay"])
 def test_activation_variants(mode):
     m, p, _ = configs.membrane_small()
     x, y, z = configs.smooth_displacement(m, 0.01)
@@ -131,8 +155,14 @@ def test_activation_variants(mode):
         p.ramp_type, p.ti, p.dt = 3, 20, 0.01
         m.activation_delay = np.linspace(0.0, 0.1, m.n_elmt)
         tol_exact = 1e-13  # device exp() against glibc exp()
-    else:
+    elif mode == "ramp1":
         p.ramp_type, p.ti, p.dt, p.tisteps = 1, 25, 0.01, 100
+    elif mode == "ramp2":   # half-sine ramp, active_strain.c:1175-1178
+        p.ramp_type, p.ti, p.dt, p.tisteps = 2, 40, 0.01, 100
+    else:                   # the same with per-element delays, evaluated on the device
+        p.ramp_type, p.ti, p.dt, p.tisteps = 2, 40, 0.01, 100
+        m.activation_delay = np.linspace(0.0, 0.2, m.n_elmt)
+        tol_exact = 1e-13  # device sin() against glibc sin()
     print("case", mode, flush=True)
     check_all(m, p, x, y, z, tol_exact=tol_exact, tol_fast=1e-12)
 
@@ -250,9 +280,9 @@ def test_status_after_eval_on_caller_stream():
     ref, st, c33 = O.finternal_act(m, p, x, y, z, want_c33=True)
     fe = ActStrainFE(m, p)
     try:
-        # a finite evaluation first, on the context's stream: its status has no
-        # non-finite element
-        xs, ys, zs = configs.membrane_large(128, 96)[2]
+        # a finite evaluation first, on the context's stream (the same mesh flattened:
+        # finite forces, one CC3333 trip): its status has no non-finite element
+        xs, ys, zs = x, y, np.zeros_like(z)
         fe.finternal_act(xs, ys, zs, precision=L.BF_PREC_FP64)
         assert fe.last_error()[1] != L.BF_ELEM_NONFINITE
         dev = torch.device("cuda:0")

@@ -183,9 +183,11 @@ def run_served(args, L, log):
     rep = srv.run(specs)
     out = rep.summary()
     out.update(workload=f"{len(specs)} cases, Poisson 20/s, flat or ellipsoid 2k-400k triangles "
-                        "(log-uniform, snapped to the cached mesh sizes), 50-500 explicit steps, "
-                        "FP32-mixed, continuous batching, 10 steps per round; mesh cache and "
-                        "kernels warmed before the first arrival (warmup_s)",
+                        "(log-uniform; flat nx on a 2^(1/4) grid, ellipsoid subdivision levels "
+                        "3-6), 50-500 explicit steps, FP32-mixed, continuous batching, 10 steps "
+                        "per round; mesh cache and kernels warmed before the first arrival "
+                        "(warmup_s); cases whose force goes non-finite are retired and counted "
+                        "(failed, failed_cases) with their latency inside latency_p50/p99",
                hbm_budget_gib=args.serve_budget_gib)
     log(f"served sweep: {out}")
     return out
@@ -381,8 +383,8 @@ def main():
     ap.add_argument("--modes", default="fp64,fp64_ref_order,mixed")
     ap.add_argument("--batch", type=int, default=1, help="also run the batched 64-membrane workload")
     ap.add_argument("--batch-n", type=int, default=64)
-    ap.add_argument("--serve-cases", type=int, default=400,
-                    help="cases of the served sweep (2000 = the full sweep, ~100 s of arrivals)")
+    ap.add_argument("--serve-cases", type=int, default=2000,
+                    help="cases of the served sweep (2000, the full sweep: ~100 s of arrivals)")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--implicit-steps", type=int, default=100,
                     help="implicit Newmark steps of the 32x24 membrane (0: skip)")

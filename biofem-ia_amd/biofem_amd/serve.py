@@ -25,12 +25,10 @@ from .batch import Batch, upload_coords
 from .fem import ActParams, ActStrainFE
 from .mesh import Mesh, ellipsoid_dome, flat_membrane, lumped_mass
 
-# Ellipsoid subdivision levels a case may use.  Level 6 (184 320 triangles) is left out:
-# at that resolution the valence-5 apex patches put elements into ModElemC33's clamp regime
-# already in the reference state (the reference's a3,alpha quirk), the explicit step then
-# diverges and the case degenerates into NaN, every element handed to the reference-order
-# kernel.  Larger requests are served at level 5.
-ELLIPSOID_SIZES = {3: 2880, 4: 11520, 5: 46080}
+# Ellipsoid subdivision levels a case may use: a request is served at the level nearest
+# its triangle count (log scale).  Level 7 (737 280 triangles) lies above the sweep's
+# largest request (400 000), so the largest served ellipsoid is level 6.
+ELLIPSOID_SIZES = {3: 2880, 4: 11520, 5: 46080, 6: 184320}
 
 
 @dataclass
@@ -161,22 +159,37 @@ class SweepReport:
     warmup: float = 0.0    # seconds of Server.warm before the first arrival
 
     def summary(self) -> dict:
+        """Latency percentiles over every case that arrived: a case retired before its last
+        step (its forces went non-finite) counts with the time from its arrival to its
+        retirement, so failures stay inside p50/p99; the completed-only percentiles and
+        the retired cases with their reasons are reported beside them."""
+        def pct(a, q):
+            return float(np.percentile(a, q)) if len(a) else None
         ok = [c for c in self.cases if c.failed is None]
-        lat = np.array([c.finished - c.spec.arrival for c in ok])
+        lat_all = np.array([c.finished - c.spec.arrival for c in self.cases])
+        lat_ok = np.array([c.finished - c.spec.arrival for c in ok])
         wait = np.array([c.admitted - c.spec.arrival for c in self.cases])
         work = sum(c.mesh.n_elmt * (c.spec.steps - max(c.remaining, 0)) for c in self.cases)
+        failed = [dict(cid=c.spec.cid, kind=c.spec.kind, triangles=c.mesh.n_elmt,
+                       h0=round(c.spec.h0, 5), gamma=round(c.spec.gamma, 4),
+                       steps=c.spec.steps, reason=c.failed)
+                  for c in self.cases if c.failed is not None]
+        big_ell = [c.mesh.n_elmt for c in self.cases if c.spec.kind == "ellipsoid"]
+        big_flat = [c.mesh.n_elmt for c in self.cases if c.spec.kind == "flat"]
         return dict(
-            cases=len(self.cases), failed=len(self.cases) - len(ok), wall_s=self.wall,
+            cases=len(self.cases), completed=len(ok), failed=len(failed), wall_s=self.wall,
             throughput_cases_per_s=len(self.cases) / self.wall if self.wall else 0.0,
             elem_steps_per_s=work / self.wall if self.wall else 0.0,
             elem_qp_per_s_gpu=3 * work / (self.gpu_ms * 1e-3) if self.gpu_ms else 0.0,
-            latency_p50_s=float(np.percentile(lat, 50)) if len(lat) else None,
-            latency_p99_s=float(np.percentile(lat, 99)) if len(lat) else None,
-            latency_max_s=float(lat.max()) if len(lat) else None,
-            queue_wait_p50_s=float(np.percentile(wait, 50)) if len(wait) else None,
-            queue_wait_p99_s=float(np.percentile(wait, 99)) if len(wait) else None,
+            latency_p50_s=pct(lat_all, 50), latency_p99_s=pct(lat_all, 99),
+            latency_max_s=float(lat_all.max()) if len(lat_all) else None,
+            latency_completed_p50_s=pct(lat_ok, 50), latency_completed_p99_s=pct(lat_ok, 99),
+            queue_wait_p50_s=pct(wait, 50), queue_wait_p99_s=pct(wait, 99),
             rounds=self.rounds, gpu_busy_s=self.gpu_ms * 1e-3, max_active=self.max_active,
-            peak_footprint_gib=self.peak_footprint / 2**30, warmup_s=self.warmup)
+            peak_footprint_gib=self.peak_footprint / 2**30, warmup_s=self.warmup,
+            largest_ellipsoid=max(big_ell) if big_ell else 0,
+            largest_flat=max(big_flat) if big_flat else 0,
+            failed_cases=failed)
 
 
 class Server:

@@ -89,6 +89,7 @@ SIGNATURES = {
     "bf_params_set": (C.c_int, [_vp, C.POINTER(BfParams)]),
     "bf_force_eval": (C.c_int, [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_int]),
     "bf_force_eval_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "bf_force_eval_device_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "bf_ghost_update": (C.c_int, [_vp, _dp, _dp, _dp]),
     "bf_ghost_update_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "bf_c33_download": (C.c_int, [_vp, _dp]),

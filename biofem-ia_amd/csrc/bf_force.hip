@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -29,6 +30,15 @@
 #include "bf_kernels.h"
 
 using namespace bf;
+
+// Scratch of bf_force_eval_device_noalloc: the per-evaluation arrays of one force evaluation
+// (element-force buffer, hand-over segments and list, per-qp stresses and status, redo
+// list) carved out of one file-scope device array instead of the context's allocations.
+// Each GPU the library runs on holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in: its device address comes from hipGetSymbolAddress,
+// once per device (noalloc_scratch_init), and only that address is used.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 // ------------------------------------------------------------------ kernels ------------
 // Staged element body for lane q of the P-lane group that holds element e (P >= n_qp):
@@ -606,6 +616,48 @@ int harvest_ring(bf_ctx* c) {
   return BF_OK;
 }
 
+// Device address of `scratch` per device (hipGetSymbolAddress with that device current),
+// filled once per device by the first context created on it.
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return BF_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return BF_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    fprintf(stderr, "biofem_hip: hipGetSymbolAddress(scratch) failed on device %d: %s\n", device,
+            hipGetErrorString(e));
+    return BF_ERR_DEVICE;
+  }
+  g_scratch_ptr[device] = ptr;
+  return BF_OK;
+}
+
+// Byte offsets of the per-evaluation arrays of a mesh (E elements, NS patch slots) in the
+// static scratch, 256-byte aligned; returns the total.
+struct ScratchLayout {
+  size_t Fe, fallback, wlist, wcount, qsv, qst, redo, total;
+};
+ScratchLayout noalloc_layout(size_t E, size_t NS) {
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  ScratchLayout L{};
+  const size_t En = std::max<size_t>(E, 1);
+  size_t o = 0;
+  L.Fe = o;       o += up(sizeof(double) * 3 * NS * En);
+  L.fallback = o; o += up(sizeof(int) * En);
+  L.wlist = o;    o += up(sizeof(int) * (En + 64));
+  L.wcount = o;   o += up(sizeof(int) * En);
+  L.qsv = o;      o += up(sizeof(double) * 3 * kMaxQp * En);
+  L.qst = o;      o += up(sizeof(int) * kMaxQp * En);
+  L.redo = o;     o += up(sizeof(int) * En);
+  L.total = o;
+  return L;
+}
+
 int kernel_launch_check() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -637,6 +689,7 @@ int bf_ctx_create(int device, bf_ctx** out) {
       c->n_cu = prop.multiProcessorCount;
   }
   int rc = upload_tables(device);
+  if (rc == BF_OK) rc = noalloc_scratch_init(device);
   if (rc != BF_OK) { delete c; return rc; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->own_ev[0]) != hipSuccess || hipEventCreate(&c->own_ev[1]) != hipSuccess ||
@@ -929,14 +982,16 @@ static KParams dev_params(const bf_ctx* c) {
   return kp;
 }
 
+// One force evaluation on stream s.  `view` (optional) replaces the context's own arrays:
+// bf_force_eval_device_noalloc points the per-evaluation scratch into the static array.
 static int launch_eval(bf_ctx* c, const double* x, const double* y, const double* z,
                        double* fint, int accumulate, int precision, hipStream_t s,
-                       bool step = false) {
+                       bool step = false, const DevMesh* view = nullptr) {
   if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
   if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
     return BF_ERR_ARG;
   const KParams kp = dev_params(c);
-  const DevMesh dm = dev_mesh(c);
+  const DevMesh dm = view ? *view : dev_mesh(c);
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
   hipEvent_t* ev = c->own_ev;
   if (c->ring_on) {
@@ -970,20 +1025,20 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
       hipLaunchKernelGGL(bf_tier2_kernel, dim3((c->E + per_block - 1) / per_block),
                          dim3(kStagedBlock), 0, s, dm, kp, x, y, z);
     }
-    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, c->d_wcount,
-                       c->d_wlist, n_waves, per_wave, c->d_fallback, c->d_status);
+    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, dm.wcount,
+                       dm.wlist, n_waves, per_wave, dm.fallback, dm.status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
                                          s));
     c->fb_prec = precision;
   }
   BF_HIP(hipEventRecord(e2, s));
   if (step)
-    BF_HIP(launch_gather_step(c->d_ptr, c->d_ent, c->d_Fe, c->E, c->n_v, c->n_nodes, c->d_fint,
+    BF_HIP(launch_gather_step(c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, c->d_fint,
                               c->d_x, c->d_y, c->d_z, c->d_v, c->d_winv, c->d_fixed, c->dt,
                               c->damp, s));
   else if (c->n_nodes > 0)
     hipLaunchKernelGGL(bf_gather_kernel, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s,
-                       c->d_ptr, c->d_ent, c->d_Fe, c->E, c->n_v, c->n_nodes, accumulate, fint);
+                       c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, accumulate, fint);
   BF_HIP(hipEventRecord(e3, s));
   c->timing = true;
   return kernel_launch_check();
@@ -1038,6 +1093,33 @@ int bf_force_eval_device(bf_ctx* c, const double* d_x, const double* d_y, const 
   BF_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   return launch_eval(c, d_x, d_y, d_z, d_fint, accumulate, precision, s);
+}
+
+int bf_force_eval_device_noalloc(bf_ctx* c, const double* d_x, const double* d_y,
+                                 const double* d_z, double* d_fint, int accumulate,
+                                 int precision, void* stream) {
+  if (!c || !d_x || !d_y || !d_z || !d_fint) return BF_ERR_ARG;
+  if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
+  if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device]) return BF_ERR_STATE;
+  const ScratchLayout L = noalloc_layout((size_t)c->E, (size_t)c->NS);
+  if (L.total > kNoAllocScratchBytes) {
+    fprintf(stderr, "biofem_hip: bf_force_eval_device_noalloc: the mesh needs %zu bytes of "
+            "scratch (%d elements, %d patch slots), above the static maximum of %zu\n", L.total,
+            c->E, c->NS, kNoAllocScratchBytes);
+    return BF_ERR_UNSUPPORTED;
+  }
+  BF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  unsigned char* base = (unsigned char*)g_scratch_ptr[c->device];
+  DevMesh dm = dev_mesh(c);
+  dm.Fe = (double*)(base + L.Fe);
+  dm.fallback = (int*)(base + L.fallback);
+  dm.wlist = (int*)(base + L.wlist);
+  dm.wcount = (int*)(base + L.wcount);
+  dm.qsv = (double*)(base + L.qsv);
+  dm.qst = (int*)(base + L.qst);
+  dm.redo = (int*)(base + L.redo);
+  return launch_eval(c, d_x, d_y, d_z, d_fint, accumulate, precision, s, false, &dm);
 }
 
 int bf_ghost_update_device(bf_ctx* c, double* d_x, double* d_y, double* d_z, void* stream) {

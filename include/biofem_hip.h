@@ -126,6 +126,19 @@ int bf_force_eval(bf_ctx *ctx, const double *x, const double *y, const double *z
 int bf_force_eval_device(bf_ctx *ctx, const double *d_x, const double *d_y, const double *d_z,
                          double *d_fint, int accumulate, int precision, void *stream);
 
+/* bf_force_eval_device without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * reference counterpart.  The per-evaluation scratch (element-force buffer, hand-over lists,
+ * per-qp stresses) lives in one static device array of 64 MiB per GPU, held until the
+ * process ends, instead of the context's buffers; meshes needing more (about 150 000
+ * elements with 12-slot patches) get BF_ERR_UNSUPPORTED.  The context still owns the mesh,
+ * the C33 state, the flags and the status.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (different streams, threads or contexts) race.  Same results as
+ * bf_force_eval_device. */
+int bf_force_eval_device_noalloc(bf_ctx *ctx, const double *d_x, const double *d_y,
+                                 const double *d_z, double *d_fint, int accumulate,
+                                 int precision, void *stream);
+
 /* GlobalGhost (src/bending.c:1763-1814) as an O(n_ghosts) gather:
  * x[n_v + g] = x[a] + x[b] - x[c] for every ghost rule.  Host and device forms. */
 int bf_ghost_update(bf_ctx *ctx, double *x, double *y, double *z);

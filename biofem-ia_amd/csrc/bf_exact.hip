@@ -745,6 +745,146 @@ hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---- node pass of the single-pass staged path ----
+// Final force of node n: a partial node sums its tiles' partials in tile order (after the
+// caller's value when accumulating); a complete node keeps what its tile wrote; a node no
+// element touches keeps the caller's value or zero.  A node of a handed-over element then
+// adds, in ascending element order, the forces of its handed-over elements from the
+// element-force buffer (written by the reference-order finish or the FP64 second tier), and
+// its touched mark is cleared for the next evaluation.  Ghost rows are zeroed when not
+// accumulating.  Returns whether f holds the force (always for real nodes when need_f).
+__device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __restrict__ ptr,
+                                                const unsigned* __restrict__ ent,
+                                                const double* __restrict__ Fe, int E, int n_v,
+                                                int accumulate, double* __restrict__ fint,
+                                                int n, bool need_f, double (&f)[3]) {
+  double* fo = fint + 3 * (size_t)n;
+  if (n >= n_v) {
+    if (!accumulate) fo[0] = fo[1] = fo[2] = 0.0;
+    return false;
+  }
+  const int p0 = td.pptr[n], p1 = td.pptr[n + 1];
+  const bool tch = td.touched[n] != 0;
+  const bool comp = td.complete[n] != 0;
+  if (p0 == p1 && comp && !tch && !need_f) return false;  // its tile wrote it
+  bool write = false;
+  if (p0 < p1) {
+    f[0] = accumulate ? fo[0] : 0.0;
+    f[1] = accumulate ? fo[1] : 0.0;
+    f[2] = accumulate ? fo[2] : 0.0;
+    for (int j = p0; j < p1; j++) {
+      const double* pr = td.partial + 3 * (size_t)td.plist[j];
+      f[0] += pr[0];
+      f[1] += pr[1];
+      f[2] += pr[2];
+    }
+    write = true;
+  } else if (comp || accumulate) {
+    f[0] = fo[0];
+    f[1] = fo[1];
+    f[2] = fo[2];
+  } else {
+    f[0] = f[1] = f[2] = 0.0;
+    write = true;
+  }
+  if (tch) {
+    td.touched[n] = 0;
+    for (int j = ptr[n]; j < ptr[n + 1]; j++) {
+      const unsigned v = ent[j];
+      const int e = (int)(v >> 5);
+      if (!td.deferred[e]) continue;
+      const double* fe = Fe + 3 * ((size_t)(v & 31u) * E + e);
+      f[0] += fe[0];
+      f[1] += fe[1];
+      f[2] += fe[2];
+    }
+    write = true;
+  }
+  if (write) {
+    fo[0] = f[0];
+    fo[1] = f[1];
+    fo[2] = f[2];
+  }
+  return true;
+}
+
+// Explicit damped update of node n from its force f (as step_node).
+__device__ __forceinline__ void step_update(int n, const double (&f)[3], double* x, double* y,
+                                            double* z, double* v, const double* winv,
+                                            const unsigned char* fixed, double dt, double damp) {
+  if (fixed[n]) return;
+  const double w = winv[n];
+  double* vn = v + 3 * (size_t)n;
+  const double vx = damp * vn[0] - w * f[0];
+  const double vy = damp * vn[1] - w * f[1];
+  const double vz = damp * vn[2] - w * f[2];
+  vn[0] = vx;
+  vn[1] = vy;
+  vn[2] = vz;
+  x[n] = x[n] + dt * vx;
+  y[n] = y[n] + dt * vy;
+  z[n] = z[n] + dt * vz;
+}
+
+__global__ void __launch_bounds__(256) bf_tile_fixup_kernel(TileDev td, const int* __restrict__ ptr,
+                                                            const unsigned* __restrict__ ent,
+                                                            const double* __restrict__ Fe, int E,
+                                                            int n_v, int n_nodes, int accumulate,
+                                                            double* __restrict__ fint) {
+  const int n = logical_block() * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  double f[3];
+  tile_node_final(td, ptr, ent, Fe, E, n_v, accumulate, fint, n, false, f);
+}
+
+__global__ void __launch_bounds__(256) bf_tile_fixup_step_kernel(
+    TileDev td, const int* __restrict__ ptr, const unsigned* __restrict__ ent,
+    const double* __restrict__ Fe, int E, int n_v, int n_nodes, double* __restrict__ fint,
+    double* x, double* y, double* z, double* v, const double* winv, const unsigned char* fixed,
+    double dt, double damp) {
+  const int n = logical_block() * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  double f[3];
+  if (tile_node_final(td, ptr, ent, Fe, E, n_v, 0, fint, n, true, f))
+    step_update(n, f, x, y, z, v, winv, fixed, dt, damp);
+}
+
+__global__ void __launch_bounds__(256) bf_batch_tile_fixup_kernel(BatchDev b, int step) {
+  const Tile tl = b.ntiles[logical_block()];
+  const CaseDev& cd = b.cases[tl.c];
+  const int i = threadIdx.x;
+  if (i >= tl.n) return;
+  const int n = tl.b + i;
+  double f[3];
+  const bool adv = step && cd.v;
+  if (tile_node_final(cd.td, cd.ptr, cd.ent, cd.m.Fe, cd.kp.E, cd.n_v, 0, cd.fint, n, adv, f) &&
+      adv)
+    step_update(n, f, cd.x, cd.y, cd.z, cd.v, cd.winv, cd.fixed, cd.dt, cd.damp);
+}
+
+hipError_t launch_tile_fixup(const TileDev& td, const int* ptr, const unsigned* ent,
+                             const double* Fe, int E, int n_v, int n_nodes, int accumulate,
+                             double* fint, bool step, double* x, double* y, double* z, double* v,
+                             const double* winv, const unsigned char* fixed, double dt,
+                             double damp, hipStream_t s) {
+  if (n_nodes <= 0) return hipSuccess;
+  const dim3 grid((n_nodes + 255) / 256), block(256);
+  if (step)
+    hipLaunchKernelGGL(bf_tile_fixup_step_kernel, grid, block, 0, s, td, ptr, ent, Fe, E, n_v,
+                       n_nodes, fint, x, y, z, v, winv, fixed, dt, damp);
+  else
+    hipLaunchKernelGGL(bf_tile_fixup_kernel, grid, block, 0, s, td, ptr, ent, Fe, E, n_v, n_nodes,
+                       accumulate, fint);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_tile_fixup(const BatchDev& b, bool step, hipStream_t s) {
+  if (b.n_ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bf_batch_tile_fixup_kernel, dim3(b.n_ntiles), dim3(256), 0, s, b,
+                     step ? 1 : 0);
+  return hipGetLastError();
+}
+
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_IN), tab, bytes);
 }

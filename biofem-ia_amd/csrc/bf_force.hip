@@ -28,6 +28,7 @@
 #include "../../include/biofem_hip.h"
 #include "bf_elem.h"
 #include "bf_kernels.h"
+#include "bf_tiles.h"
 
 using namespace bf;
 
@@ -197,44 +198,236 @@ __device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& 
 
 constexpr int kStagedBlock = 256;
 
+// ---- single-pass staged element phase ----
+// One tile per workgroup, one element per lane (tiles of td.T elements, bf_tiles.h):
+//   1. the coordinates of the tile's local nodes are staged in LDS;
+//   2. each lane forms its element's current midsurface geometry once (FP64, patch
+//      coordinates from LDS), runs the staged chain of every through-thickness point in R,
+//      sums the resultants in qp order and decides the hand-over as the reference's
+//      conditioning requires (needs_exact_bits);
+//   3. slot by slot the lanes add their nodal forces into LDS node accumulators, in the
+//      phases the tiling assigned (no two lanes of a phase share a node; a barrier between
+//      phases), so every node sum has a fixed order;
+//   4. complete nodes are written to fint, partial nodes to their tile partial.
+// A handed-over element adds nothing here: it is listed in its wave's segment, marks its
+// nodes as touched and sets deferred; the node pass adds its force once it is known.
+// smem: 6 * td.lmax doubles (coordinates and accumulators, SoA).
 template <class R>
-__global__ void __launch_bounds__(kStagedBlock) bf_element_kernel(DevMesh m, KParams kp,
-                                                                  const double* __restrict__ x,
-                                                                  const double* __restrict__ y,
-                                                                  const double* __restrict__ z) {
-  const int P = kp.n_qp, per_wave = 64 / P;
-  int slot, q, base;
-  lane_group(P, threadIdx.x & 63, slot, q, base);
-  const int wave = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
-  const int e = wave * per_wave + slot;
-  if (slot >= per_wave || e >= kp.E) return;
-  const bool take = staged_element<R>(m, kp, x, y, z, e, q, base);
-  WaveSink{m.wlist, m.wcount, per_wave, wave}.push(take, e);
+__device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, const TileDev& td,
+                                          int t, const double* __restrict__ x,
+                                          const double* __restrict__ y,
+                                          const double* __restrict__ z,
+                                          double* __restrict__ fint, int accumulate, int n_v,
+                                          double* smem) {
+  const int T = td.T, NS = td.NS, lm = td.lmax;
+  const int lane = (int)threadIdx.x;
+  double *sx = smem, *sy = smem + lm, *sz = smem + 2 * lm;
+  double *fx = smem + 3 * lm, *fy = smem + 4 * lm, *fz = smem + 5 * lm;
+  const int j0 = td.node_ptr[t], L = td.node_ptr[t + 1] - j0;
+  __syncthreads();  // the previous tile of a persistent workgroup is done with the LDS
+  for (int l = lane; l < L; l += T) {
+    const int n = td.node[j0 + l];
+    sx[l] = x[n];
+    sy[l] = y[n];
+    sz[l] = z[n];
+    fx[l] = 0.0;
+    fy[l] = 0.0;
+    fz[l] = 0.0;
+  }
+  __syncthreads();
+  const int E = kp.E;
+  const int e = td.elem[(size_t)t * T + lane];
+  const unsigned short* ls = td.lslot + (size_t)t * NS * T + lane;  // slot a: ls[a * T]
+  const bool live = e >= 0;
+  MidGeom G;
+  const double *na0 = nullptr, *na1 = nullptr, *nb0 = nullptr, *nb1 = nullptr, *nb2 = nullptr;
+  int kind = kRegular;
+  bool handed = false;
+  Resultants<R> r;
+  r.N0 = r.N1 = r.N2 = r.M0 = r.M1 = r.M2 = 0.0;
+  double pref0 = 0.0;
+  int csl[3] = {-1, -1, -1};
+  V3<double> fc[3];
+  if (live) {
+    const int meta = m.meta[e];
+    double rv[15];
+#pragma unroll
+    for (int k = 0; k < 15; k++) rv[k] = m.ref[(size_t)k * E + e];
+    const double f0 = m.fib[e], f1 = m.fib[(size_t)E + e], f2 = m.fib[(size_t)2 * E + e];
+    const double gs = m.gscale ? m.gscale[e] : 1.0;
+    const double dly = kp.has_delay ? m.delay[e] : 0.0;
+    const double A0 = m.dA0[e];
+    kind = meta & 3;
+    const int val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
+    auto xyz = [&](int i) {
+      const int l = ls[(size_t)i * T] & 0xFFF;
+      return mk(sx[l], sy[l], sz[l]);
+    };
+    if (kind == kRegular) {
+      na0 = c_Na[0]; na1 = c_Na[1]; nb0 = c_Nab[0]; nb1 = c_Nab[1]; nb2 = c_Nab[2];
+      G = mid_geom_regular(xyz);
+    } else if (kind == kIrregular) {
+      const double(*tb)[kMaxNen] = c_IN[val];
+      na0 = tb[0]; na1 = tb[1]; nb0 = tb[2]; nb1 = tb[3]; nb2 = tb[4];
+      G = mid_geom(nen, na0, na1, nb0, nb1, nb2, xyz);
+    } else {
+      for (int k = 0; k < 3; k++) csl[k] = m.corner[(size_t)(3 + k) * E + e];
+      G = cap_geom(xyz(csl[0]), xyz(csl[1]), xyz(csl[2]));
+    }
+    V3<double> a31c, a32c;
+    a3_alpha(G.a1, G.a2, G.Aaa, G.Aab, G.Abb, a31c, a32c);
+    RefGeom rg;
+    rg.a1 = mk(rv[0], rv[1], rv[2]);
+    rg.a2 = mk(rv[3], rv[4], rv[5]);
+    rg.a3 = mk(rv[6], rv[7], rv[8]);
+    rg.a31 = mk(rv[9], rv[10], rv[11]);
+    rg.a32 = mk(rv[12], rv[13], rv[14]);
+    double g = kp.has_delay ? ramp_gamma(kp.ramp, dly) : kp.gamma_t;
+    g *= gs;
+    Activation act;
+    act.f = mk(f0, f1, f2);
+    if (kp.act_mode == 0) {
+      act.d0 = act.d1 = act.d2 = 1.0;
+      act.gam = g;
+    } else if (kp.act_mode == 1) {
+      act.d0 = 1.0 - kp.man_gamma;
+      act.d1 = act.d2 = 1.0;
+      act.gam = 0.0;
+    } else {
+      act.d0 = act.d1 = 1.0 - g;
+      act.d2 = 1.0 / (act.d0 * act.d1);
+      act.gam = 0.0;
+    }
+    auto own = [](bool fail, int) { return fail; };
+    int st = 0;
+    for (int q = 0; q < kp.n_qp; q++) {
+      const double th = kp.theta[q];
+      StagedQp<R> sq;
+      if constexpr (sizeof(R) == 4)
+        sq = staged_qp_mixed(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K,
+                             kp.iters, th, own, kp.fallback != 0);
+      else
+        sq = staged_qp<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters,
+                          th, own, kp.fallback != 0, kC33NoiseMax);
+      add_resultants(r, sq, kp.w[q], th);
+      st |= sq.status | c33_range_bit(sq.c33, sizeof(R) == 4);
+      // handed-over elements get their C33 from the reference-order solve later
+      m.c33[(size_t)q * E + e] = sq.c33;
+    }
+    const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) &&
+                        isfinite(r.M0) && isfinite(r.M1) && isfinite(r.M2);
+    handed = kp.fallback && needs_exact_bits(st, finite);
+    td.deferred[e] = handed ? 1 : 0;
+    pref0 = 0.5 * kp.h0 * A0;
+    if (handed) {
+      for (int a = 0; a < nen; a++) {
+        const unsigned short code = ls[(size_t)a * T];
+        if (code == kTileIdle) continue;  // a slot that adds no force (LV cap)
+        const int n = td.node[j0 + (code & 0xFFF)];
+        if (n < n_v) td.touched[n] = 1;
+      }
+    } else {
+      if (!finite) atomicMin(&m.status[0], e);
+      m.flags[e] = (unsigned char)((st & (kStatCC3333 | kStatClamp)) | (finite ? 0 : kStatNonFinite));
+      if (st & kStatCC3333) {
+        atomicMin(&m.status[2], e);
+        atomicAdd(&m.status[3], 1);
+      }
+      if (kind == kCap) {  // CST membrane into the corner slots (:1807-1851)
+        const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+        for (int k = 0; k < 3; k++)
+          fc[k] = node_force(dN[k][0], dN[k][1], 0.0, 0.0, 0.0, G, pref0, 0.5 * kp.h0, r.N0,
+                             r.N1, r.N2, 0.0, 0.0, 0.0);
+      }
+    }
+  }
+  WaveSink{td.wlist, td.wcount, 64, t * (T >> 6) + (lane >> 6)}.push(live && handed, e);
+  // ---- nodal forces into the LDS accumulators, slot by slot, phase by phase ----
+  const double hh = 0.5 * kp.h0;
+  for (int a = 0; a < NS; a++) {
+    const unsigned short code = ls[(size_t)a * T];
+    const bool contributes = live && !handed && code != kTileIdle;
+    V3<double> f = mk(0.0, 0.0, 0.0);
+    if (contributes) {
+      if (kind != kCap) {
+        f = node_force(na0[a], na1[a], nb0[a], nb1[a], nb2[a], G, pref0, hh, r.N0, r.N1, r.N2,
+                       r.M0, r.M1, r.M2);
+      } else {
+        for (int k = 0; k < 3; k++)
+          if (csl[k] == a) f = add(f, fc[k]);
+      }
+    }
+    const int nph = td.nphase[(size_t)t * NS + a];
+    const int ph_mine = code >> 12, l = code & 0xFFF;
+    for (int ph = 0; ph < nph; ph++) {
+      if (contributes && ph_mine == ph) {
+        fx[l] += f.x;
+        fy[l] += f.y;
+        fz[l] += f.z;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  for (int l = lane; l < L; l += T) {
+    const int o = td.out[j0 + l];
+    if (o == -1) {
+      double* fo = fint + 3 * (size_t)td.node[j0 + l];
+      const double bx = accumulate ? fo[0] : 0.0, by = accumulate ? fo[1] : 0.0,
+                   bz = accumulate ? fo[2] : 0.0;
+      fo[0] = bx + fx[l];
+      fo[1] = by + fy[l];
+      fo[2] = bz + fz[l];
+    } else if (o >= 0) {
+      double* pr = td.partial + 3 * (size_t)o;
+      pr[0] = fx[l];
+      pr[1] = fy[l];
+      pr[2] = fz[l];
+    }
+  }
 }
 
-// Second tier of the FP32-mixed path: wave w re-evaluates the elements the FP32 kernel's
-// wave w handed over (its segment of m.wlist) with the FP64 staged chain and keeps in the
-// segment only those that chain cannot hold within the mixed path's budget either, for the
-// reference-order kernel: the FP32 chain keeps the well-conditioned elements (rounding
-// bound 1e-11, C33 in [0.4, 2.5]), the FP64 chain the moderately conditioned ones (bound
-// 1e-8, C33 in [0.1, 10]).  Same grid as the element kernel; a wave whose segment is empty
-// leaves at once.
-__global__ void __launch_bounds__(kStagedBlock) bf_tier2_kernel(DevMesh m, KParams kp,
-                                                                const double* __restrict__ x,
-                                                                const double* __restrict__ y,
-                                                                const double* __restrict__ z) {
+template <class R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+bf_tile_kernel(DevMesh m, KParams kp, TileDev td, const double* __restrict__ x,
+               const double* __restrict__ y, const double* __restrict__ z,
+               double* __restrict__ fint, int accumulate, int n_v) {
+  extern __shared__ double tile_smem[];
+  const int t = logical_block();
+  if (t >= td.n_tiles) return;
+  tile_body<R>(m, kp, td, t, x, y, z, fint, accumulate, n_v, tile_smem);
+}
+
+// Second tier of the FP32-mixed path: wave w re-evaluates the elements its tile wave handed
+// over (segment w of td.wlist, up to 64) with the FP64 staged chain, in passes of
+// floor(64 / n_qp) lane groups, and rewrites the segment in place with what that chain
+// cannot hold within the mixed budget either (the FP32 chain keeps the well-conditioned
+// elements: rounding bound 1e-11, C33 in [0.4, 2.5]; the FP64 chain the moderately
+// conditioned ones: bound 1e-8, C33 in [0.1, 10]).  The forces of the elements it keeps go to
+// the element-force buffer (they stay deferred: the node pass adds them).
+__global__ void __launch_bounds__(kStagedBlock) bf_tile_tier2_kernel(DevMesh m, KParams kp,
+                                                                     TileDev td, int n_waves,
+                                                                     const double* __restrict__ x,
+                                                                     const double* __restrict__ y,
+                                                                     const double* __restrict__ z) {
   const int P = kp.n_qp, per_wave = 64 / P;
   int slot, q, base;
   lane_group(P, threadIdx.x & 63, slot, q, base);
-  const int wave = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
-  if ((long long)wave * per_wave >= kp.E) return;
-  const int n = m.wcount[wave];
+  const int w = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
+  if (w >= n_waves) return;
+  const int n = td.wcount[w];
   if (n == 0) return;
-  const bool mine = slot < n;
-  const int e = mine ? m.wlist[(size_t)wave * per_wave + slot] : 0;
-  bool take = false;
-  if (mine) take = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
-  WaveSink{m.wlist, m.wcount, per_wave, wave}.push(take, e);
+  int* seg = td.wlist + (size_t)w * 64;
+  int pushed = 0;
+  for (int k0 = 0; k0 < n; k0 += per_wave) {
+    const int i = k0 + slot;
+    if (slot < per_wave && i < n) {
+      const int e = seg[i];
+      const bool take = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
+      TileSink{seg, &pushed}.push(take, e);
+    }
+  }
+  if ((threadIdx.x & 63) == 0) td.wcount[w] = pushed;
 }
 
 // Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
@@ -364,33 +557,21 @@ __global__ void __launch_bounds__(256) bf_batch_prologue(BatchDev b, int ghosts)
   }
 }
 
-// Staged element phase of a batch: each wave takes element tiles from the device work
-// list (persistent: a fixed stride over the list; otherwise one tile per wave) and runs
-// them in passes of floor(64 / P) element groups.  Every tile belongs to one case, so the
-// case's mesh and options are wave-uniform.
+// Single-pass staged element phase of a batch: each workgroup takes tiles (case, tile) of
+// the work list -- persistent: a fixed stride over the list; otherwise one tile each.
 template <class R>
-__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(2)))
-bf_batch_element_kernel(BatchDev b) {
-  const int P = b.P, per_wave = 64 / P;
-  int slot, q, base;
-  lane_group(P, threadIdx.x & 63, slot, q, base);
-  int t0, t1, step;
-  batch_tile_range(b, t0, t1, step);
-  for (int t = t0; t < t1; t += step) {
-    const Tile tl = b.etiles[t];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+bf_batch_tile_kernel(BatchDev b) {
+  extern __shared__ double tile_smem[];
+  const int n_wg = (int)gridDim.x;
+  const int first = b.persistent ? (int)blockIdx.x : logical_block();
+  const int stride = b.persistent ? n_wg : b.n_ttiles;
+  for (int i = first; i < b.n_ttiles; i += stride) {
+    const Tile tl = b.ttiles[i];
     const int c = __builtin_amdgcn_readfirstlane(tl.c);
-    const int e0 = __builtin_amdgcn_readfirstlane(tl.b);
-    const int n = __builtin_amdgcn_readfirstlane(tl.n);
+    const int t = __builtin_amdgcn_readfirstlane(tl.b);
     const CaseDev& cd = b.cases[c];
-    int pushed = 0;
-    for (int k0 = 0; k0 < n; k0 += per_wave) {
-      const int i = k0 + slot;
-      if (slot < per_wave && i < n) {
-        const bool take = staged_element<R>(cd.m, cd.kp, cd.x, cd.y, cd.z, e0 + i, q, base);
-        TileSink{b.tlist + (size_t)t * b.tile_cap, &pushed}.push(take, e0 + i);
-      }
-    }
-    if ((threadIdx.x & 63) == 0) b.tcount[t] = pushed;
+    tile_body<R>(cd.m, cd.kp, cd.td, t, cd.x, cd.y, cd.z, cd.fint, 0, cd.n_v, tile_smem);
   }
 }
 
@@ -402,9 +583,10 @@ __global__ void __launch_bounds__(kStagedBlock) bf_batch_tier2_kernel(BatchDev b
   const int P = b.P, per_wave = 64 / P;
   int slot, q, base;
   lane_group(P, threadIdx.x & 63, slot, q, base);
-  int t0, t1, step;
-  batch_tile_range(b, t0, t1, step);
-  for (int t = t0; t < t1; t += step) {
+  const int step = (int)gridDim.x * (kStagedBlock / 64);
+  const int t0 = __builtin_amdgcn_readfirstlane(logical_block() * (kStagedBlock / 64) +
+                                                (int)(threadIdx.x >> 6));
+  for (int t = t0; t < b.n_etiles; t += step) {
     const int n = b.tcount[t];
     if (n == 0) continue;
     const Tile tl = b.etiles[t];
@@ -469,6 +651,11 @@ __global__ void __launch_bounds__(256) bf_batch_case_status_kernel(const CaseDev
   const int* st = cases[c].m.status;
   out[2 * c] = st[0];
   out[2 * c + 1] = st[2];
+}
+
+__global__ void __launch_bounds__(256) bf_fill_u8(unsigned char* p, int n, unsigned char v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 __global__ void bf_status_reset(int* st) {
@@ -541,6 +728,27 @@ struct bf_ctx {
   // params
   bf_params p{};
   KParams kp{};
+  // single-pass staged tiling (bf_tiles.h): host copy of what it is built from, the device
+  // arrays and the per-evaluation arrays
+  std::vector<int> h_patch;  // [NS][E] as uploaded (parked slots)
+  std::vector<int> h_nen;    // [E]
+  std::vector<double> h_x0, h_y0, h_z0;
+  int tileT = 0;
+  int n_tiles = 0, n_partials = 0, tile_lmax = 0;
+  int* d_telem = nullptr;
+  unsigned short* d_tlslot = nullptr;
+  unsigned char* d_tnphase = nullptr;
+  int* d_tnode_ptr = nullptr;
+  int* d_tnode = nullptr;
+  int* d_tout = nullptr;
+  int* d_tpptr = nullptr;
+  int* d_tplist = nullptr;
+  unsigned char* d_tcomplete = nullptr;
+  double* d_tpartial = nullptr;
+  unsigned char* d_touched = nullptr;
+  unsigned char* d_deferred = nullptr;
+  int* d_twlist = nullptr;
+  int* d_twcount = nullptr;
 };
 
 namespace {
@@ -570,6 +778,70 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+void free_tiles(bf_ctx* c) {
+  dfree(c->d_telem); dfree(c->d_tlslot); dfree(c->d_tnphase); dfree(c->d_tnode_ptr);
+  dfree(c->d_tnode); dfree(c->d_tout); dfree(c->d_tpptr); dfree(c->d_tplist);
+  dfree(c->d_tcomplete); dfree(c->d_tpartial); dfree(c->d_touched); dfree(c->d_deferred);
+  dfree(c->d_twlist); dfree(c->d_twcount);
+  c->n_tiles = c->n_partials = c->tileT = 0;
+}
+
+// (Re)builds the single-pass tiling of a context's mesh with T elements per tile and
+// uploads it (synchronous on the context stream).
+int build_ctx_tiles(bf_ctx* c, int T) {
+  TileHost th;
+  if (build_tiles(c->E, c->n_v, c->n_nodes, c->NS, c->h_patch.data(), c->h_nen.data(),
+                  c->h_x0.data(), c->h_y0.data(), c->h_z0.data(), T, th) != 0) {
+    fprintf(stderr, "biofem_hip: element tiling needs more than %d phases per slot\n",
+            kTilePhases);
+    return BF_ERR_UNSUPPORTED;
+  }
+  BF_HIP(hipSetDevice(c->device));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  free_tiles(c);
+  const size_t nt = (size_t)std::max(th.n_tiles, 1);
+  const size_t nloc = std::max<size_t>(th.node.size(), 1);
+  int rc;
+#define TALLOC(p, n) \
+  if ((rc = dalloc(&(p), (n))) != BF_OK) { free_tiles(c); return rc; }
+  TALLOC(c->d_telem, nt * T);
+  TALLOC(c->d_tlslot, nt * (size_t)c->NS * T);
+  TALLOC(c->d_tnphase, nt * (size_t)c->NS);
+  TALLOC(c->d_tnode_ptr, nt + 1);
+  TALLOC(c->d_tnode, nloc);
+  TALLOC(c->d_tout, nloc);
+  TALLOC(c->d_tpptr, (size_t)c->n_v + 1);
+  TALLOC(c->d_tplist, (size_t)std::max(th.n_partials, 1));
+  TALLOC(c->d_tcomplete, (size_t)std::max(c->n_v, 1));
+  TALLOC(c->d_tpartial, 3 * (size_t)std::max(th.n_partials, 1));
+  TALLOC(c->d_touched, (size_t)std::max(c->n_nodes, 1));
+  TALLOC(c->d_deferred, (size_t)std::max(c->E, 1));
+  TALLOC(c->d_twlist, nt * T);
+  TALLOC(c->d_twcount, nt * T / 64);
+#undef TALLOC
+  hipStream_t s = c->stream;
+  if (th.n_tiles > 0) {
+    BF_HIP(hipMemcpyAsync(c->d_telem, th.elem.data(), sizeof(int) * th.elem.size(), hipMemcpyHostToDevice, s));
+    BF_HIP(hipMemcpyAsync(c->d_tlslot, th.lslot.data(), sizeof(unsigned short) * th.lslot.size(), hipMemcpyHostToDevice, s));
+    BF_HIP(hipMemcpyAsync(c->d_tnphase, th.nphase.data(), th.nphase.size(), hipMemcpyHostToDevice, s));
+    BF_HIP(hipMemcpyAsync(c->d_tnode, th.node.data(), sizeof(int) * th.node.size(), hipMemcpyHostToDevice, s));
+    BF_HIP(hipMemcpyAsync(c->d_tout, th.out.data(), sizeof(int) * th.out.size(), hipMemcpyHostToDevice, s));
+  }
+  BF_HIP(hipMemcpyAsync(c->d_tnode_ptr, th.node_ptr.data(), sizeof(int) * th.node_ptr.size(), hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_tpptr, th.pptr.data(), sizeof(int) * th.pptr.size(), hipMemcpyHostToDevice, s));
+  if (th.n_partials > 0)
+    BF_HIP(hipMemcpyAsync(c->d_tplist, th.plist.data(), sizeof(int) * th.plist.size(), hipMemcpyHostToDevice, s));
+  if (c->n_v > 0)
+    BF_HIP(hipMemcpyAsync(c->d_tcomplete, th.complete.data(), th.complete.size(), hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemsetAsync(c->d_touched, 0, (size_t)std::max(c->n_nodes, 1), s));
+  BF_HIP(hipStreamSynchronize(s));
+  c->tileT = T;
+  c->n_tiles = th.n_tiles;
+  c->n_partials = th.n_partials;
+  c->tile_lmax = tile_node_cap(T);
+  return BF_OK;
+}
+
 void free_mesh(bf_ctx* c) {
   dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
   dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
@@ -579,6 +851,7 @@ void free_mesh(bf_ctx* c) {
   c->has_state = false;
   dfree(c->d_ptr); dfree(c->d_ent); dfree(c->d_rules); dfree(c->d_x); dfree(c->d_y);
   dfree(c->d_z); dfree(c->d_fint);
+  free_tiles(c);
   c->has_mesh = false;
 }
 
@@ -640,12 +913,14 @@ int noalloc_scratch_init(int device) {
 // Byte offsets of the per-evaluation arrays of a mesh (E elements, NS patch slots) in the
 // static scratch, 256-byte aligned; returns the total.
 struct ScratchLayout {
-  size_t Fe, fallback, wlist, wcount, qsv, qst, redo, total;
+  size_t Fe, fallback, wlist, wcount, qsv, qst, redo, partial, touched, deferred, twlist,
+      twcount, total;
 };
-ScratchLayout noalloc_layout(size_t E, size_t NS) {
+ScratchLayout noalloc_layout(size_t E, size_t NS, size_t n_nodes, size_t n_tiles, size_t T,
+                             size_t n_partials) {
   auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
   ScratchLayout L{};
-  const size_t En = std::max<size_t>(E, 1);
+  const size_t En = std::max<size_t>(E, 1), nt = std::max<size_t>(n_tiles, 1);
   size_t o = 0;
   L.Fe = o;       o += up(sizeof(double) * 3 * NS * En);
   L.fallback = o; o += up(sizeof(int) * En);
@@ -654,6 +929,11 @@ ScratchLayout noalloc_layout(size_t E, size_t NS) {
   L.qsv = o;      o += up(sizeof(double) * 3 * kMaxQp * En);
   L.qst = o;      o += up(sizeof(int) * kMaxQp * En);
   L.redo = o;     o += up(sizeof(int) * En);
+  L.partial = o;  o += up(sizeof(double) * 3 * std::max<size_t>(n_partials, 1));
+  L.touched = o;  o += up(std::max<size_t>(n_nodes, 1));
+  L.deferred = o; o += up(En);
+  L.twlist = o;   o += up(sizeof(int) * nt * T);
+  L.twcount = o;  o += up(sizeof(int) * nt * T / 64);
   L.total = o;
   return L;
 }
@@ -897,6 +1177,23 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
     BF_HIP(launch_geom0(dm, E, c->d_x, c->d_y, c->d_z, c->d_ref, s));
   }
   BF_HIP(hipStreamSynchronize(s));
+  // single-pass staged tiling: an LV apex-cap element adds force to its three corner slots
+  // only, the other slots are left out of the tiles
+  if (cap)
+    for (int e = m->n_elmt_base; e < E; e++)
+      for (int i = 0; i < nenv[e]; i++)
+        if (i != corner[(size_t)3 * E + e] && i != corner[(size_t)4 * E + e] &&
+            i != corner[(size_t)5 * E + e])
+          patch[(size_t)i * E + e] = -1;
+  c->h_patch.swap(patch);
+  c->h_nen.swap(nenv);
+  c->h_x0.assign(m->x0, m->x0 + m->n_nodes);
+  c->h_y0.assign(m->y0, m->y0 + m->n_nodes);
+  c->h_z0.assign(m->z0, m->z0 + m->n_nodes);
+  if ((rc = build_ctx_tiles(c, kTileElemsDefault)) != BF_OK) {
+    free_mesh(c);
+    return rc;
+  }
   c->has_mesh = true;
   return BF_OK;
 }
@@ -982,16 +1279,49 @@ static KParams dev_params(const bf_ctx* c) {
   return kp;
 }
 
-// One force evaluation on stream s.  `view` (optional) replaces the context's own arrays:
-// bf_force_eval_device_noalloc points the per-evaluation scratch into the static array.
+// The single-pass tiling of a context as the kernels see it (its own per-evaluation arrays).
+static TileDev tile_dev(const bf_ctx* c) {
+  TileDev td{};
+  td.n_tiles = c->n_tiles;
+  td.NS = c->NS;
+  td.T = c->tileT;
+  td.lmax = c->tile_lmax;
+  td.elem = c->d_telem;
+  td.lslot = c->d_tlslot;
+  td.nphase = c->d_tnphase;
+  td.node_ptr = c->d_tnode_ptr;
+  td.node = c->d_tnode;
+  td.out = c->d_tout;
+  td.pptr = c->d_tpptr;
+  td.plist = c->d_tplist;
+  td.complete = c->d_tcomplete;
+  td.partial = c->d_tpartial;
+  td.touched = c->d_touched;
+  td.deferred = c->d_deferred;
+  td.wlist = c->d_twlist;
+  td.wcount = c->d_twcount;
+  return td;
+}
+
+// LDS of the single-pass staged kernel for tiles of T elements.
+static size_t tile_smem_bytes(int T) { return 6 * sizeof(double) * (size_t)tile_node_cap(T); }
+
+// One force evaluation on stream s.  `view` / `tview` (optional) replace the context's own
+// arrays: bf_force_eval_device_noalloc points the per-evaluation scratch into the static
+// array.  The plain FP64 path evaluates every element in the reference's operation order and
+// gathers node sums in ascending element order; the staged paths run the single-pass tile
+// kernel, hand ill-conditioned elements to the reference order and finish in the node pass.
 static int launch_eval(bf_ctx* c, const double* x, const double* y, const double* z,
                        double* fint, int accumulate, int precision, hipStream_t s,
-                       bool step = false, const DevMesh* view = nullptr) {
+                       bool step = false, const DevMesh* view = nullptr,
+                       const TileDev* tview = nullptr) {
   if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
   if (precision != BF_PREC_FP64 && precision != BF_PREC_MIXED && precision != BF_PREC_FP64_FAST)
     return BF_ERR_ARG;
   const KParams kp = dev_params(c);
   const DevMesh dm = view ? *view : dev_mesh(c);
+  const TileDev td = tview ? *tview : tile_dev(c);
+  const bool staged = precision != BF_PREC_FP64;
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
   hipEvent_t* ev = c->own_ev;
   if (c->ring_on) {
@@ -1004,41 +1334,48 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   }
   for (int i = 0; i < 4; i++) c->ev[i] = ev[i];  // bf_last_timing reads the latest set
   hipEvent_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
+  double* fout = step ? c->d_fint : fint;
+  const int acc = step ? 0 : accumulate;
   BF_HIP(hipEventRecord(e0, s));
   if (c->E > 0) {
-    const int per_block = (kStagedBlock / 64) * (64 / kp.n_qp);
-    const dim3 grid((c->E + per_block - 1) / per_block), block(kStagedBlock);
-    if (precision == BF_PREC_FP64)
+    if (!staged) {
       BF_HIP(launch_element_exact(dm, kp, x, y, z, c->n_cu, s));
-    else if (precision == BF_PREC_FP64_FAST)
-      hipLaunchKernelGGL(bf_element_kernel<double>, grid, block, 0, s, dm, kp, x, y, z);
-    else
-      hipLaunchKernelGGL(bf_element_kernel<float>, grid, block, 0, s, dm, kp, x, y, z);
+    } else {
+      const size_t smem = tile_smem_bytes(td.T);
+      if (precision == BF_PREC_FP64_FAST)
+        hipLaunchKernelGGL(bf_tile_kernel<double>, dim3(td.n_tiles), dim3(td.T), smem, s, dm, kp,
+                           td, x, y, z, fout, acc, c->n_v);
+      else
+        hipLaunchKernelGGL(bf_tile_kernel<float>, dim3(td.n_tiles), dim3(td.T), smem, s, dm, kp,
+                           td, x, y, z, fout, acc, c->n_v);
+    }
   }
   BF_HIP(hipEventRecord(e1, s));
   c->fb_prec = -1;
-  if (c->E > 0 && precision != BF_PREC_FP64 && kp.fallback) {
-    const int per_wave = 64 / kp.n_qp;
-    const int n_waves = (c->E + per_wave - 1) / per_wave;
-    if (precision == BF_PREC_MIXED) {
-      const int per_block = (kStagedBlock / 64) * per_wave;
-      hipLaunchKernelGGL(bf_tier2_kernel, dim3((c->E + per_block - 1) / per_block),
-                         dim3(kStagedBlock), 0, s, dm, kp, x, y, z);
-    }
-    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, dm.wcount,
-                       dm.wlist, n_waves, per_wave, dm.fallback, dm.status);
+  if (c->E > 0 && staged && kp.fallback) {
+    const int n_waves = td.n_tiles * (td.T / 64);
+    if (precision == BF_PREC_MIXED)
+      hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3((n_waves + 3) / 4), dim3(kStagedBlock), 0, s,
+                         dm, kp, td, n_waves, x, y, z);
+    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, td.wcount,
+                       td.wlist, n_waves, 64, dm.fallback, dm.status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
                                          s));
     c->fb_prec = precision;
   }
   BF_HIP(hipEventRecord(e2, s));
-  if (step)
+  if (staged) {
+    BF_HIP(launch_tile_fixup(td, c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, acc, fout,
+                             step, c->d_x, c->d_y, c->d_z, c->d_v, c->d_winv, c->d_fixed, c->dt,
+                             c->damp, s));
+  } else if (step) {
     BF_HIP(launch_gather_step(c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, c->d_fint,
                               c->d_x, c->d_y, c->d_z, c->d_v, c->d_winv, c->d_fixed, c->dt,
                               c->damp, s));
-  else if (c->n_nodes > 0)
+  } else if (c->n_nodes > 0) {
     hipLaunchKernelGGL(bf_gather_kernel, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s,
                        c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, accumulate, fint);
+  }
   BF_HIP(hipEventRecord(e3, s));
   c->timing = true;
   return kernel_launch_check();
@@ -1101,7 +1438,9 @@ int bf_force_eval_device_noalloc(bf_ctx* c, const double* d_x, const double* d_y
   if (!c || !d_x || !d_y || !d_z || !d_fint) return BF_ERR_ARG;
   if (!c->has_mesh || !c->has_params) return BF_ERR_STATE;
   if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device]) return BF_ERR_STATE;
-  const ScratchLayout L = noalloc_layout((size_t)c->E, (size_t)c->NS);
+  const ScratchLayout L = noalloc_layout((size_t)c->E, (size_t)c->NS, (size_t)c->n_nodes,
+                                        (size_t)c->n_tiles, (size_t)c->tileT,
+                                        (size_t)c->n_partials);
   if (L.total > kNoAllocScratchBytes) {
     fprintf(stderr, "biofem_hip: bf_force_eval_device_noalloc: the mesh needs %zu bytes of "
             "scratch (%d elements, %d patch slots), above the static maximum of %zu\n", L.total,
@@ -1119,7 +1458,18 @@ int bf_force_eval_device_noalloc(bf_ctx* c, const double* d_x, const double* d_y
   dm.qsv = (double*)(base + L.qsv);
   dm.qst = (int*)(base + L.qst);
   dm.redo = (int*)(base + L.redo);
-  return launch_eval(c, d_x, d_y, d_z, d_fint, accumulate, precision, s, false, &dm);
+  TileDev td = tile_dev(c);
+  td.partial = (double*)(base + L.partial);
+  td.touched = (unsigned char*)(base + L.touched);
+  td.deferred = (unsigned char*)(base + L.deferred);
+  td.wlist = (int*)(base + L.twlist);
+  td.wcount = (int*)(base + L.twcount);
+  // the touched marks start at zero (the node pass clears what the tile kernel sets; the
+  // array is shared with other meshes' calls, so it is cleared here, by a kernel)
+  if (c->n_nodes > 0)
+    hipLaunchKernelGGL(bf_fill_u8, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s, td.touched,
+                       c->n_nodes, (unsigned char)0);
+  return launch_eval(c, d_x, d_y, d_z, d_fint, accumulate, precision, s, false, &dm, &td);
 }
 
 int bf_ghost_update_device(bf_ctx* c, double* d_x, double* d_y, double* d_z, void* stream) {
@@ -1393,11 +1743,12 @@ struct bf_batch {
   int persistent = 1;
   // device state
   CaseDev* d_cases = nullptr;
-  Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr, *d_xtiles = nullptr;
+  Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr, *d_xtiles = nullptr,
+       *d_ttiles = nullptr;
   int *d_tlist = nullptr, *d_tcount = nullptr, *d_htotal = nullptr, *d_rtotal = nullptr;
   int2 *d_hlist = nullptr, *d_rlist = nullptr;
   int* d_cstat = nullptr;  // [2 n_cases]: per-case status words read by bf_batch_case_status
-  int n_etiles = 0, n_ntiles = 0, n_gtiles = 0, n_xtiles = 0, tile_cap = 0;
+  int n_etiles = 0, n_ntiles = 0, n_gtiles = 0, n_xtiles = 0, n_ttiles = 0, tile_cap = 0;
   long long E_total = 0;
   int* h_total = nullptr;  // pinned
   // captured step
@@ -1418,10 +1769,10 @@ namespace {
 
 void batch_free_lists(bf_batch* b) {
   dfree(b->d_cases); dfree(b->d_etiles); dfree(b->d_ntiles); dfree(b->d_gtiles);
-  dfree(b->d_xtiles);
+  dfree(b->d_xtiles); dfree(b->d_ttiles);
   dfree(b->d_tlist); dfree(b->d_tcount); dfree(b->d_hlist); dfree(b->d_htotal);
   dfree(b->d_rlist); dfree(b->d_rtotal); dfree(b->d_cstat);
-  b->n_etiles = b->n_ntiles = b->n_gtiles = b->n_xtiles = 0;
+  b->n_etiles = b->n_ntiles = b->n_gtiles = b->n_xtiles = b->n_ttiles = 0;
 }
 
 void batch_drop_graph(bf_batch* b) {
@@ -1445,22 +1796,40 @@ int batch_build(bf_batch* b) {
     E_all += c->E;
   }
   b->P = P;
-  if (!b->launch_set) {
-    // untuned: as many element passes per work item (1 .. 4) as keep every wave slot of
-    // the staged kernels (two per SIMD) busy -- a small case set gets one pass per wave
-    // instead of four passes on a quarter of the waves
-    const long long slots = (long long)b->n_cu * 8 * (64 / P);
-    b->passes = (int)std::max(1LL, std::min(4LL, E_all / std::max(1LL, slots)));
-  }
-  b->tile_cap = b->passes * (64 / P);
+  (void)E_all;
+  // Single-pass staged kernel: every case tiled with the batch's tile size (`block`
+  // threads, one element each), work items (case, tile); hand-over segments per tile wave
+  // (64 entries), which the FP32-mixed second tier and the compaction walk.
+  const int T = b->launch_set ? b->block : kTileElemsDefault;
+  b->block = T;
+  for (bf_ctx* c : b->cases)
+    if (c->tileT != T) {
+      const int rc = build_ctx_tiles(c, T);
+      if (rc != BF_OK) return rc;
+    }
+  b->tile_cap = 64;
+  long long n_tw = 0;
+  for (bf_ctx* c : b->cases) n_tw += (long long)c->n_tiles * (T / 64);
+  int rc;
+  if ((rc = dalloc(&b->d_tlist, (size_t)std::max<long long>(n_tw, 1) * 64)) != BF_OK) return rc;
   std::vector<CaseDev> cases((size_t)nc);
-  std::vector<Tile> et, nt, gt, xt;
-  long long Et = 0;
+  std::vector<Tile> et, nt, gt, xt, tt;
+  std::vector<long long> case_wave((size_t)nc, 0);  // first tile wave of each case
+  long long Et = 0, wave_base = 0;
   for (int i = 0; i < nc; i++) {
     bf_ctx* c = b->cases[i];
     CaseDev& cd = cases[i];
     cd.m = dev_mesh(c);
     cd.kp = dev_params(c);
+    cd.td = tile_dev(c);
+    cd.td.wlist = b->d_tlist + wave_base * 64;
+    cd.td.wcount = nullptr;  // set below, once the count array exists
+    for (int t = 0; t < c->n_tiles; t++) {
+      tt.push_back(Tile{i, t, 0, 0});
+      for (int w = 0; w < T / 64; w++) et.push_back(Tile{i, t, w, 0});
+    }
+    case_wave[i] = wave_base;
+    wave_base += (long long)c->n_tiles * (T / 64);
     cd.x = c->d_x;
     cd.y = c->d_y;
     cd.z = c->d_z;
@@ -1477,7 +1846,6 @@ int batch_build(bf_batch* b) {
     cd.fixed = c->d_fixed;
     cd.dt = c->dt;
     cd.damp = c->damp;
-    for (int e0 = 0; e0 < c->E; e0 += b->tile_cap) et.push_back(Tile{i, e0, std::min(b->tile_cap, c->E - e0), 0});
     for (int n0 = 0; n0 < c->n_nodes; n0 += 256) nt.push_back(Tile{i, n0, std::min(256, c->n_nodes - n0), 0});
     for (int g0 = 0; g0 < c->n_ghost; g0 += 256) gt.push_back(Tile{i, g0, std::min(256, c->n_ghost - g0), 0});
     for (int e0 = 0; e0 < c->E; e0 += 64) xt.push_back(Tile{i, e0, std::min(64, c->E - e0), 0});
@@ -1488,25 +1856,29 @@ int batch_build(bf_batch* b) {
   b->n_ntiles = (int)nt.size();
   b->n_gtiles = (int)gt.size();
   b->n_xtiles = (int)xt.size();
-  int rc;
+  b->n_ttiles = (int)tt.size();
 #define BALLOC(p, n) \
   if ((rc = dalloc(&(p), (n))) != BF_OK) { batch_free_lists(b); return rc; }
   BALLOC(b->d_cases, (size_t)nc);
-  BALLOC(b->d_etiles, et.size());
-  BALLOC(b->d_ntiles, nt.size());
+  BALLOC(b->d_etiles, std::max<size_t>(et.size(), 1));
+  BALLOC(b->d_ntiles, std::max<size_t>(nt.size(), 1));
   BALLOC(b->d_gtiles, std::max<size_t>(gt.size(), 1));
   BALLOC(b->d_xtiles, std::max<size_t>(xt.size(), 1));
-  BALLOC(b->d_tlist, (size_t)b->n_etiles * b->tile_cap);
-  BALLOC(b->d_tcount, (size_t)b->n_etiles);
+  BALLOC(b->d_ttiles, std::max<size_t>(tt.size(), 1));
+  BALLOC(b->d_tcount, (size_t)std::max(b->n_etiles, 1));
   BALLOC(b->d_hlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_htotal, 1);
   BALLOC(b->d_rlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_rtotal, 2);
   BALLOC(b->d_cstat, 2 * cases.size());
 #undef BALLOC
+  for (int i = 0; i < nc; i++) cases[i].td.wcount = b->d_tcount + case_wave[i];
   hipStream_t s = b->stream;
   BF_HIP(hipMemcpyAsync(b->d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, s));
-  BF_HIP(hipMemcpyAsync(b->d_etiles, et.data(), sizeof(Tile) * et.size(), hipMemcpyHostToDevice, s));
+  if (!et.empty())
+    BF_HIP(hipMemcpyAsync(b->d_etiles, et.data(), sizeof(Tile) * et.size(), hipMemcpyHostToDevice, s));
+  if (!tt.empty())
+    BF_HIP(hipMemcpyAsync(b->d_ttiles, tt.data(), sizeof(Tile) * tt.size(), hipMemcpyHostToDevice, s));
   BF_HIP(hipMemcpyAsync(b->d_ntiles, nt.data(), sizeof(Tile) * nt.size(), hipMemcpyHostToDevice, s));
   if (!gt.empty())
     BF_HIP(hipMemcpyAsync(b->d_gtiles, gt.data(), sizeof(Tile) * gt.size(), hipMemcpyHostToDevice, s));
@@ -1531,6 +1903,9 @@ BatchDev batch_dev(const bf_batch* b) {
   d.n_gtiles = b->n_gtiles;
   d.xtiles = b->d_xtiles;
   d.n_xtiles = b->n_xtiles;
+  d.ttiles = b->d_ttiles;
+  d.n_ttiles = b->n_ttiles;
+  d.tileT = b->block;
   d.tlist = b->d_tlist;
   d.tcount = b->d_tcount;
   d.hlist = b->d_hlist;
@@ -1547,26 +1922,31 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
   const BatchDev d = batch_dev(b);
   const int pblocks = std::max(1, std::min(b->n_cu * 2, std::max(b->n_gtiles, (d.n_cases + 255) / 256)));
   hipLaunchKernelGGL(bf_batch_prologue, dim3(pblocks), dim3(256), 0, s, d, ghosts);
-  const int wpb = b->block / 64;
   if (precision == BF_PREC_FP64) {
     BF_HIP(launch_batch_exact(d, b->n_cu, s));
-  } else {
-    // the staged kernels hold two waves per SIMD: 8 per CU
+    if (step)
+      BF_HIP(launch_batch_gather_step(d, s));
+    else
+      hipLaunchKernelGGL(bf_batch_gather_kernel, dim3(b->n_ntiles), dim3(256), 0, s, d);
+    return kernel_launch_check();
+  }
+  if (b->n_ttiles > 0) {
+    // the tile kernel holds two waves per SIMD: 8 per CU
+    const int wpb = d.tileT / 64;
     int blocks = b->blocks > 0 ? b->blocks : (b->n_cu * 8 + wpb - 1) / wpb;
-    if (!b->persistent) blocks = (b->n_etiles + wpb - 1) / wpb;
+    if (!b->persistent) blocks = b->n_ttiles;
+    const size_t smem = tile_smem_bytes(d.tileT);
     if (precision == BF_PREC_FP64_FAST) {
-      hipLaunchKernelGGL(bf_batch_element_kernel<double>, dim3(blocks), dim3(b->block), 0, s, d);
+      hipLaunchKernelGGL(bf_batch_tile_kernel<double>, dim3(blocks), dim3(d.tileT), smem, s, d);
     } else {
-      hipLaunchKernelGGL(bf_batch_element_kernel<float>, dim3(blocks), dim3(b->block), 0, s, d);
-      hipLaunchKernelGGL(bf_batch_tier2_kernel, dim3(blocks), dim3(b->block), 0, s, d);
+      hipLaunchKernelGGL(bf_batch_tile_kernel<float>, dim3(blocks), dim3(d.tileT), smem, s, d);
+      // one wave per tile wave (its segment), persistent over the work list
+      hipLaunchKernelGGL(bf_batch_tier2_kernel, dim3(b->n_cu * 2), dim3(kStagedBlock), 0, s, d);
     }
     hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d);
     BF_HIP(launch_batch_exact_list(d, b->n_cu, s));
   }
-  if (step)
-    BF_HIP(launch_batch_gather_step(d, s));
-  else
-    hipLaunchKernelGGL(bf_batch_gather_kernel, dim3(b->n_ntiles), dim3(256), 0, s, d);
+  BF_HIP(launch_batch_tile_fixup(d, step, s));
   return kernel_launch_check();
 }
 
@@ -1825,17 +2205,16 @@ int bf_batch_tune(bf_batch* b, int precision, int reps, int* passes, int* block,
   if (!b || reps < 1) return BF_ERR_ARG;
   if (b->cases.empty()) return BF_ERR_STATE;
   BF_HIP(hipSetDevice(b->device));
+  // tile size (elements = threads per workgroup) and grid of the single-pass staged kernel:
+  // one workgroup per tile, or persistent workgroups for one or two waves per SIMD
   struct Cand { int passes, block, blocks, persistent; };
   std::vector<Cand> cands;
-  const int pass_opts[] = {1, 2, 4, 8, 16};
   const int block_opts[] = {64, 128, 256};
-  for (int ps : pass_opts)
-    for (int bl : block_opts) {
-      const int wpb = bl / 64;
-      cands.push_back({ps, bl, 0, 0});  // one tile per wave
-      for (int wps : {1, 2})            // persistent: waves per SIMD
-        cands.push_back({ps, bl, (b->n_cu * 4 * wps + wpb - 1) / wpb, 1});
-    }
+  for (int bl : block_opts) {
+    const int wpb = bl / 64;
+    cands.push_back({1, bl, 0, 0});
+    for (int wps : {1, 2}) cands.push_back({1, bl, (b->n_cu * 4 * wps + wpb - 1) / wpb, 1});
+  }
   float best = 1e30f;
   Cand bc{b->passes, b->block, b->blocks, b->persistent};
   b->launch_set = 1;

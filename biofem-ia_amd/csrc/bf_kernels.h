@@ -44,6 +44,28 @@ struct DevMesh {
   int* qst;            // [kMaxQp][E] reference-order path: status bits per qp
   int* redo;           // [E] elements whose CC3333 guard tripped: redone with coupled qps
 };
+// Element tiling of the single-pass staged kernel (host side: bf_tiles.h) as the kernels
+// see it, plus its per-evaluation arrays.
+struct TileDev {
+  int n_tiles, NS, T, lmax;     // tiles, patch slots, elements (threads) per tile, node cap
+  const int* elem;              // [n_tiles * T] element of each lane, -1 if none
+  const unsigned short* lslot;  // [n_tiles][NS][T] local node | phase << 12, 0xFFFF unused
+  const unsigned char* nphase;  // [n_tiles][NS] phases of each slot
+  const int* node_ptr;          // [n_tiles + 1] range of each tile's local nodes
+  const int* node;              // global node of each local node
+  const int* out;               // -1 complete (the tile writes fint), -2 none, >= 0 partial
+  const int* pptr;              // [n_v + 1] partial slots of each real node, tile order
+  const int* plist;
+  const unsigned char* complete;  // [n_v] 1: one tile holds every contribution
+  // per evaluation
+  double* partial;              // [n_partials][3] tile partial sums
+  unsigned char* touched;       // [n_nodes] node of a handed-over element (zero between
+                                // evaluations: the node pass clears what the tiles set)
+  unsigned char* deferred;      // [E] 1: handed over, its force lands in the element buffer
+  int* wlist;                   // [n_tiles * T] hand-over segments, 64 entries per wave
+  int* wcount;                  // [n_tiles * T / 64]
+};
+
 // status[5]: number of entries in redo
 // status[6]: work counter of the hand-over solve
 // status[7]: the narrow hand-over solve's list length, when the wide solve runs first
@@ -141,6 +163,7 @@ struct CaseDev {
   const unsigned* ent;    // element << 5 | slot, ascending per node
   const int* rules;       // ghost rules [3 * n_ghost]
   int n_v, n_nodes, n_ghost, pad;
+  TileDev td;             // tiling of the single-pass staged kernels
   // time-stepping state (StepState); v == nullptr: force evaluation only
   double* v;              // [3 * n_nodes] velocities
   const double* winv;     // [n_nodes] dt / lumped mass
@@ -165,6 +188,9 @@ struct BatchDev {
   int n_gtiles;
   const Tile* xtiles;     // element tiles of 64 for the reference-order path
   int n_xtiles;
+  const Tile* ttiles;     // element tiles of the single-pass staged kernel: (case, tile)
+  int n_ttiles;
+  int tileT;              // elements (threads) per tile, the same for every case
   int* tlist;             // [n_etiles * tile_cap] hand-over segments
   int* tcount;            // [n_etiles]
   int2* hlist;            // compacted hand-over list: (case, element)
@@ -337,5 +363,15 @@ hipError_t launch_gather_step(const int* ptr, const unsigned* ent, const double*
                               const unsigned char* fixed, double dt, double damp, hipStream_t s);
 hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s);
 hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, hipStream_t s);
+// Node pass of the single-pass staged path: each node's force from its tile partial sums
+// (tile order) or the value its tile wrote, then the forces of handed-over elements
+// (element-force buffer, ascending element order) for nodes that have any; step != 0 also
+// applies the explicit damped update (as launch_gather_step).  Batch form over node tiles.
+hipError_t launch_tile_fixup(const TileDev& td, const int* ptr, const unsigned* ent,
+                             const double* Fe, int E, int n_v, int n_nodes, int accumulate,
+                             double* fint, bool step, double* x, double* y, double* z, double* v,
+                             const double* winv, const unsigned char* fixed, double dt,
+                             double damp, hipStream_t s);
+hipError_t launch_batch_tile_fixup(const BatchDev& b, bool step, hipStream_t s);
 
 }  // namespace bf

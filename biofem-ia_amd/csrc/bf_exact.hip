@@ -773,7 +773,7 @@ __device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __
     f[1] = accumulate ? fo[1] : 0.0;
     f[2] = accumulate ? fo[2] : 0.0;
     for (int j = p0; j < p1; j++) {
-      const double* pr = td.partial + 3 * (size_t)td.plist[j];
+      const double* pr = td.partial + 3 * (size_t)j;
       f[0] += pr[0];
       f[1] += pr[1];
       f[2] += pr[2];

@@ -199,19 +199,54 @@ __device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& 
 constexpr int kStagedBlock = 256;
 
 // ---- single-pass staged element phase ----
-// One tile per workgroup, one element per lane (tiles of td.T elements, bf_tiles.h):
-//   1. the coordinates of the tile's local nodes are staged in LDS;
-//   2. each lane forms its element's current midsurface geometry once (FP64, patch
-//      coordinates from LDS), runs the staged chain of every through-thickness point in R,
-//      sums the resultants in qp order and decides the hand-over as the reference's
-//      conditioning requires (needs_exact_bits);
-//   3. slot by slot the lanes add their nodal forces into LDS node accumulators, in the
-//      phases the tiling assigned (no two lanes of a phase share a node; a barrier between
-//      phases), so every node sum has a fixed order;
-//   4. complete nodes are written to fint, partial nodes to their tile partial.
-// A handed-over element adds nothing here: it is listed in its wave's segment, marks its
-// nodes as touched and sets deferred; the node pass adds its force once it is known.
-// smem: 6 * td.lmax doubles (coordinates and accumulators, SoA).
+// One tile of td.T elements (bf_tiles.h) per workgroup of at least td.T * n_qp threads:
+//   0. the coordinates of the tile's local nodes are staged in LDS;
+//   1. thread i < T forms element i's current midsurface geometry once (FP64, patch
+//      coordinates from LDS), its a3,alpha and activation, and leaves them in LDS;
+//   2. thread i + q T runs through-thickness point q of element i's staged chain in R
+//      (q is uniform over a wave: T is a multiple of 64) and leaves its stresses, C33 and
+//      status bits in LDS;
+//   3. thread i sums the resultants in qp order, decides the hand-over as the reference's
+//      conditioning requires (needs_exact_bits), and folds geometry and resultants into
+//      five coefficient vectors: the force on patch slot a is
+//        Na0[a] U0 + Na1[a] U1 + (Nab0[a] v0 + Nab1[a] v1 + Nab2[a] v2) a3
+//      (ElemUpdFint's row with its sums regrouped), which it leaves in LDS per slot;
+//   4. thread l sums local node l's contributions in the tiling's order (lane order, then
+//      slot order) and writes a complete node to fint, a partial node to its tile partial.
+// A handed-over element contributes zeros here: it is listed in its wave's segment, marks
+// its nodes as touched and sets deferred; the node pass adds its force once it is known.
+// LDS (doubles, tile_smem_doubles): region A = node coordinates [3][lmax] and the element
+// block [rows][T] -- rows kept to phase 3 (a1, a2, a3, the six products gc_k . A_l, pref0),
+// then the chain-only inputs, which the qp results [4][P][T] replace once every chain thread
+// holds its inputs -- and, once phase 3 has read all of it, the slot forces [3][NS][T];
+// then the qp status bits [P][T] (ints), the tile's local slot codes [NS][T] (ushort), its
+// contribution lists (cent, ushort, at most NS T entries; cptr, lmax + 1 ints) and a touched
+// flag per local node (bytes).
+// Every global load that depends on the tile or element index alone is issued before the
+// first barrier, and the barriers order LDS only: the global stores of a phase drain while
+// the next one runs.
+constexpr int kTileMaxThreads = 512;  // T * n_qp: two waves per SIMD at 256 VGPRs
+enum : int { kTbA1 = 0, kTbA2 = 3, kTbA3 = 6, kTbG = 9, kTbPref = 15, kTbKeep = 16,
+             kTbA31 = 16, kTbA32 = 19, kTbFib = 22, kTbD = 25, kTbGam = 28, kTbChain = 13 };
+__host__ __device__ inline int tile_elem_rows(int P) {
+  return kTbKeep + (4 * P > kTbChain ? 4 * P : kTbChain);
+}
+__host__ __device__ inline size_t tile_region_a(int T, int P, int NS, int lmax) {
+  const size_t a = 3 * (size_t)lmax + (size_t)tile_elem_rows(P) * T, f = 3 * (size_t)NS * T;
+  return a > f ? a : f;
+}
+__host__ __device__ inline size_t tile_smem_doubles(int T, int P, int NS, int lmax) {
+  return tile_region_a(T, P, NS, lmax) + ((size_t)P * T + 1) / 2 + ((size_t)NS * T + 3) / 4 +
+         ((size_t)NS * T + 3) / 4 + ((size_t)lmax + 2) / 2 + ((size_t)lmax + 7) / 8;
+}
+
+// Workgroup barrier that orders LDS only (global stores stay in flight across it).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <class R>
 __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, const TileDev& td,
                                           int t, const double* __restrict__ x,
@@ -220,111 +255,179 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
                                           double* __restrict__ fint, int accumulate, int n_v,
                                           double* smem) {
   const int T = td.T, NS = td.NS, lm = td.lmax;
-  const int lane = (int)threadIdx.x;
+  const int NT = (int)blockDim.x;
+  const int P = kp.n_qp;  // chain threads per element; threads past T * P only take barriers
+  const int tid = (int)threadIdx.x;
   double *sx = smem, *sy = smem + lm, *sz = smem + 2 * lm;
-  double *fx = smem + 3 * lm, *fy = smem + 4 * lm, *fz = smem + 5 * lm;
+  double* eb = smem + 3 * lm;                           // element block [rows][T]
+  double* qb = eb + (size_t)kTbKeep * T;                // qp results [4][P][T] (after 2a)
+  double* sf = smem;                                    // slot forces [3][NS][T] (phase 3b)
+  const size_t ra = tile_region_a(T, P, NS, lm);
+  int* qs = (int*)(smem + ra);                          // qp status [P][T]
+  unsigned short* sc = (unsigned short*)(smem + ra + ((size_t)P * T + 1) / 2);  // [NS][T]
+  unsigned short* ce = sc + (size_t)((NS * T + 3) / 4) * 4;  // contributions of the tile
+  int* cp = (int*)(ce + (size_t)((NS * T + 3) / 4) * 4);    // [lmax + 1] ranges in ce
+  unsigned char* tf = (unsigned char*)(cp + ((lm + 2) / 2) * 2);  // [lmax] touched flags
+  const int E = kp.E;
+  const int q = tid / T, i = tid - q * T;  // q: wave-uniform
+  const bool elem_thread = q == 0;
+  const int e = td.elem[(size_t)t * T + i];
+  const bool live = e >= 0;
+  const bool chain = live && q < P;
+  // ---- loads that depend on the tile and the element only ----
+  double rv[15];
+  if (chain) {
+#pragma unroll
+    for (int k = 0; k < 15; k++) rv[k] = m.ref[(size_t)k * E + e];
+  }
+  int meta = 0;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0, gs = 1.0, dly = 0.0, A0 = 0.0;
+  int csl[3] = {-1, -1, -1};
+  if (elem_thread) {
+    const unsigned short* ls = td.lslot + (size_t)t * NS * T + i;
+    unsigned short cv[kMaxNen];  // all loads in flight before the first store
+#pragma unroll
+    for (int a = 0; a < kMaxNen; a++) cv[a] = a < NS ? ls[(size_t)a * T] : kTileIdle;
+#pragma unroll
+    for (int a = 0; a < kMaxNen; a++)
+      if (a < NS) sc[(size_t)a * T + i] = cv[a];
+    if (live) {
+      meta = m.meta[e];
+      f0 = m.fib[e];
+      f1 = m.fib[(size_t)E + e];
+      f2 = m.fib[(size_t)2 * E + e];
+      gs = m.gscale ? m.gscale[e] : 1.0;
+      dly = kp.has_delay ? m.delay[e] : 0.0;
+      A0 = m.dA0[e];
+      if ((meta & 3) == kCap)
+        for (int k = 0; k < 3; k++) csl[k] = m.corner[(size_t)(3 + k) * E + e];
+    }
+  }
   const int j0 = td.node_ptr[t], L = td.node_ptr[t + 1] - j0;
-  __syncthreads();  // the previous tile of a persistent workgroup is done with the LDS
-  for (int l = lane; l < L; l += T) {
+  const int cb = td.cptr[j0], nc = td.cptr[j0 + L] - cb;
+  for (int l = tid; l < L; l += NT) {
     const int n = td.node[j0 + l];
+    const int c = td.cptr[j0 + l + 1];
     sx[l] = x[n];
     sy[l] = y[n];
     sz[l] = z[n];
-    fx[l] = 0.0;
-    fy[l] = 0.0;
-    fz[l] = 0.0;
+    cp[l + 1] = c - cb;
+    tf[l] = 0;
   }
-  __syncthreads();
-  const int E = kp.E;
-  const int e = td.elem[(size_t)t * T + lane];
-  const unsigned short* ls = td.lslot + (size_t)t * NS * T + lane;  // slot a: ls[a * T]
-  const bool live = e >= 0;
-  MidGeom G;
-  const double *na0 = nullptr, *na1 = nullptr, *nb0 = nullptr, *nb1 = nullptr, *nb2 = nullptr;
-  int kind = kRegular;
-  bool handed = false;
-  Resultants<R> r;
-  r.N0 = r.N1 = r.N2 = r.M0 = r.M1 = r.M2 = 0.0;
-  double pref0 = 0.0;
-  int csl[3] = {-1, -1, -1};
-  V3<double> fc[3];
-  if (live) {
-    const int meta = m.meta[e];
-    double rv[15];
-#pragma unroll
-    for (int k = 0; k < 15; k++) rv[k] = m.ref[(size_t)k * E + e];
-    const double f0 = m.fib[e], f1 = m.fib[(size_t)E + e], f2 = m.fib[(size_t)2 * E + e];
-    const double gs = m.gscale ? m.gscale[e] : 1.0;
-    const double dly = kp.has_delay ? m.delay[e] : 0.0;
-    const double A0 = m.dA0[e];
-    kind = meta & 3;
-    const int val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
-    auto xyz = [&](int i) {
-      const int l = ls[(size_t)i * T] & 0xFFF;
+  for (int k = tid; k < nc; k += NT) ce[k] = td.cent[cb + k];
+  if (tid == 0) cp[0] = 0;
+  lds_barrier();
+  const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
+  // ---- 1. geometry and activation, once per element ----
+  if (elem_thread && live) {
+    auto xyz = [&](int s) {
+      const int l = sc[(size_t)s * T + i];
       return mk(sx[l], sy[l], sz[l]);
     };
+    MidGeom G;
     if (kind == kRegular) {
-      na0 = c_Na[0]; na1 = c_Na[1]; nb0 = c_Nab[0]; nb1 = c_Nab[1]; nb2 = c_Nab[2];
       G = mid_geom_regular(xyz);
     } else if (kind == kIrregular) {
       const double(*tb)[kMaxNen] = c_IN[val];
-      na0 = tb[0]; na1 = tb[1]; nb0 = tb[2]; nb1 = tb[3]; nb2 = tb[4];
-      G = mid_geom(nen, na0, na1, nb0, nb1, nb2, xyz);
+      G = mid_geom(nen, tb[0], tb[1], tb[2], tb[3], tb[4], xyz);
     } else {
-      for (int k = 0; k < 3; k++) csl[k] = m.corner[(size_t)(3 + k) * E + e];
       G = cap_geom(xyz(csl[0]), xyz(csl[1]), xyz(csl[2]));
     }
     V3<double> a31c, a32c;
     a3_alpha(G.a1, G.a2, G.Aaa, G.Aab, G.Abb, a31c, a32c);
+    double g = kp.has_delay ? ramp_gamma(kp.ramp, dly) : kp.gamma_t;
+    g *= gs;
+    double d0, d1, d2, gam;
+    if (kp.act_mode == 0) {
+      d0 = d1 = d2 = 1.0;
+      gam = g;
+    } else if (kp.act_mode == 1) {
+      d0 = 1.0 - kp.man_gamma;
+      d1 = d2 = 1.0;
+      gam = 0.0;
+    } else {
+      d0 = d1 = 1.0 - g;
+      d2 = 1.0 / (d0 * d1);
+      gam = 0.0;
+    }
+    const double v[kTbKeep + kTbChain] = {
+        G.a1.x, G.a1.y, G.a1.z, G.a2.x, G.a2.y, G.a2.z, G.a3.x, G.a3.y, G.a3.z,
+        dot(G.gc1, G.Aaa), dot(G.gc1, G.Abb), dot(G.gc1, G.Aab),
+        dot(G.gc2, G.Aaa), dot(G.gc2, G.Abb), dot(G.gc2, G.Aab), 0.5 * kp.h0 * A0,
+        a31c.x, a31c.y, a31c.z, a32c.x, a32c.y, a32c.z, f0, f1, f2, d0, d1, d2, gam};
+#pragma unroll
+    for (int k = 0; k < kTbKeep + kTbChain; k++) eb[(size_t)k * T + i] = v[k];
+  }
+  lds_barrier();
+  // ---- 2. one (element, qp) per thread ----
+  V3<double> a1c, a2c, a3c, a31c, a32c;
+  Activation act;
+  auto ld3 = [&](int k) { return mk(eb[(size_t)k * T + i], eb[(size_t)(k + 1) * T + i],
+                                    eb[(size_t)(k + 2) * T + i]); };
+  if (chain) {
+    a1c = ld3(kTbA1);
+    a2c = ld3(kTbA2);
+    a3c = ld3(kTbA3);
+    a31c = ld3(kTbA31);
+    a32c = ld3(kTbA32);
+    act.f = ld3(kTbFib);
+    act.d0 = eb[(size_t)kTbD * T + i];
+    act.d1 = eb[(size_t)(kTbD + 1) * T + i];
+    act.d2 = eb[(size_t)(kTbD + 2) * T + i];
+    act.gam = eb[(size_t)kTbGam * T + i];
+  }
+  lds_barrier();  // every chain thread holds its inputs: the qp results take their place
+  if (chain) {
     RefGeom rg;
     rg.a1 = mk(rv[0], rv[1], rv[2]);
     rg.a2 = mk(rv[3], rv[4], rv[5]);
     rg.a3 = mk(rv[6], rv[7], rv[8]);
     rg.a31 = mk(rv[9], rv[10], rv[11]);
     rg.a32 = mk(rv[12], rv[13], rv[14]);
-    double g = kp.has_delay ? ramp_gamma(kp.ramp, dly) : kp.gamma_t;
-    g *= gs;
-    Activation act;
-    act.f = mk(f0, f1, f2);
-    if (kp.act_mode == 0) {
-      act.d0 = act.d1 = act.d2 = 1.0;
-      act.gam = g;
-    } else if (kp.act_mode == 1) {
-      act.d0 = 1.0 - kp.man_gamma;
-      act.d1 = act.d2 = 1.0;
-      act.gam = 0.0;
-    } else {
-      act.d0 = act.d1 = 1.0 - g;
-      act.d2 = 1.0 / (act.d0 * act.d1);
-      act.gam = 0.0;
-    }
     auto own = [](bool fail, int) { return fail; };
+    const double th = kp.theta[q];
+    StagedQp<R> sq;
+    if constexpr (sizeof(R) == 4)
+      sq = staged_qp_mixed(a1c, a2c, a3c, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters, th,
+                           own, kp.fallback != 0);
+    else
+      sq = staged_qp<R>(a1c, a2c, a3c, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters, th, own,
+                        kp.fallback != 0, kC33NoiseMax);
+    double* o = qb + (size_t)q * T + i;
+    o[0] = (double)sq.S00;
+    o[(size_t)P * T] = (double)sq.S11;
+    o[(size_t)2 * P * T] = (double)sq.S01;
+    o[(size_t)3 * P * T] = sq.c33;
+    qs[q * T + i] = sq.status | c33_range_bit(sq.c33, sizeof(R) == 4);
+    // handed-over elements get their C33 from the reference-order solve later
+    m.c33[(size_t)q * E + e] = sq.c33;
+  }
+  lds_barrier();
+  // ---- 3. resultants, hand-over, force coefficients ----
+  bool handed = false;
+  V3<double> U0 = mk(0.0, 0.0, 0.0), U1 = U0, A3 = U0;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  if (elem_thread && live) {
+    Resultants<R> r;
+    r.N0 = r.N1 = r.N2 = r.M0 = r.M1 = r.M2 = 0.0;
     int st = 0;
-    for (int q = 0; q < kp.n_qp; q++) {
-      const double th = kp.theta[q];
-      StagedQp<R> sq;
-      if constexpr (sizeof(R) == 4)
-        sq = staged_qp_mixed(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K,
-                             kp.iters, th, own, kp.fallback != 0);
-      else
-        sq = staged_qp<R>(G.a1, G.a2, G.a3, a31c, a32c, rg, act, kp.h0, kp.mu, kp.K, kp.iters,
-                          th, own, kp.fallback != 0, kC33NoiseMax);
-      add_resultants(r, sq, kp.w[q], th);
-      st |= sq.status | c33_range_bit(sq.c33, sizeof(R) == 4);
-      // handed-over elements get their C33 from the reference-order solve later
-      m.c33[(size_t)q * E + e] = sq.c33;
+    for (int k = 0; k < P; k++) {
+      const double* o = qb + (size_t)k * T + i;
+      StagedQp<double> s;
+      s.S00 = o[0];
+      s.S11 = o[(size_t)P * T];
+      s.S01 = o[(size_t)2 * P * T];
+      add_resultants(r, s, kp.w[k], kp.theta[k]);
+      st |= qs[k * T + i];
     }
     const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) &&
                         isfinite(r.M0) && isfinite(r.M1) && isfinite(r.M2);
     handed = kp.fallback && needs_exact_bits(st, finite);
     td.deferred[e] = handed ? 1 : 0;
-    pref0 = 0.5 * kp.h0 * A0;
-    if (handed) {
+    if (handed) {  // its nodes are marked in LDS; phase 4 writes the marks
       for (int a = 0; a < nen; a++) {
-        const unsigned short code = ls[(size_t)a * T];
-        if (code == kTileIdle) continue;  // a slot that adds no force (LV cap)
-        const int n = td.node[j0 + (code & 0xFFF)];
-        if (n < n_v) td.touched[n] = 1;
+        const unsigned short code = sc[(size_t)a * T + i];
+        if (code != kTileIdle) tf[code] = 1;  // kTileIdle: a slot without force (LV cap)
       }
     } else {
       if (!finite) atomicMin(&m.status[0], e);
@@ -333,62 +436,96 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
         atomicMin(&m.status[2], e);
         atomicAdd(&m.status[3], 1);
       }
-      if (kind == kCap) {  // CST membrane into the corner slots (:1807-1851)
-        const double dN[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
-        for (int k = 0; k < 3; k++)
-          fc[k] = node_force(dN[k][0], dN[k][1], 0.0, 0.0, 0.0, G, pref0, 0.5 * kp.h0, r.N0,
-                             r.N1, r.N2, 0.0, 0.0, 0.0);
+      // the kept rows: the qp results took the chain-input rows only
+      const V3<double> a1 = ld3(kTbA1), a2 = ld3(kTbA2);
+      A3 = ld3(kTbA3);
+      const double* gg = eb + (size_t)kTbG * T + i;
+      const double pref0 = eb[(size_t)kTbPref * T + i];
+      const double hh = 0.5 * kp.h0;
+      // LV cap: CST membrane only (:1807-1851)
+      const double M0 = kind == kCap ? 0.0 : r.M0, M1 = kind == kCap ? 0.0 : r.M1,
+                   M2 = kind == kCap ? 0.0 : r.M2;
+      const double b0 = gg[0] * M0 + gg[(size_t)T] * M1 + 2.0 * gg[(size_t)2 * T] * M2;
+      const double b1 = gg[(size_t)3 * T] * M0 + gg[(size_t)4 * T] * M1 +
+                        2.0 * gg[(size_t)5 * T] * M2;
+      U0 = scale(pref0, add(add(scale(r.N0, a1), scale(r.N2, a2)), scale(-hh * b0, A3)));
+      U1 = scale(pref0, add(add(scale(r.N2, a1), scale(r.N1, a2)), scale(-hh * b1, A3)));
+      v0 = -pref0 * hh * M0;
+      v1 = -pref0 * hh * M1;
+      v2 = -2.0 * pref0 * hh * M2;
+    }
+  }
+  if (elem_thread) WaveSink{td.wlist, td.wcount, 64, t * (T >> 6) + (tid >> 6)}.push(live && handed, e);
+  lds_barrier();  // region A is read: the slot forces take its place
+  if (elem_thread) {
+    const bool adds = live && !handed;
+    auto put = [&](int a, const V3<double>& f) {
+      sf[(size_t)a * T + i] = f.x;
+      sf[(size_t)(NS + a) * T + i] = f.y;
+      sf[(size_t)(2 * NS + a) * T + i] = f.z;
+    };
+    if (adds && kind == kRegular) {  // the Loop tables as immediates
+#pragma unroll
+      for (int a = 0; a < 12; a++) {
+        const double s = kLoopNab[0][a] * v0 + kLoopNab[1][a] * v1 + kLoopNab[2][a] * v2;
+        put(a, add(add(scale(kLoopNa[0][a], U0), scale(kLoopNa[1][a], U1)), scale(s, A3)));
+      }
+      for (int a = 12; a < NS; a++) put(a, mk(0.0, 0.0, 0.0));
+    } else if (adds && kind == kIrregular) {
+      const double(*tb)[kMaxNen] = c_IN[val];
+      for (int a = 0; a < NS; a++) {
+        V3<double> f = mk(0.0, 0.0, 0.0);
+        if (a < nen) {
+          const double s = tb[2][a] * v0 + tb[3][a] * v1 + tb[4][a] * v2;
+          f = add(add(scale(tb[0][a], U0), scale(tb[1][a], U1)), scale(s, A3));
+        }
+        put(a, f);
+      }
+    } else {
+      for (int a = 0; a < NS; a++) {
+        V3<double> f = mk(0.0, 0.0, 0.0);
+        if (adds) {  // LV cap, corner k of the CST: dN = (-1, -1), (1, 0), (0, 1)
+          if (csl[0] == a) f = add(f, scale(-1.0, add(U0, U1)));
+          if (csl[1] == a) f = add(f, U0);
+          if (csl[2] == a) f = add(f, U1);
+        }
+        put(a, f);
       }
     }
   }
-  WaveSink{td.wlist, td.wcount, 64, t * (T >> 6) + (lane >> 6)}.push(live && handed, e);
-  // ---- nodal forces into the LDS accumulators, slot by slot, phase by phase ----
-  const double hh = 0.5 * kp.h0;
-  for (int a = 0; a < NS; a++) {
-    const unsigned short code = ls[(size_t)a * T];
-    const bool contributes = live && !handed && code != kTileIdle;
-    V3<double> f = mk(0.0, 0.0, 0.0);
-    if (contributes) {
-      if (kind != kCap) {
-        f = node_force(na0[a], na1[a], nb0[a], nb1[a], nb2[a], G, pref0, hh, r.N0, r.N1, r.N2,
-                       r.M0, r.M1, r.M2);
-      } else {
-        for (int k = 0; k < 3; k++)
-          if (csl[k] == a) f = add(f, fc[k]);
-      }
+  lds_barrier();
+  // ---- 4. node sums in the tiling's order ----
+  for (int l = tid; l < L; l += NT) {
+    const int c0 = cp[l], c1 = cp[l + 1];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int c = c0; c < c1; c++) {
+      const unsigned v = ce[c];
+      const size_t k = (size_t)(v >> 8) * T + (v & 0xFFu);
+      ax += sf[k];
+      ay += sf[(size_t)NS * T + k];
+      az += sf[(size_t)2 * NS * T + k];
     }
-    const int nph = td.nphase[(size_t)t * NS + a];
-    const int ph_mine = code >> 12, l = code & 0xFFF;
-    for (int ph = 0; ph < nph; ph++) {
-      if (contributes && ph_mine == ph) {
-        fx[l] += f.x;
-        fy[l] += f.y;
-        fz[l] += f.z;
-      }
-      __syncthreads();
-    }
-  }
-  __syncthreads();
-  for (int l = lane; l < L; l += T) {
     const int o = td.out[j0 + l];
+    const int n = td.node[j0 + l];
+    if (tf[l] && n < n_v) td.touched[n] = 1;
     if (o == -1) {
-      double* fo = fint + 3 * (size_t)td.node[j0 + l];
+      double* fo = fint + 3 * (size_t)n;
       const double bx = accumulate ? fo[0] : 0.0, by = accumulate ? fo[1] : 0.0,
                    bz = accumulate ? fo[2] : 0.0;
-      fo[0] = bx + fx[l];
-      fo[1] = by + fy[l];
-      fo[2] = bz + fz[l];
+      fo[0] = bx + ax;
+      fo[1] = by + ay;
+      fo[2] = bz + az;
     } else if (o >= 0) {
       double* pr = td.partial + 3 * (size_t)o;
-      pr[0] = fx[l];
-      pr[1] = fy[l];
-      pr[2] = fz[l];
+      pr[0] = ax;
+      pr[1] = ay;
+      pr[2] = az;
     }
   }
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_per_eu(3)))
 bf_tile_kernel(DevMesh m, KParams kp, TileDev td, const double* __restrict__ x,
                const double* __restrict__ y, const double* __restrict__ z,
                double* __restrict__ fint, int accumulate, int n_v) {
@@ -557,22 +694,20 @@ __global__ void __launch_bounds__(256) bf_batch_prologue(BatchDev b, int ghosts)
   }
 }
 
-// Single-pass staged element phase of a batch: each workgroup takes tiles (case, tile) of
-// the work list -- persistent: a fixed stride over the list; otherwise one tile each.
+// Single-pass staged element phase of a batch: one tile (case, tile) of the work list per
+// workgroup.  (A persistent loop over the list held ~70 more VGPRs live across tiles: two
+// waves per SIMD instead of three.)
 template <class R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kTileMaxThreads) __attribute__((amdgpu_waves_per_eu(3)))
 bf_batch_tile_kernel(BatchDev b) {
   extern __shared__ double tile_smem[];
-  const int n_wg = (int)gridDim.x;
-  const int first = b.persistent ? (int)blockIdx.x : logical_block();
-  const int stride = b.persistent ? n_wg : b.n_ttiles;
-  for (int i = first; i < b.n_ttiles; i += stride) {
-    const Tile tl = b.ttiles[i];
-    const int c = __builtin_amdgcn_readfirstlane(tl.c);
-    const int t = __builtin_amdgcn_readfirstlane(tl.b);
-    const CaseDev& cd = b.cases[c];
-    tile_body<R>(cd.m, cd.kp, cd.td, t, cd.x, cd.y, cd.z, cd.fint, 0, cd.n_v, tile_smem);
-  }
+  const int i = logical_block();
+  if (i >= b.n_ttiles) return;
+  const Tile tl = b.ttiles[i];
+  const int c = __builtin_amdgcn_readfirstlane(tl.c);
+  const int t = __builtin_amdgcn_readfirstlane(tl.b);
+  const CaseDev& cd = b.cases[c];
+  tile_body<R>(cd.m, cd.kp, cd.td, t, cd.x, cd.y, cd.z, cd.fint, 0, cd.n_v, tile_smem);
 }
 
 // Second tier of the FP32-mixed batch path (as bf_tier2_kernel): the wave of tile t
@@ -737,12 +872,12 @@ struct bf_ctx {
   int n_tiles = 0, n_partials = 0, tile_lmax = 0;
   int* d_telem = nullptr;
   unsigned short* d_tlslot = nullptr;
-  unsigned char* d_tnphase = nullptr;
+  int* d_tcptr = nullptr;
+  unsigned short* d_tcent = nullptr;
   int* d_tnode_ptr = nullptr;
   int* d_tnode = nullptr;
   int* d_tout = nullptr;
   int* d_tpptr = nullptr;
-  int* d_tplist = nullptr;
   unsigned char* d_tcomplete = nullptr;
   double* d_tpartial = nullptr;
   unsigned char* d_touched = nullptr;
@@ -779,11 +914,25 @@ void dfree(T*& p) {
 }
 
 void free_tiles(bf_ctx* c) {
-  dfree(c->d_telem); dfree(c->d_tlslot); dfree(c->d_tnphase); dfree(c->d_tnode_ptr);
-  dfree(c->d_tnode); dfree(c->d_tout); dfree(c->d_tpptr); dfree(c->d_tplist);
+  dfree(c->d_telem); dfree(c->d_tlslot); dfree(c->d_tcptr); dfree(c->d_tcent);
+  dfree(c->d_tnode_ptr); dfree(c->d_tnode); dfree(c->d_tout); dfree(c->d_tpptr);
   dfree(c->d_tcomplete); dfree(c->d_tpartial); dfree(c->d_touched); dfree(c->d_deferred);
   dfree(c->d_twlist); dfree(c->d_twcount);
   c->n_tiles = c->n_partials = c->tileT = 0;
+}
+
+// LDS of the single-pass staged kernel for tiles of T elements and P chain threads each.
+static size_t tile_smem_bytes(int T, int P, int NS) {
+  return sizeof(double) * tile_smem_doubles(T, P, NS, tile_node_cap(T));
+}
+constexpr size_t kTileMaxLds = size_t(64) << 10;  // LDS of one workgroup
+
+// Tile size for a request of T elements per tile and P points per element: 128 or 64, the
+// larger whose workgroup (T * P threads) and LDS fit.
+static int tile_fit(int T, int P, int NS) {
+  T = T >= 128 ? 128 : 64;
+  while (T > 64 && (T * P > kTileMaxThreads || tile_smem_bytes(T, P, NS) > kTileMaxLds)) T /= 2;
+  return T;
 }
 
 // (Re)builds the single-pass tiling of a context's mesh with T elements per tile and
@@ -792,9 +941,7 @@ int build_ctx_tiles(bf_ctx* c, int T) {
   TileHost th;
   if (build_tiles(c->E, c->n_v, c->n_nodes, c->NS, c->h_patch.data(), c->h_nen.data(),
                   c->h_x0.data(), c->h_y0.data(), c->h_z0.data(), T, th) != 0) {
-    fprintf(stderr, "biofem_hip: element tiling needs more than %d phases per slot\n",
-            kTilePhases);
-    return BF_ERR_UNSUPPORTED;
+    return BF_ERR_ARG;
   }
   BF_HIP(hipSetDevice(c->device));
   BF_HIP(hipStreamSynchronize(c->stream));
@@ -806,12 +953,12 @@ int build_ctx_tiles(bf_ctx* c, int T) {
   if ((rc = dalloc(&(p), (n))) != BF_OK) { free_tiles(c); return rc; }
   TALLOC(c->d_telem, nt * T);
   TALLOC(c->d_tlslot, nt * (size_t)c->NS * T);
-  TALLOC(c->d_tnphase, nt * (size_t)c->NS);
+  TALLOC(c->d_tcptr, nloc + 1);
+  TALLOC(c->d_tcent, std::max<size_t>(th.cent.size(), 1));
   TALLOC(c->d_tnode_ptr, nt + 1);
   TALLOC(c->d_tnode, nloc);
   TALLOC(c->d_tout, nloc);
   TALLOC(c->d_tpptr, (size_t)c->n_v + 1);
-  TALLOC(c->d_tplist, (size_t)std::max(th.n_partials, 1));
   TALLOC(c->d_tcomplete, (size_t)std::max(c->n_v, 1));
   TALLOC(c->d_tpartial, 3 * (size_t)std::max(th.n_partials, 1));
   TALLOC(c->d_touched, (size_t)std::max(c->n_nodes, 1));
@@ -823,14 +970,13 @@ int build_ctx_tiles(bf_ctx* c, int T) {
   if (th.n_tiles > 0) {
     BF_HIP(hipMemcpyAsync(c->d_telem, th.elem.data(), sizeof(int) * th.elem.size(), hipMemcpyHostToDevice, s));
     BF_HIP(hipMemcpyAsync(c->d_tlslot, th.lslot.data(), sizeof(unsigned short) * th.lslot.size(), hipMemcpyHostToDevice, s));
-    BF_HIP(hipMemcpyAsync(c->d_tnphase, th.nphase.data(), th.nphase.size(), hipMemcpyHostToDevice, s));
+    BF_HIP(hipMemcpyAsync(c->d_tcent, th.cent.data(), sizeof(unsigned short) * th.cent.size(), hipMemcpyHostToDevice, s));
     BF_HIP(hipMemcpyAsync(c->d_tnode, th.node.data(), sizeof(int) * th.node.size(), hipMemcpyHostToDevice, s));
     BF_HIP(hipMemcpyAsync(c->d_tout, th.out.data(), sizeof(int) * th.out.size(), hipMemcpyHostToDevice, s));
   }
   BF_HIP(hipMemcpyAsync(c->d_tnode_ptr, th.node_ptr.data(), sizeof(int) * th.node_ptr.size(), hipMemcpyHostToDevice, s));
   BF_HIP(hipMemcpyAsync(c->d_tpptr, th.pptr.data(), sizeof(int) * th.pptr.size(), hipMemcpyHostToDevice, s));
-  if (th.n_partials > 0)
-    BF_HIP(hipMemcpyAsync(c->d_tplist, th.plist.data(), sizeof(int) * th.plist.size(), hipMemcpyHostToDevice, s));
+  BF_HIP(hipMemcpyAsync(c->d_tcptr, th.cptr.data(), sizeof(int) * th.cptr.size(), hipMemcpyHostToDevice, s));
   if (c->n_v > 0)
     BF_HIP(hipMemcpyAsync(c->d_tcomplete, th.complete.data(), th.complete.size(), hipMemcpyHostToDevice, s));
   BF_HIP(hipMemsetAsync(c->d_touched, 0, (size_t)std::max(c->n_nodes, 1), s));
@@ -1190,7 +1336,8 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
   c->h_x0.assign(m->x0, m->x0 + m->n_nodes);
   c->h_y0.assign(m->y0, m->y0 + m->n_nodes);
   c->h_z0.assign(m->z0, m->z0 + m->n_nodes);
-  if ((rc = build_ctx_tiles(c, kTileElemsDefault)) != BF_OK) {
+  if ((rc = build_ctx_tiles(c, tile_fit(kTileElemsDefault, c->has_params ? c->kp.n_qp : 1, c->NS))) !=
+      BF_OK) {
     free_mesh(c);
     return rc;
   }
@@ -1244,6 +1391,11 @@ int bf_params_set(bf_ctx* c, const bf_params* p) {
     const double s = sin(M_PI * t / p->manufactured_T);
     k.man_gamma = p->manufactured ? p->manufactured_gamma0 * pow(s, 2.0) : 0.0;
   }
+  // a workgroup of the single-pass staged kernel holds T * n_qp threads
+  if (c->has_mesh && tile_fit(c->tileT, p->n_qp, c->NS) != c->tileT) {
+    const int rc = build_ctx_tiles(c, tile_fit(c->tileT, p->n_qp, c->NS));
+    if (rc != BF_OK) return rc;
+  }
   c->has_params = true;
   return BF_OK;
 }
@@ -1288,12 +1440,12 @@ static TileDev tile_dev(const bf_ctx* c) {
   td.lmax = c->tile_lmax;
   td.elem = c->d_telem;
   td.lslot = c->d_tlslot;
-  td.nphase = c->d_tnphase;
+  td.cptr = c->d_tcptr;
+  td.cent = c->d_tcent;
   td.node_ptr = c->d_tnode_ptr;
   td.node = c->d_tnode;
   td.out = c->d_tout;
   td.pptr = c->d_tpptr;
-  td.plist = c->d_tplist;
   td.complete = c->d_tcomplete;
   td.partial = c->d_tpartial;
   td.touched = c->d_touched;
@@ -1303,8 +1455,6 @@ static TileDev tile_dev(const bf_ctx* c) {
   return td;
 }
 
-// LDS of the single-pass staged kernel for tiles of T elements.
-static size_t tile_smem_bytes(int T) { return 6 * sizeof(double) * (size_t)tile_node_cap(T); }
 
 // One force evaluation on stream s.  `view` / `tview` (optional) replace the context's own
 // arrays: bf_force_eval_device_noalloc points the per-evaluation scratch into the static
@@ -1341,13 +1491,15 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     if (!staged) {
       BF_HIP(launch_element_exact(dm, kp, x, y, z, c->n_cu, s));
     } else {
-      const size_t smem = tile_smem_bytes(td.T);
+      const int P = kp.n_qp;
+      if (td.T * P > kTileMaxThreads) return BF_ERR_STATE;  // tiles not refitted to n_qp
+      const size_t smem = tile_smem_bytes(td.T, P, td.NS);
       if (precision == BF_PREC_FP64_FAST)
-        hipLaunchKernelGGL(bf_tile_kernel<double>, dim3(td.n_tiles), dim3(td.T), smem, s, dm, kp,
-                           td, x, y, z, fout, acc, c->n_v);
+        hipLaunchKernelGGL(bf_tile_kernel<double>, dim3(td.n_tiles), dim3(td.T * P), smem, s, dm,
+                           kp, td, x, y, z, fout, acc, c->n_v);
       else
-        hipLaunchKernelGGL(bf_tile_kernel<float>, dim3(td.n_tiles), dim3(td.T), smem, s, dm, kp,
-                           td, x, y, z, fout, acc, c->n_v);
+        hipLaunchKernelGGL(bf_tile_kernel<float>, dim3(td.n_tiles), dim3(td.T * P), smem, s, dm,
+                           kp, td, x, y, z, fout, acc, c->n_v);
     }
   }
   BF_HIP(hipEventRecord(e1, s));
@@ -1736,11 +1888,13 @@ struct bf_batch {
   std::vector<bf_ctx*> cases;
   int P = 1;
   // launch shape of the element phase
-  int passes = 4;       // element groups per work item: tile_cap = passes * floor(64 / P)
-  int launch_set = 0;   // 0: passes follows the case set (batch_build), 1: set or tuned
-  int block = 256;      // threads per block of the staged kernel
-  int blocks = 0;       // 0: persistent default (waves for every SIMD at the kernel's occupancy)
-  int persistent = 1;
+  int passes = 1;       // kept for bf_batch_get_launch
+  int launch_set = 0;   // 0: default tile size, 1: set or tuned
+  int block = kTileElemsDefault;  // elements per tile requested for every case (64 or 128)
+  int blocks = 0;       // kept for bf_batch_get_launch
+  int persistent = 0;   // kept for bf_batch_get_launch
+  int tile_threads = 0;   // threads per workgroup of the staged kernel: max over cases T n_qp
+  size_t tile_smem = 0;   // its LDS: max over cases
   // device state
   CaseDev* d_cases = nullptr;
   Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr, *d_xtiles = nullptr,
@@ -1797,19 +1951,27 @@ int batch_build(bf_batch* b) {
   }
   b->P = P;
   (void)E_all;
-  // Single-pass staged kernel: every case tiled with the batch's tile size (`block`
-  // threads, one element each), work items (case, tile); hand-over segments per tile wave
-  // (64 entries), which the FP32-mixed second tier and the compaction walk.
-  const int T = b->launch_set ? b->block : kTileElemsDefault;
-  b->block = T;
-  for (bf_ctx* c : b->cases)
+  // Single-pass staged kernel: every case tiled with the requested tile size as a single
+  // context of its n_qp would be (tile_fit), so a case's node sums have the same order in
+  // the batch and alone; work items (case, tile), one per workgroup of the largest T n_qp
+  // threads; hand-over segments per tile wave (64 entries), which the FP32-mixed second
+  // tier and the compaction walk.
+  const int Treq = (b->launch_set ? b->block : kTileElemsDefault) >= 128 ? 128 : 64;
+  b->block = Treq;
+  b->tile_threads = 64;
+  b->tile_smem = 0;
+  long long n_tw = 0;
+  for (bf_ctx* c : b->cases) {
+    const int T = tile_fit(Treq, c->kp.n_qp, c->NS);
     if (c->tileT != T) {
       const int rc = build_ctx_tiles(c, T);
       if (rc != BF_OK) return rc;
     }
+    b->tile_threads = std::max(b->tile_threads, T * c->kp.n_qp);
+    b->tile_smem = std::max(b->tile_smem, tile_smem_bytes(T, c->kp.n_qp, c->NS));
+    n_tw += (long long)c->n_tiles * (T / 64);
+  }
   b->tile_cap = 64;
-  long long n_tw = 0;
-  for (bf_ctx* c : b->cases) n_tw += (long long)c->n_tiles * (T / 64);
   int rc;
   if ((rc = dalloc(&b->d_tlist, (size_t)std::max<long long>(n_tw, 1) * 64)) != BF_OK) return rc;
   std::vector<CaseDev> cases((size_t)nc);
@@ -1826,10 +1988,10 @@ int batch_build(bf_batch* b) {
     cd.td.wcount = nullptr;  // set below, once the count array exists
     for (int t = 0; t < c->n_tiles; t++) {
       tt.push_back(Tile{i, t, 0, 0});
-      for (int w = 0; w < T / 64; w++) et.push_back(Tile{i, t, w, 0});
+      for (int w = 0; w < c->tileT / 64; w++) et.push_back(Tile{i, t, w, 0});
     }
     case_wave[i] = wave_base;
-    wave_base += (long long)c->n_tiles * (T / 64);
+    wave_base += (long long)c->n_tiles * (c->tileT / 64);
     cd.x = c->d_x;
     cd.y = c->d_y;
     cd.z = c->d_z;
@@ -1905,7 +2067,6 @@ BatchDev batch_dev(const bf_batch* b) {
   d.n_xtiles = b->n_xtiles;
   d.ttiles = b->d_ttiles;
   d.n_ttiles = b->n_ttiles;
-  d.tileT = b->block;
   d.tlist = b->d_tlist;
   d.tcount = b->d_tcount;
   d.hlist = b->d_hlist;
@@ -1931,15 +2092,12 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
     return kernel_launch_check();
   }
   if (b->n_ttiles > 0) {
-    // the tile kernel holds two waves per SIMD: 8 per CU
-    const int wpb = d.tileT / 64;
-    int blocks = b->blocks > 0 ? b->blocks : (b->n_cu * 8 + wpb - 1) / wpb;
-    if (!b->persistent) blocks = b->n_ttiles;
-    const size_t smem = tile_smem_bytes(d.tileT);
+    const int nt = b->tile_threads, blocks = b->n_ttiles;
+    const size_t smem = b->tile_smem;
     if (precision == BF_PREC_FP64_FAST) {
-      hipLaunchKernelGGL(bf_batch_tile_kernel<double>, dim3(blocks), dim3(d.tileT), smem, s, d);
+      hipLaunchKernelGGL(bf_batch_tile_kernel<double>, dim3(blocks), dim3(nt), smem, s, d);
     } else {
-      hipLaunchKernelGGL(bf_batch_tile_kernel<float>, dim3(blocks), dim3(d.tileT), smem, s, d);
+      hipLaunchKernelGGL(bf_batch_tile_kernel<float>, dim3(blocks), dim3(nt), smem, s, d);
       // one wave per tile wave (its segment), persistent over the work list
       hipLaunchKernelGGL(bf_batch_tier2_kernel, dim3(b->n_cu * 2), dim3(kStagedBlock), 0, s, d);
     }
@@ -2205,16 +2363,11 @@ int bf_batch_tune(bf_batch* b, int precision, int reps, int* passes, int* block,
   if (!b || reps < 1) return BF_ERR_ARG;
   if (b->cases.empty()) return BF_ERR_STATE;
   BF_HIP(hipSetDevice(b->device));
-  // tile size (elements = threads per workgroup) and grid of the single-pass staged kernel:
-  // one workgroup per tile, or persistent workgroups for one or two waves per SIMD
+  // tile size of the single-pass staged kernel (elements per tile; one tile per workgroup)
   struct Cand { int passes, block, blocks, persistent; };
   std::vector<Cand> cands;
-  const int block_opts[] = {64, 128, 256};
-  for (int bl : block_opts) {
-    const int wpb = bl / 64;
-    cands.push_back({1, bl, 0, 0});
-    for (int wps : {1, 2}) cands.push_back({1, bl, (b->n_cu * 4 * wps + wpb - 1) / wpb, 1});
-  }
+  const int block_opts[] = {64, 128};
+  for (int bl : block_opts) cands.push_back({1, bl, 0, 0});
   float best = 1e30f;
   Cand bc{b->passes, b->block, b->blocks, b->persistent};
   b->launch_set = 1;

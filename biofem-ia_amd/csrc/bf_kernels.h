@@ -49,13 +49,14 @@ struct DevMesh {
 struct TileDev {
   int n_tiles, NS, T, lmax;     // tiles, patch slots, elements (threads) per tile, node cap
   const int* elem;              // [n_tiles * T] element of each lane, -1 if none
-  const unsigned short* lslot;  // [n_tiles][NS][T] local node | phase << 12, 0xFFFF unused
-  const unsigned char* nphase;  // [n_tiles][NS] phases of each slot
+  const unsigned short* lslot;  // [n_tiles][NS][T] local node of each slot, 0xFFFF unused
   const int* node_ptr;          // [n_tiles + 1] range of each tile's local nodes
   const int* node;              // global node of each local node
   const int* out;               // -1 complete (the tile writes fint), -2 none, >= 0 partial
-  const int* pptr;              // [n_v + 1] partial slots of each real node, tile order
-  const int* plist;
+  const int* cptr;              // [local nodes + 1] range of each local node in cent
+  const unsigned short* cent;   // contributions: lane | slot << 8, lane order then slot
+  const int* pptr;              // [n_v + 1] partial slots of each real node (consecutive,
+                                // tile order)
   const unsigned char* complete;  // [n_v] 1: one tile holds every contribution
   // per evaluation
   double* partial;              // [n_partials][3] tile partial sums
@@ -190,7 +191,6 @@ struct BatchDev {
   int n_xtiles;
   const Tile* ttiles;     // element tiles of the single-pass staged kernel: (case, tile)
   int n_ttiles;
-  int tileT;              // elements (threads) per tile, the same for every case
   int* tlist;             // [n_etiles * tile_cap] hand-over segments
   int* tcount;            // [n_etiles]
   int2* hlist;            // compacted hand-over list: (case, element)
@@ -289,35 +289,6 @@ static __constant__ double c_Nab[3][12] = {
 // Each kernel translation unit holds its own copy (no relocatable device code).
 static __constant__ double c_IN[13][5][kMaxNen];
 
-
-// The element tiles this wave works on, [t0, t1) with stride `step`.  Persistent launches
-// give each XCD one contiguous share of the work list, strided over that XCD's waves; a
-// launch with one tile per wave takes the tile of its XCD-remapped wave index.
-__device__ __forceinline__ void batch_tile_range(const BatchDev& b, int& t0, int& t1, int& step) {
-  const int wpb = (int)(blockDim.x >> 6);
-  const int wib = (int)(threadIdx.x >> 6);
-  if (!b.persistent) {
-    const int w = __builtin_amdgcn_readfirstlane(logical_block() * wpb + wib);
-    t0 = w;
-    t1 = (w < b.n_etiles) ? w + 1 : w;
-    step = 1;
-    return;
-  }
-  const int nb = (int)gridDim.x, bid = (int)blockIdx.x;
-  if (nb < kXcds) {  // too few blocks to give every XCD a share: plain stride
-    t0 = __builtin_amdgcn_readfirstlane(bid * wpb + wib);
-    t1 = b.n_etiles;
-    step = nb * wpb;
-    return;
-  }
-  const int x = bid % kXcds, k = bid / kXcds;
-  const int nbx = nb / kXcds + (x < nb % kXcds ? 1 : 0);     // blocks on this XCD
-  const long long T = b.n_etiles;
-  const int lo = (int)(T * x / kXcds), hi = (int)(T * (x + 1) / kXcds);
-  t0 = __builtin_amdgcn_readfirstlane(lo + k * wpb + wib);
-  t1 = hi;
-  step = nbx * wpb;
-}
 
 // Launchers of the kernels compiled without floating-point contraction (bf_exact.hip).
 hipError_t upload_in_tables_exact(const double* tab, size_t bytes);

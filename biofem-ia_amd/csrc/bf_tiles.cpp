@@ -29,8 +29,10 @@ int build_tiles(int E, int n_v, int n_nodes, int NS, const int* patch, const int
   T.NS = NS;
   T.T = TE;
   const int kTileElems = TE, kTileNodes = tile_node_cap(TE);
+  if (TE != 64 && TE != 128) return -1;
   if (E <= 0) {
     T.node_ptr.assign(1, 0);
+    T.cptr.assign(1, 0);
     T.pptr.assign((size_t)std::max(n_v, 0) + 1, 0);
     T.complete.assign((size_t)std::max(n_v, 0), 0);
     return 0;
@@ -75,29 +77,37 @@ int build_tiles(int E, int n_v, int n_nodes, int NS, const int* patch, const int
   std::vector<int> tile_elems;
   std::vector<int> node_tiles((size_t)n_nodes, 0);  // tiles that hold a node
   T.node_ptr.push_back(0);
+  T.cptr.push_back(0);
+  std::vector<int> cnt;
   auto close_tile = [&]() {
     const int t = T.n_tiles++;
     T.elem.resize((size_t)T.n_tiles * kTileElems, -1);
     T.lslot.resize((size_t)T.n_tiles * NS * kTileElems, kTileIdle);
-    T.nphase.resize((size_t)T.n_tiles * NS, 0);
     for (int l = 0; l < (int)tile_elems.size(); l++) T.elem[(size_t)t * kTileElems + l] = tile_elems[l];
-    // phases per slot: lanes in order, the first phase in which the node is still free
-    std::vector<uint16_t> used((size_t)kTilePhases * tile_nodes.size(), 0);
-    for (int a = 0; a < NS; a++) {
-      std::fill(used.begin(), used.end(), 0);
-      int nph = 0;
-      for (int l = 0; l < (int)tile_elems.size(); l++) {
-        const int e = tile_elems[l];
-        if (a >= nen[e] || patch[(size_t)a * E + e] < 0) continue;
-        const int ln = loc[patch[(size_t)a * E + e]];
-        int ph = 0;
-        while (ph < kTilePhases && used[(size_t)ph * tile_nodes.size() + ln]) ph++;
-        if (ph == kTilePhases) return false;
-        used[(size_t)ph * tile_nodes.size() + ln] = 1;
-        nph = std::max(nph, ph + 1);
-        T.lslot[((size_t)t * NS + a) * kTileElems + l] = (uint16_t)(ln | (ph << 12));
+    // slot codes; contributions of each local node in lane order, then slot order
+    const int L = (int)tile_nodes.size();
+    cnt.assign((size_t)L + 1, 0);
+    for (int l = 0; l < (int)tile_elems.size(); l++) {
+      const int e = tile_elems[l];
+      for (int a = 0; a < nen[e]; a++) {
+        const int n = patch[(size_t)a * E + e];
+        if (n < 0) continue;
+        const int ln = loc[n];
+        T.lslot[((size_t)t * NS + a) * kTileElems + l] = (uint16_t)ln;
+        cnt[ln + 1]++;
       }
-      T.nphase[(size_t)t * NS + a] = (uint8_t)nph;
+    }
+    for (int k = 0; k < L; k++) cnt[k + 1] += cnt[k];
+    const size_t c0 = T.cent.size();
+    T.cent.resize(c0 + (size_t)cnt[L]);
+    for (int k = 0; k < L; k++) T.cptr.push_back((int)c0 + cnt[k + 1]);
+    for (int l = 0; l < (int)tile_elems.size(); l++) {
+      const int e = tile_elems[l];
+      for (int a = 0; a < nen[e]; a++) {
+        const int n = patch[(size_t)a * E + e];
+        if (n < 0) continue;
+        T.cent[c0 + cnt[loc[n]]++] = (uint16_t)(l | (a << 8));
+      }
     }
     for (int n : tile_nodes) {
       T.node.push_back(n);
@@ -107,7 +117,6 @@ int build_tiles(int E, int n_v, int n_nodes, int NS, const int* patch, const int
     T.node_ptr.push_back((int)T.node.size());
     tile_nodes.clear();
     tile_elems.clear();
-    return true;
   };
   for (int i = 0; i < E; i++) {
     const int e = order[i];
@@ -123,7 +132,7 @@ int build_tiles(int E, int n_v, int n_nodes, int NS, const int* patch, const int
     }
     if (!tile_elems.empty() &&
         ((int)tile_elems.size() == kTileElems || (int)tile_nodes.size() + fresh > kTileNodes))
-      if (!close_tile()) return -1;
+      close_tile();
     for (int a = 0; a < nen[e]; a++) {
       const int n = patch[(size_t)a * E + e];
       if (n < 0) continue;
@@ -134,9 +143,9 @@ int build_tiles(int E, int n_v, int n_nodes, int NS, const int* patch, const int
     }
     tile_elems.push_back(e);
   }
-  if (!close_tile()) return -1;
+  close_tile();
 
-  // ---- outputs: complete nodes, partial slots in tile order ----
+  // ---- outputs: complete nodes, partial slots (consecutive per node, tile order) ----
   T.out.assign(T.node.size(), -1);
   T.complete.assign((size_t)n_v, 0);
   T.pptr.assign((size_t)n_v + 1, 0);
@@ -146,19 +155,14 @@ int build_tiles(int E, int n_v, int n_nodes, int NS, const int* patch, const int
   }
   for (int n = 0; n < n_v; n++) T.pptr[n + 1] += T.pptr[n];
   T.n_partials = T.pptr[n_v];
-  T.plist.assign((size_t)T.n_partials, 0);
   std::vector<int> fill(T.pptr.begin(), T.pptr.end() - 1);
-  int next = 0;
   for (int t = 0; t < T.n_tiles; t++)
     for (int j = T.node_ptr[t]; j < T.node_ptr[t + 1]; j++) {
       const int n = T.node[j];
-      if (n >= n_v) {
+      if (n >= n_v)
         T.out[j] = -2;
-      } else if (node_tiles[n] > 1) {
-        T.out[j] = next;
-        T.plist[fill[n]++] = next;
-        next++;
-      }
+      else if (node_tiles[n] > 1)
+        T.out[j] = fill[n]++;
     }
   return 0;
 }

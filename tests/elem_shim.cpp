@@ -175,13 +175,13 @@ extern "C" int shim_finternal(int n_v, int n_nodes, int n_elmt, int stride, cons
 
 // ---- the single-pass kernel's element tiling (biofem-ia_amd/csrc/bf_tiles.cpp), exported
 // for the CPU checks of its index invariants.  Call with null outputs to get the sizes:
-// sizes = {n_tiles, local nodes, n_partials}.
+// sizes = {n_tiles, local nodes, n_partials, contributions}.
 #include "../biofem-ia_amd/csrc/bf_tiles.h"
 
 extern "C" int shim_tiles(int E, int n_v, int n_nodes, int NS, const int* patch, const int* nen,
                           const double* x0, const double* y0, const double* z0, int T,
-                          int* sizes, int* elem, unsigned short* lslot, unsigned char* nphase,
-                          int* node_ptr, int* node, int* out, int* pptr, int* plist,
+                          int* sizes, int* elem, unsigned short* lslot, int* node_ptr, int* node,
+                          int* out, int* cptr, unsigned short* cent, int* pptr,
                           unsigned char* complete) {
   bf::TileHost th;
   const int rc = bf::build_tiles(E, n_v, n_nodes, NS, patch, nen, x0, y0, z0, T, th);
@@ -189,15 +189,16 @@ extern "C" int shim_tiles(int E, int n_v, int n_nodes, int NS, const int* patch,
   sizes[0] = th.n_tiles;
   sizes[1] = (int)th.node.size();
   sizes[2] = th.n_partials;
+  sizes[3] = (int)th.cent.size();
   if (!elem) return 0;
   std::copy(th.elem.begin(), th.elem.end(), elem);
   std::copy(th.lslot.begin(), th.lslot.end(), lslot);
-  std::copy(th.nphase.begin(), th.nphase.end(), nphase);
   std::copy(th.node_ptr.begin(), th.node_ptr.end(), node_ptr);
   std::copy(th.node.begin(), th.node.end(), node);
   std::copy(th.out.begin(), th.out.end(), out);
+  std::copy(th.cptr.begin(), th.cptr.end(), cptr);
+  std::copy(th.cent.begin(), th.cent.end(), cent);
   std::copy(th.pptr.begin(), th.pptr.end(), pptr);
-  std::copy(th.plist.begin(), th.plist.end(), plist);
   std::copy(th.complete.begin(), th.complete.end(), complete);
   return 0;
 }

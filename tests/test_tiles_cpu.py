@@ -1,8 +1,9 @@
 """The element tiling of the single-pass staged kernel (biofem-ia_amd/csrc/bf_tiles.cpp),
-checked on the CPU: every index the kernels follow stays inside its tile and buffers, the
-phases of a slot never share a node, and replaying the kernels' summation order (tile
-accumulators in (slot, phase, lane) order, complete nodes written by their tile, partial
-nodes summed over tiles in tile order) reproduces the direct node sums of arbitrary element
+checked on the CPU: every index the kernels follow stays inside its tile and buffers, every
+(element, slot) contribution is listed exactly once, under the local node its slot code
+names, and replaying the kernels' summation order (each local node's contributions in list
+order, complete nodes written by their tile, partial nodes summed over their consecutive
+tile partials in tile order) reproduces the direct node sums of arbitrary element
 contributions."""
 import ctypes as C
 import os
@@ -58,19 +59,19 @@ def tiles(shim, m, T):
     dp = C.POINTER(C.c_double)
     args = [E, m.n_v, m.n_nodes, NS, P.ctypes.data_as(ip), nen.ctypes.data_as(ip),
             m.x0.ctypes.data_as(dp), m.y0.ctypes.data_as(dp), m.z0.ctypes.data_as(dp), T]
-    sizes = np.zeros(3, np.int32)
+    sizes = np.zeros(4, np.int32)
     assert shim.shim_tiles(*args, sizes.ctypes.data_as(ip), *([None] * 9)) == 0
-    nt, nl, npart = (int(v) for v in sizes)
+    nt, nl, npart, nc = (int(v) for v in sizes)
     out = dict(elem=np.zeros(nt * T, np.int32), lslot=np.zeros(nt * NS * T, np.uint16),
-               nphase=np.zeros(nt * NS, np.uint8), node_ptr=np.zeros(nt + 1, np.int32),
-               node=np.zeros(max(nl, 1), np.int32), out=np.zeros(max(nl, 1), np.int32),
-               pptr=np.zeros(m.n_v + 1, np.int32), plist=np.zeros(max(npart, 1), np.int32),
+               node_ptr=np.zeros(nt + 1, np.int32), node=np.zeros(max(nl, 1), np.int32),
+               out=np.zeros(max(nl, 1), np.int32), cptr=np.zeros(nl + 1, np.int32),
+               cent=np.zeros(max(nc, 1), np.uint16), pptr=np.zeros(m.n_v + 1, np.int32),
                complete=np.zeros(max(m.n_v, 1), np.uint8))
     assert shim.shim_tiles(*args, sizes.ctypes.data_as(ip),
                            *[out[k].ctypes.data_as(C.c_void_p) for k in
-                             ("elem", "lslot", "nphase", "node_ptr", "node", "out", "pptr",
-                              "plist", "complete")]) == 0
-    out.update(n_tiles=nt, NS=NS, n_partials=npart, P=P, nen=nen)
+                             ("elem", "lslot", "node_ptr", "node", "out", "cptr", "cent", "pptr",
+                              "complete")]) == 0
+    out.update(n_tiles=nt, NS=NS, n_partials=npart, n_cent=nc, P=P, nen=nen)
     return out
 
 
@@ -86,7 +87,7 @@ def meshes():
     return ms
 
 
-@pytest.mark.parametrize("T", [64, 128, 256])
+@pytest.mark.parametrize("T", [64, 128])
 @pytest.mark.parametrize("name", ["flat 64x48", "ellipsoid L4", "lv cap"])
 def test_tiling_invariants_and_summation(shim, name, T):
     m = meshes()[name]
@@ -95,53 +96,60 @@ def test_tiling_invariants_and_summation(shim, name, T):
     P, nen = t["P"], t["nen"]
     elem = t["elem"].reshape(nt, T)
     lslot = t["lslot"].reshape(nt, NS, T)
-    nph = t["nphase"].reshape(nt, NS)
-    nptr, node, out = t["node_ptr"], t["node"], t["out"]
+    nptr, node, out, cptr, cent = t["node_ptr"], t["node"], t["out"], t["cptr"], t["cent"]
     # every element in exactly one lane
     live = elem[elem >= 0]
     assert np.array_equal(np.sort(live), np.arange(E))
+    assert cptr[0] == 0 and cptr[-1] == t["n_cent"] and (np.diff(cptr) >= 0).all()
     rng = np.random.default_rng(7)
     contrib = rng.standard_normal((E, NS, 3))           # element e's force on slot a
-    partial = np.zeros((max(t["n_partials"], 1), 3))
+    partial = np.full((max(t["n_partials"], 1), 3), np.nan)
     fint = np.full((m.n_nodes, 3), np.nan)
     for k in range(nt):
         j0, L = nptr[k], nptr[k + 1] - nptr[k]
-        assert 0 < L <= node_cap(T) and L < 4096
+        assert 0 < L <= node_cap(T)
         loc_nodes = node[j0:j0 + L]
         assert len(set(loc_nodes.tolist())) == L
-        acc = np.zeros((L, 3))
+        used = lslot[k] != IDLE
         for a in range(NS):
-            codes = lslot[k, a]
-            used = codes != IDLE
             for lane in range(T):
                 e = elem[k, lane]
-                assert used[lane] == (e >= 0 and a < nen[e] and P[a, e] >= 0)
-            l = codes & 0xFFF
-            ph = codes >> 12
-            assert (l[used] < L).all() and (ph[used] < nph[k, a]).all()
-            assert np.array_equal(loc_nodes[l[used]], P[a, elem[k, used]])
-            for p in range(int(nph[k, a])):
-                sel = used & (ph == p)
-                assert len(set(l[sel].tolist())) == int(sel.sum())   # no shared node
-                for lane in np.nonzero(sel)[0]:                      # lane order
-                    acc[l[lane]] += contrib[elem[k, lane], a]
+                assert used[a, lane] == (e >= 0 and a < nen[e] and P[a, e] >= 0)
+        assert (lslot[k][used] < L).all()
+        for a in range(NS):
+            assert np.array_equal(loc_nodes[lslot[k, a][used[a]]], P[a, elem[k, used[a]]])
+        # the node lists hold every used (lane, slot) once, under its slot's local node,
+        # in lane order, then slot order
+        seen = set()
         for li in range(L):
+            ent = cent[cptr[j0 + li]:cptr[j0 + li + 1]]
+            lanes, slots = ent & 0xFF, ent >> 8
+            assert (lanes < T).all() and (slots < NS).all()
+            key = lanes.astype(np.int64) * 64 + slots
+            assert (np.diff(key) > 0).all()
+            for lane, a in zip(lanes, slots):
+                assert lslot[k, a, lane] == li
+                seen.add((int(lane), int(a)))
+            acc = np.zeros(3)
+            for lane, a in zip(lanes, slots):                          # list order
+                acc = acc + contrib[elem[k, lane], a]
             n, o = loc_nodes[li], out[j0 + li]
             if n >= m.n_v:
                 assert o == -2
             elif o == -1:
                 assert t["complete"][n] == 1
-                fint[n] = acc[li]
+                fint[n] = acc
             else:
-                assert o >= 0 and t["complete"][n] == 0
-                partial[o] = acc[li]
-    # node pass: partial nodes in tile order
-    pptr, plist = t["pptr"], t["plist"]
+                assert t["pptr"][n] <= o < t["pptr"][n + 1] and t["complete"][n] == 0
+                partial[o] = acc
+        assert seen == {(int(lane), int(a)) for a, lane in zip(*np.nonzero(used))}
+    # node pass: a partial node's consecutive partials, in tile order
+    pptr = t["pptr"]
     for n in range(m.n_v):
         if pptr[n + 1] > pptr[n]:
             f = np.zeros(3)
             for j in range(pptr[n], pptr[n + 1]):
-                f = f + partial[plist[j]]
+                f = f + partial[j]
             fint[n] = f
     direct = np.zeros((m.n_nodes, 3))
     valid = (np.arange(NS)[:, None] < nen[None, :]) & (P >= 0)
@@ -154,5 +162,4 @@ def test_tiling_invariants_and_summation(shim, name, T):
     touched[P[valid]] = True
     chk = real & touched
     np.testing.assert_allclose(fint[chk], direct[chk], rtol=1e-12, atol=1e-12)
-    print(f"  {name} T={T}: {nt} tiles, {len(node)} local nodes, {t['n_partials']} partials, "
-          f"max phases {int(nph.max())}")
+    print(f"  {name} T={T}: {nt} tiles, {len(node)} local nodes, {t['n_partials']} partials")

@@ -204,28 +204,31 @@ __device__ __forceinline__ void exact_solve(const DevMesh& m, const KParams& kp,
 // the time of one wave, not by the device's throughput: with one (element, qp) per lane
 // most SIMDs idle while the waves that hold elements add their 6561 terms one by one.
 // Here K lanes share one solve.  In round r lane j forms the 81 terms of block r K + j of
-// the CC^{2222} sum (all lanes at once) and stores them in LDS; the group's first lane adds
-// its own block's terms as they are formed, then the other K - 1 blocks' terms from LDS,
-// in block order -- the same additions in the same order as cc_sum_serial, so the sum is
-// bit for bit the serial one.  Everything else of the iteration is formed by all K lanes
-// alike.  A wave holds kSplitLanes / K groups (kSplitLanes = 60); its term buffer is
-// [81][groups][K] doubles, kSplitLanes x 648 bytes, so four one-wave blocks fit a CU's
-// 160 KB of LDS.
+// the CC^{2222} sum into its own registers (all lanes at once); then the running sum
+// travels through the group: lane 0 adds its block's terms in order and hands the sum to
+// lane 1 (a DPP shift by one lane), which adds its own, and so on to lane K - 1, which hands
+// it back to lane 0 for the next round.  These are the additions of cc_sum_serial in its
+// order, so the sum is bit for bit the serial one, and every addition reads its term from
+// the adding lane's registers (a dependent FP64 add is ~5 cycles there, ~16 when its term
+// comes from another lane through LDS).  Everything else of the iteration is formed by all
+// K lanes alike.  A wave holds split_groups(K) = 64 / K groups.
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Lane i receives lane i - 1's v (lane 0 keeps its own): DPP wave_shr:1, the whole-wave
+// shift of GFX9's data-parallel primitives.
+__device__ __forceinline__ double from_lane_below(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x138, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x138, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
 }
 
 template <int K>
 struct CcSplit {
-  double* buf;  // the wave's term buffer
-  int g, j;     // group within the wave, lane within the group
+  int j;  // lane within the group
   __device__ double operator()(const exact::CcIn& c) const {
-    constexpr int NG = kSplitLanes / K;
     const int lead = (threadIdx.x & 63) - j;
-    double CC = 0.0;  // the lead's is the sum; the other lanes' is not used
+    double CC = 0.0;  // the running sum, valid in the lane whose block comes next
 #pragma unroll 1
     for (int b0 = 0; b0 < 81; b0 += K) {
       const int b = b0 + j;
@@ -235,33 +238,20 @@ struct CcSplit {
         k = exact::cc_block_at(c, b);
         act = exact::cc_block_active(c, k);
       }
-      const unsigned long long bal = __ballot(act);
-      if (bal == 0ull) continue;
-      if (act)
-        exact::cc_block_terms(c, k, [&](int i, double t) {
-          CC += t;
-          buf[(i * NG + g) * K + j] = t;
-        });
-      wave_lds_sync();
-      // the group's active blocks (64-bit only for groups wider than 31 lanes)
-      using Mask = typename std::conditional<(K > 31), unsigned long long, unsigned>::type;
-      const Mask grp = (Mask)(bal >> lead) & (Mask)(((Mask)1 << K) - (Mask)1);
-      // only the leads read: a quarter (K = 4) of the LDS traffic of a full wave
+      if (__ballot(act) == 0ull) continue;
+      double t[81];
+#pragma unroll
+      for (int i = 0; i < 81; i++) t[i] = 0.0;
+      if (act) exact::cc_block_terms(c, k, [&](int i, double v) { t[i] = v; });
 #pragma unroll 1
-      for (int sl = 1; sl < K; sl++) {
-        if (j != 0 || !((grp >> sl) & 1u)) continue;
-        const double* src = buf + g * K + sl;
-        // all 81 loads in flight before the first add: the LDS latency is paid once per
-        // block, and the dependent FP64 adds (about 16 cycles each) set the pace
-        double v[81];
+      for (int sl = 0; sl < K; sl++) {
+        if (j == sl && act) {
 #pragma unroll
-        for (int i = 0; i < 81; i++) v[i] = src[i * NG * K];
-#pragma unroll
-        for (int i = 0; i < 81; i++) CC += v[i];
-        __builtin_amdgcn_sched_group_barrier(0x100, 81, 0);
-        __builtin_amdgcn_sched_group_barrier(0x002, 81, 0);
+          for (int i = 0; i < 81; i++) CC += t[i];
+        }
+        if (sl + 1 < K) CC = from_lane_below(CC);
       }
-      wave_lds_sync();
+      CC = __shfl(CC, lead + K - 1);  // back to lane 0 (every lane of the group holds it)
     }
     return __shfl(CC, lead);
   }
@@ -297,7 +287,7 @@ __device__ __forceinline__ void exact_solve_k(const DevMesh& m, const KParams& k
                                               const double* __restrict__ x,
                                               const double* __restrict__ y,
                                               const double* __restrict__ z, int e, int q,
-                                              double* buf, int g, int j) {
+                                              int j) {
   if constexpr (K == 1) {
     exact_solve(m, kp, x, y, z, e, q);
   } else {
@@ -305,7 +295,7 @@ __device__ __forceinline__ void exact_solve_k(const DevMesh& m, const KParams& k
     exact::ElemIn in;
     exact_inputs(m, kp, x, y, z, e, in, true);
     const exact::QpOut r = exact::integrate_qp(in, q, [](bool fail, int) { return fail; },
-                                               CcSplit<K>{buf, g, j}, true);
+                                               CcSplit<K>{j}, true);
     if (j != 0) return;
     m.c33[(size_t)q * E + e] = r.c33;
     double* o = m.qsv + 3 * ((size_t)q * E + e);
@@ -453,8 +443,8 @@ __device__ __forceinline__ int next_item(int* counter) {
 template <int K>
 __device__ void solve_list_k(const DevMesh& m, const KParams& kp, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
-                             const int* __restrict__ list, int N, int* counter, double* buf) {
-  constexpr int NG = K == 1 ? 64 : kSplitLanes / K;
+                             const int* __restrict__ list, int N, int* counter) {
+  constexpr int NG = split_groups(K);
   const int P = kp.n_qp;
   const int chunks = (N + NG - 1) / NG;
   const int items = chunks * P;
@@ -462,7 +452,7 @@ __device__ void solve_list_k(const DevMesh& m, const KParams& kp, const double* 
   const int g = lane / K, j = lane % K;
   for (int i = wave_id(); i < items; i = next_item(counter)) {
     const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * NG + g;
-    if (g < NG && idx < N) exact_solve_k<K>(m, kp, x, y, z, list[idx], q, buf, g, j);
+    if (g < NG && idx < N) exact_solve_k<K>(m, kp, x, y, z, list[idx], q, j);
   }
 }
 
@@ -472,14 +462,13 @@ __global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
     DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
     int* __restrict__ counter, int n_waves) {
-  __shared__ double buf[kSplitLanes * 81];
   const int N = *count;
   switch (split_factor(N, kp.n_qp, n_waves, false)) {
-    case 8: solve_list_k<8>(m, kp, x, y, z, list, N, counter, buf); break;
-    case 6: solve_list_k<6>(m, kp, x, y, z, list, N, counter, buf); break;
-    case 4: solve_list_k<4>(m, kp, x, y, z, list, N, counter, buf); break;
-    case 2: solve_list_k<2>(m, kp, x, y, z, list, N, counter, buf); break;
-    default: solve_list_k<1>(m, kp, x, y, z, list, N, counter, buf); break;
+    case 8: solve_list_k<8>(m, kp, x, y, z, list, N, counter); break;
+    case 6: solve_list_k<6>(m, kp, x, y, z, list, N, counter); break;
+    case 4: solve_list_k<4>(m, kp, x, y, z, list, N, counter); break;
+    case 2: solve_list_k<2>(m, kp, x, y, z, list, N, counter); break;
+    default: solve_list_k<1>(m, kp, x, y, z, list, N, counter); break;
   }
 }
 
@@ -491,14 +480,13 @@ __global__ void __launch_bounds__(64) bf_exact_solve_wide_kernel(
     DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
     int* __restrict__ counter, int n_waves, int* __restrict__ narrow_count) {
-  __shared__ double buf[kSplitLanes * 81];
   const int N = *count;
   const int K = N > 0 ? split_factor(N, kp.n_qp, n_waves, true) : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_count = K >= kWideSplitMin ? 0 : N;
   switch (K) {
-    case 60: solve_list_k<60>(m, kp, x, y, z, list, N, counter, buf); break;
-    case 30: solve_list_k<30>(m, kp, x, y, z, list, N, counter, buf); break;
-    case 15: solve_list_k<15>(m, kp, x, y, z, list, N, counter, buf); break;
+    case 60: solve_list_k<60>(m, kp, x, y, z, list, N, counter); break;
+    case 30: solve_list_k<30>(m, kp, x, y, z, list, N, counter); break;
+    case 15: solve_list_k<15>(m, kp, x, y, z, list, N, counter); break;
     default: break;  // the queue kernel's
   }
 }
@@ -565,8 +553,8 @@ __global__ void __launch_bounds__(kExactBlock) bf_batch_exact_solve_kernel(Batch
 // Solve over the compacted hand-over list b.hlist[0 .. *b.htotal) as the single-context
 // form (work counter b.rtotal[1]).
 template <int K>
-__device__ void batch_solve_list_k(const BatchDev& b, int N, double* buf) {
-  constexpr int NG = K == 1 ? 64 : kSplitLanes / K;
+__device__ void batch_solve_list_k(const BatchDev& b, int N) {
+  constexpr int NG = split_groups(K);
   const int P = b.P;
   const int chunks = (N + NG - 1) / NG;
   const int items = chunks * P;
@@ -577,19 +565,18 @@ __device__ void batch_solve_list_k(const BatchDev& b, int N, double* buf) {
     if (g >= NG || idx >= N) continue;
     const int2 v = b.hlist[idx];
     const CaseDev& cd = b.cases[v.x];
-    if (q < cd.kp.n_qp) exact_solve_k<K>(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, q, buf, g, j);
+    if (q < cd.kp.n_qp) exact_solve_k<K>(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, q, j);
   }
 }
 
 __global__ void __launch_bounds__(64) bf_batch_exact_solve_list_kernel(BatchDev b, int n_waves) {
-  __shared__ double buf[kSplitLanes * 81];
   const int N = *b.htotal;
   switch (split_factor(N, b.P, n_waves, false)) {
-    case 8: batch_solve_list_k<8>(b, N, buf); break;
-    case 6: batch_solve_list_k<6>(b, N, buf); break;
-    case 4: batch_solve_list_k<4>(b, N, buf); break;
-    case 2: batch_solve_list_k<2>(b, N, buf); break;
-    default: batch_solve_list_k<1>(b, N, buf); break;
+    case 8: batch_solve_list_k<8>(b, N); break;
+    case 6: batch_solve_list_k<6>(b, N); break;
+    case 4: batch_solve_list_k<4>(b, N); break;
+    case 2: batch_solve_list_k<2>(b, N); break;
+    default: batch_solve_list_k<1>(b, N); break;
   }
 }
 

@@ -304,13 +304,13 @@ hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const doubl
 // split fits when its items fill at most one round of the n_waves persistent waves, the
 // expensive qps (all but the mid-surface point of an odd rule) counted whole and the
 // mid-surface point's at 0.4 of one (measured on flat reference shells, where its zero blocks
-// are skipped).  kSplitLanes lanes of each wave take part.
-constexpr int kSplitLanes = 60;
+// are skipped).  A wave holds split_groups(K) groups of K lanes.
 constexpr int kWideSplitMin = 15;
+__host__ __device__ constexpr int split_groups(int K) { return 64 / K; }
 __host__ __device__ inline bool split_fits(int N, int K, int P, int n_waves) {
   const bool mid = P > 1 && (P & 1);
   const int pe = mid ? P - 1 : P;
-  const long long chunks = (N + kSplitLanes / K - 1) / (kSplitLanes / K);
+  const long long chunks = (N + split_groups(K) - 1) / split_groups(K);
   return chunks * (10 * pe + (mid ? 4 : 0)) <= 10LL * n_waves;
 }
 

@@ -97,6 +97,7 @@ SIGNATURES = {
     "bf_elem_flags_download": (C.c_int, [_vp, C.POINTER(C.c_ubyte)]),
     "bf_last_error": (C.c_int, [_vp, _ip, _ip, _ip]),
     "bf_last_fallback_count": (C.c_int, [_vp, _ip]),
+    "bf_last_handed_qps": (C.c_int, [_vp, C.POINTER(C.c_ubyte), _ip]),
     "bf_set_fallback": (C.c_int, [_vp, C.c_int]),
     "bf_last_timing": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "bf_timing_enable": (C.c_int, [_vp, C.c_int]),

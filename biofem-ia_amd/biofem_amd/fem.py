@@ -104,6 +104,19 @@ class ActStrainFE:
         L.check("bf_last_fallback_count", self.lib.bf_last_fallback_count(self.ctx, C.byref(n)))
         return n.value
 
+    def handed_qps(self) -> np.ndarray:
+        """[n_elmt, n_qp] bool: the (element, qp) pairs the last staged evaluation solved in
+        the reference's operation order (bf_last_handed_qps); elements whose CC3333 guard
+        tripped (FLAG_CC3333) were redone in that order with all their points."""
+        mask = np.zeros(self.mesh.n_elmt, np.uint8)
+        n = C.c_int()
+        L.check("bf_last_handed_qps", self.lib.bf_last_handed_qps(
+            self.ctx, mask.ctypes.data_as(C.POINTER(C.c_ubyte)), C.byref(n)))
+        bits = (mask[:, None] >> np.arange(self.params.n_qp)[None, :]) & 1
+        out = bits.astype(bool)
+        assert int(out.sum()) == n.value
+        return out
+
     def set_fallback(self, on: bool) -> None:
         L.check("bf_set_fallback", self.lib.bf_set_fallback(self.ctx, int(on)))
 

@@ -262,24 +262,46 @@ struct CcSplit {
   }
 };
 
-// Split factor for a hand-over list of N elements: the largest K that fits (split_fits).
-// The fit weights the mid-surface qp because the waves left over by the expensive items must
-// absorb the cheap ones, or a wave that takes a cheap item after an expensive one sets the
-// kernel's time.  K = 60, 30, 15 (short lists, up to ~1.7k elements at n_qp = 3 on 256
-// CUs) run in bf_exact_solve_wide_kernel (`wide`), K = 8, 6, 4, 2, 1 in
-// bf_exact_solve_queue_kernel and the batch list kernel (not `wide`): the two sets compile
-// as separate kernels, so neither's register allocation pays for the other's
-// instantiations (more K in one kernel slowed its common cases).
-__device__ __forceinline__ int split_factor(int N, int P, int n_waves, bool wide) {
+// Split factor for a hand-over list (split_fits): the largest K that fits.  K = 60, 30, 15
+// (short lists) run in bf_exact_solve_wide_kernel (`wide`), K = 8, 6, 4, 2, 1 in
+// bf_exact_solve_queue_kernel and the batch list kernel (not `wide`): the two sets compile as
+// separate kernels, so neither's register allocation pays for the other's instantiations
+// (more K in one kernel slowed its common cases).
+__device__ __forceinline__ int split_factor(int N, int N_mid, int P, bool pairs, int n_waves,
+                                            bool wide) {
   if (wide) {
-    if (split_fits(N, 60, P, n_waves)) return 60;
-    if (split_fits(N, 30, P, n_waves)) return 30;
-    if (split_fits(N, 15, P, n_waves)) return 15;
+    if (split_fits(N, N_mid, P, pairs, 60, n_waves)) return 60;
+    if (split_fits(N, N_mid, P, pairs, 30, n_waves)) return 30;
+    if (split_fits(N, N_mid, P, pairs, 15, n_waves)) return 15;
   }
   for (int K = 8; K >= 2; K -= 2)
-    if (split_fits(N, K, P, n_waves)) return K;
+    if (split_fits(N, N_mid, P, pairs, K, n_waves)) return K;
   return 1;
 }
+
+// Work items of a hand-over list for G groups per wave: an element list (entries are
+// elements, every qp solved; item i = (qp rank i / chunks, chunk of G entries i % chunks)) or
+// a pair list (entries qp_pair(e, q); item i = entries i G ..).  at(i, g) gives the entry
+// and qp of group g of item i, false past the list.
+struct HandItems {
+  int N, P, G;
+  bool pairs;
+  __device__ int n_items() const {
+    const int c = (N + G - 1) / G;
+    return pairs ? c : c * P;
+  }
+  __device__ bool at(int i, int g, int& idx, int& q) const {
+    if (pairs) {
+      idx = i * G + g;
+      q = -1;  // from the entry
+      return idx < N;
+    }
+    const int c = (N + G - 1) / G;
+    idx = (i % c) * G + g;
+    q = qp_of_rank(i / c, P);
+    return idx < N;
+  }
+};
 
 // Solve of qp q of element e by lane j of a K-lane group (K = 1: one lane, serial sum).
 template <int K>
@@ -394,15 +416,7 @@ __device__ __forceinline__ void exact_element(const DevMesh& m, const KParams& k
 }
 
 // ---- single context ----
-// Solve work items are issued qp by qp, the mid-surface point (theta = 0, odd n_qp) last:
-// on shells with a flat reference its blocks of exactly zero terms are skipped and its
-// solve costs a fraction of the others', so the expensive qps start first and the cheap
-// ones fill the tail.
-__device__ __forceinline__ int qp_of_rank(int r, int P) {
-  const int mid = P >> 1;
-  if (!(P & 1)) return r;
-  return r < mid ? r : (r == P - 1 ? mid : r + 1);
-}
+// Solve work items are issued qp by qp in qp_of_rank order (the mid-surface point last).
 
 // Work item i of the solve = (qp rank i / chunks, chunk of 64 list entries i % chunks).
 // list == nullptr: element = entry index.
@@ -438,37 +452,43 @@ __device__ __forceinline__ int next_item(int* counter) {
   return __shfl(i, 0) + gridDim.x * (blockDim.x >> 6);
 }
 
-// Work items of the hand-over solve at split factor K: (qp rank i / chunks, chunk of
-// `groups` list entries i % chunks), wave w first item w, then from the counter.
+// The hand-over solve at split factor K over list[0 .. N) (HandItems): wave w takes item w
+// first, then items from the counter.
 template <int K>
 __device__ void solve_list_k(const DevMesh& m, const KParams& kp, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
-                             const int* __restrict__ list, int N, int* counter) {
+                             const int* __restrict__ list, int N, bool pairs, int* counter) {
   constexpr int NG = split_groups(K);
-  const int P = kp.n_qp;
-  const int chunks = (N + NG - 1) / NG;
-  const int items = chunks * P;
+  const HandItems hi{N, kp.n_qp, NG, pairs};
+  const int items = hi.n_items();
   const int lane = threadIdx.x & 63;
   const int g = lane / K, j = lane % K;
   for (int i = wave_id(); i < items; i = next_item(counter)) {
-    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * NG + g;
-    if (g < NG && idx < N) exact_solve_k<K>(m, kp, x, y, z, list[idx], q, j);
+    int idx, q;
+    if (g < NG && hi.at(i, g, idx, q)) {
+      const int v = list[idx];
+      const int e = pairs ? pair_elem(v) : v;
+      exact_solve_k<K>(m, kp, x, y, z, e, pairs ? pair_qp(v) : q, j);
+    }
   }
 }
 
-// Solve over the hand-over list m.fallback[0 .. *count): one-wave blocks, persistent grid
-// of n_waves waves, split factor from the list length.
+// Solve over the hand-over list: one-wave blocks, persistent grid of n_waves waves, split
+// factor from the list length.  pairs: list = m.qlist, *count pairs (m.status[9] of them
+// mid-surface pairs at the end when *count is the list's full length), else list =
+// m.fallback, *count elements.
 __global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
     DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
-    int* __restrict__ counter, int n_waves) {
-  const int N = *count;
-  switch (split_factor(N, kp.n_qp, n_waves, false)) {
-    case 8: solve_list_k<8>(m, kp, x, y, z, list, N, counter); break;
-    case 6: solve_list_k<6>(m, kp, x, y, z, list, N, counter); break;
-    case 4: solve_list_k<4>(m, kp, x, y, z, list, N, counter); break;
-    case 2: solve_list_k<2>(m, kp, x, y, z, list, N, counter); break;
-    default: solve_list_k<1>(m, kp, x, y, z, list, N, counter); break;
+    int pairs, int* __restrict__ counter, int n_waves) {
+  const int N = count[0];
+  const int N_mid = (pairs && N > 0) ? m.status[9] : 0;
+  switch (split_factor(N, N_mid, kp.n_qp, pairs != 0, n_waves, false)) {
+    case 8: solve_list_k<8>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    case 6: solve_list_k<6>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    case 4: solve_list_k<4>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    case 2: solve_list_k<2>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    default: solve_list_k<1>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
   }
 }
 
@@ -479,14 +499,15 @@ __global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
 __global__ void __launch_bounds__(64) bf_exact_solve_wide_kernel(
     DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
-    int* __restrict__ counter, int n_waves, int* __restrict__ narrow_count) {
+    int pairs, int* __restrict__ counter, int n_waves, int* __restrict__ narrow_count) {
   const int N = *count;
-  const int K = N > 0 ? split_factor(N, kp.n_qp, n_waves, true) : 0;
+  const int N_mid = pairs ? m.status[9] : 0;
+  const int K = N > 0 ? split_factor(N, N_mid, kp.n_qp, pairs != 0, n_waves, true) : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_count = K >= kWideSplitMin ? 0 : N;
   switch (K) {
-    case 60: solve_list_k<60>(m, kp, x, y, z, list, N, counter); break;
-    case 30: solve_list_k<30>(m, kp, x, y, z, list, N, counter); break;
-    case 15: solve_list_k<15>(m, kp, x, y, z, list, N, counter); break;
+    case 60: solve_list_k<60>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    case 30: solve_list_k<30>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    case 15: solve_list_k<15>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
     default: break;  // the queue kernel's
   }
 }
@@ -550,33 +571,38 @@ __global__ void __launch_bounds__(kExactBlock) bf_batch_exact_solve_kernel(Batch
   }
 }
 
-// Solve over the compacted hand-over list b.hlist[0 .. *b.htotal) as the single-context
-// form (work counter b.rtotal[1]).
+// Solve over the compacted hand-over list as the single-context form (work counter
+// b.rtotal[1]): pairs: b.plist[0 .. ptotal[0]) ((case, qp_pair) entries), else
+// b.hlist[0 .. *htotal) ((case, element) entries, every qp below the case's n_qp solved).
 template <int K>
-__device__ void batch_solve_list_k(const BatchDev& b, int N) {
+__device__ void batch_solve_list_k(const BatchDev& b, int N, bool pairs) {
   constexpr int NG = split_groups(K);
-  const int P = b.P;
-  const int chunks = (N + NG - 1) / NG;
-  const int items = chunks * P;
+  const HandItems hi{N, b.P, NG, pairs};
+  const int items = hi.n_items();
+  const int2* list = pairs ? b.plist : b.hlist;
   const int lane = threadIdx.x & 63;
   const int g = lane / K, j = lane % K;
   for (int i = wave_id(); i < items; i = next_item(b.rtotal + 1)) {
-    const int q = qp_of_rank(i / chunks, P), idx = (i % chunks) * NG + g;
-    if (g >= NG || idx >= N) continue;
-    const int2 v = b.hlist[idx];
+    int idx, q;
+    if (g >= NG || !hi.at(i, g, idx, q)) continue;
+    const int2 v = list[idx];
     const CaseDev& cd = b.cases[v.x];
-    if (q < cd.kp.n_qp) exact_solve_k<K>(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, q, j);
+    const int e = pairs ? pair_elem(v.y) : v.y;
+    if (pairs) q = pair_qp(v.y);
+    if (q < cd.kp.n_qp) exact_solve_k<K>(cd.m, cd.kp, cd.x, cd.y, cd.z, e, q, j);
   }
 }
 
-__global__ void __launch_bounds__(64) bf_batch_exact_solve_list_kernel(BatchDev b, int n_waves) {
-  const int N = *b.htotal;
-  switch (split_factor(N, b.P, n_waves, false)) {
-    case 8: batch_solve_list_k<8>(b, N); break;
-    case 6: batch_solve_list_k<6>(b, N); break;
-    case 4: batch_solve_list_k<4>(b, N); break;
-    case 2: batch_solve_list_k<2>(b, N); break;
-    default: batch_solve_list_k<1>(b, N); break;
+__global__ void __launch_bounds__(64) bf_batch_exact_solve_list_kernel(BatchDev b, int pairs,
+                                                                       int n_waves) {
+  const int N = pairs ? b.ptotal[0] : *b.htotal;
+  const int N_mid = pairs ? b.ptotal[1] : 0;
+  switch (split_factor(N, N_mid, b.P, pairs != 0, n_waves, false)) {
+    case 8: batch_solve_list_k<8>(b, N, pairs != 0); break;
+    case 6: batch_solve_list_k<6>(b, N, pairs != 0); break;
+    case 4: batch_solve_list_k<4>(b, N, pairs != 0); break;
+    case 2: batch_solve_list_k<2>(b, N, pairs != 0); break;
+    default: batch_solve_list_k<1>(b, N, pairs != 0); break;
   }
 }
 
@@ -648,11 +674,11 @@ hipError_t launch_batch_exact(const BatchDev& b, int n_cu, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, hipStream_t s) {
+hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, bool pairs, hipStream_t s) {
   if (b.n_etiles <= 0) return hipSuccess;
   // persistent: one wave per SIMD at the solve's occupancy of one
   hipLaunchKernelGGL(bf_batch_exact_solve_list_kernel, dim3(4 * n_cu), dim3(64), 0, s, b,
-                     4 * n_cu);
+                     pairs ? 1 : 0, 4 * n_cu);
   // 64-lane blocks: a short hand-over list still spreads over the CUs
   hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, b, 1,
                      (unsigned char)kStatFallback);
@@ -885,18 +911,20 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, int n_cu, bool wide,
-                                         hipStream_t s) {
+                                         bool pairs, hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
   // persistent grids of one wave per SIMD (the solve's occupancy; the list length is known
   // on the device only).  With `wide` the wide solve kernel runs first and leaves the
   // narrow one its count in status[7] (0 if it took the list); without it the narrow kernel
   // reads the list length itself.
+  const int* list = pairs ? m.qlist : m.fallback;
+  const int full = pairs ? 8 : 4;
   if (wide)
     hipLaunchKernelGGL(bf_exact_solve_wide_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
-                       z, (const int*)m.fallback, (const int*)&m.status[4], &m.status[6],
+                       z, list, (const int*)&m.status[full], pairs ? 1 : 0, &m.status[6],
                        4 * n_cu, &m.status[7]);
   hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
-                     z, (const int*)m.fallback, (const int*)&m.status[wide ? 7 : 4],
+                     z, list, (const int*)&m.status[wide ? 7 : full], pairs ? 1 : 0,
                      &m.status[6], 4 * n_cu);
   // 64-lane blocks: a short hand-over list still spreads over the CUs
   hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x, y, z,

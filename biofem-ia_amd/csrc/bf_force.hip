@@ -42,22 +42,42 @@ constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 // ------------------------------------------------------------------ kernels ------------
+// Outcome of staged_element for one lane's qp.
+enum : int { kStagedKept = 0, kStagedQpKept = 1, kStagedQpFail = 2 };
+
+// Whether a qp of an element the FP32-mixed path hands over goes to the reference-order
+// solve: its own conditioning bits or non-finite stresses, or forces of the element that are
+// not finite.  The other qps keep their staged stresses, from which the reference-order
+// finish forms the forces -- within the mixed path's budget (1e-5), which has orders of
+// magnitude to spare.  The FP64 path hands over every qp of such an element: a passing qp
+// of a handed element can carry staged errors above its rounding bound, and where the
+// element's stresses are far above the mesh's typical ones (a clamped neighbouring qp), one
+// such qp alone exceeds the path's 1e-12 relative budget on a whole mesh (measured: 1.5e-12
+// on the 256 x 192 membrane at z noise 7e-4, from one element).
+__device__ __forceinline__ bool qp_fails(int qstatus, double s0, double s1, double s2,
+                                         bool elem_finite) {
+  return !elem_finite ||
+         needs_exact_bits(qstatus, isfinite(s0) && isfinite(s1) && isfinite(s2));
+}
+
 // Staged element body for lane q of the P-lane group that holds element e (P >= n_qp):
 // every lane of the group forms the element's current midsurface geometry (FP64), lane q
 // runs through-thickness point q of the staged constitutive chain in R, the group sums the
 // qp resultants in qp order by cross-lane reads, and each lane writes the nodal forces of
-// patch nodes q, q + n_qp, ...  An element outside the well-conditioned regime writes
-// nothing here; the return value (true in its group's lane q = 0) asks the caller to push
-// it to the hand-over list.
+// patch nodes q, q + n_qp, ... to the element-force buffer (kStagedKept).  An element
+// outside the well-conditioned regime writes no forces: lane q stores its qp's stresses for
+// the reference-order finish and answers kStagedQpKept, or answers kStagedQpFail when its
+// qp must be solved in the reference order (qp_fails); the caller lists the element and the
+// failing pairs.
 template <class R>
-__device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& kp,
+__device__ __forceinline__ int staged_element(const DevMesh& m, const KParams& kp,
                                                const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z, int e, int q,
                                                int base, double noise_max = kC33NoiseMax) {
   const int E = kp.E;
   const int n_qp = kp.n_qp;
-  if (q >= n_qp) return false;
+  if (q >= n_qp) return kStagedKept;
   // Every per-element load that does not depend on another one is issued here, before the
   // first branch, so that they share one memory round trip: kind, the first 12 patch slots,
   // the geom0 cache, activation data and the area.  The coordinate gathers form the second.
@@ -155,8 +175,17 @@ __device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& 
   // outside the well-conditioned regime: the reference-order kernel recomputes it (or, on
   // the FP32-mixed path, first the FP64 staged chain of the second tier)
   const bool handed = kp.fallback && needs_exact_bits(st, finite);
-  if (handed) return q == 0;
-  m.c33[(size_t)q * E + e] = s.c33;
+  m.c33[(size_t)q * E + e] = s.c33;  // a failing qp's is replaced by the solve's
+  if (handed) {
+    const int qs = s.status | c33_range_bit(s.c33, sizeof(R) == 4);
+    if (qp_fails(qs, (double)s.S00, (double)s.S11, (double)s.S01, finite)) return kStagedQpFail;
+    double* o = m.qsv + 3 * ((size_t)q * E + e);
+    o[0] = (double)s.S00;
+    o[1] = (double)s.S11;
+    o[2] = (double)s.S01;
+    m.qst[(size_t)q * E + e] = 0;
+    return kStagedQpKept;
+  }
 
   // ---- nodal forces (ElemUpdFint, active_strain.c:1762-1995) ----
   const double pref0 = 0.5 * kp.h0 * A0;
@@ -186,14 +215,14 @@ __device__ __forceinline__ bool staged_element(const DevMesh& m, const KParams& 
     }
   }
   // ---- status (group leader) ----
-  if (q != 0) return false;
+  if (q != 0) return kStagedKept;
   if (!finite) atomicMin(&m.status[0], e);
   m.flags[e] = (unsigned char)((st & (kStatCC3333 | kStatClamp)) | (finite ? 0 : kStatNonFinite));
   if (st & kStatCC3333) {
     atomicMin(&m.status[2], e);
     atomicAdd(&m.status[3], 1);
   }
-  return false;
+  return kStagedKept;
 }
 
 constexpr int kStagedBlock = 256;
@@ -405,6 +434,7 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
   lds_barrier();
   // ---- 3. resultants, hand-over, force coefficients ----
   bool handed = false;
+  unsigned qfail = 0u;  // qps of a handed-over element that the reference order solves
   V3<double> U0 = mk(0.0, 0.0, 0.0), U1 = U0, A3 = U0;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   if (elem_thread && live) {
@@ -428,6 +458,23 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       for (int a = 0; a < nen; a++) {
         const unsigned short code = sc[(size_t)a * T + i];
         if (code != kTileIdle) tf[code] = 1;  // kTileIdle: a slot without force (LV cap)
+      }
+      // FP64: every qp goes to the reference-order solve (no pairs are listed: the solve
+      // takes the element list).  FP32-mixed (the second tier re-evaluates the element in
+      // FP64 first): the failing qps are listed, the others keep their staged stresses, from
+      // which the reference-order finish forms the forces.
+      if (sizeof(R) == 4) for (int k = 0; k < P; k++) {
+        const double* o = qb + (size_t)k * T + i;
+        const double s0 = o[0], s1 = o[(size_t)P * T], s2 = o[(size_t)2 * P * T];
+        if (qp_fails(qs[k * T + i], s0, s1, s2, finite)) {
+          qfail |= 1u << k;
+        } else {
+          double* d = m.qsv + 3 * ((size_t)k * E + e);
+          d[0] = s0;
+          d[1] = s1;
+          d[2] = s2;
+          m.qst[(size_t)k * E + e] = 0;
+        }
       }
     } else {
       if (!finite) atomicMin(&m.status[0], e);
@@ -455,7 +502,10 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       v2 = -2.0 * pref0 * hh * M2;
     }
   }
-  if (elem_thread) WaveSink{td.wlist, td.wcount, 64, t * (T >> 6) + (tid >> 6)}.push(live && handed, e);
+  if (elem_thread) {
+    const int w = t * (T >> 6) + (tid >> 6);
+    HandSink{td.wlist, td.wqlist, td.wcount, w}.push(live && handed, qfail, e, P);
+  }
   lds_barrier();  // region A is read: the slot forces take its place
   if (elem_thread) {
     const bool adds = live && !handed;
@@ -535,13 +585,39 @@ bf_tile_kernel(DevMesh m, KParams kp, TileDev td, const double* __restrict__ x,
   tile_body<R>(m, kp, td, t, x, y, z, fint, accumulate, n_v, tile_smem);
 }
 
+// Appends the pair (e, q) of every lane with `fail` to its qp's segment of wave w (see
+// HandSink); pushed[k]: wave-uniform counts of what earlier calls appended per qp.  Every
+// lane active here was active in the earlier calls of the wave.
+__device__ __forceinline__ unsigned long long hand_counts(int n_elem,
+                                                         const int (&pairs)[kMaxQp]) {
+  unsigned long long w = hand_count(n_elem);
+#pragma unroll
+  for (int k = 0; k < kMaxQp; k++) w |= hand_count_pairs(k, pairs[k]);
+  return w;
+}
+
+__device__ __forceinline__ void push_failing_pair(int* list, int w, bool fail, int e, int q,
+                                                  int P, int (&pushed)[kMaxQp]) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int k = 0; k < kMaxQp; k++) {
+    if (k >= P) break;
+    const bool take = fail && q == k;
+    const unsigned long long bal = __ballot(take);
+    if (take) list[pair_segment(w, k) + pushed[k] + __popcll(bal & below)] = qp_pair(e, q);
+    pushed[k] += __popcll(bal);
+  }
+}
+
 // Second tier of the FP32-mixed path: wave w re-evaluates the elements its tile wave handed
 // over (segment w of td.wlist, up to 64) with the FP64 staged chain, in passes of
 // floor(64 / n_qp) lane groups, and rewrites the segment in place with what that chain
 // cannot hold within the mixed budget either (the FP32 chain keeps the well-conditioned
 // elements: rounding bound 1e-11, C33 in [0.4, 2.5]; the FP64 chain the moderately
-// conditioned ones: bound 1e-8, C33 in [0.1, 10]).  The forces of the elements it keeps go to
-// the element-force buffer (they stay deferred: the node pass adds them).
+// conditioned ones: bound 1e-8, C33 in [0.1, 10]), and its pair segment with their failing
+// qps.  The forces of the elements it keeps go to the element-force buffer (they stay
+// deferred: the node pass adds them).
 __global__ void __launch_bounds__(kStagedBlock) bf_tile_tier2_kernel(DevMesh m, KParams kp,
                                                                      TileDev td, int n_waves,
                                                                      const double* __restrict__ x,
@@ -552,105 +628,154 @@ __global__ void __launch_bounds__(kStagedBlock) bf_tile_tier2_kernel(DevMesh m, 
   lane_group(P, threadIdx.x & 63, slot, q, base);
   const int w = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
   if (w >= n_waves) return;
-  const int n = td.wcount[w];
+  const int n = count_elems(td.wcount[w]);
   if (n == 0) return;
   int* seg = td.wlist + (size_t)w * 64;
   int pushed = 0;
+  int qpushed[kMaxQp] = {};
   for (int k0 = 0; k0 < n; k0 += per_wave) {
     const int i = k0 + slot;
     if (slot < per_wave && i < n) {
       const int e = seg[i];
-      const bool take = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
-      TileSink{seg, &pushed}.push(take, e);
+      const int r = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
+      TileSink{seg, &pushed}.push(r != kStagedKept && q == 0, e);
+      push_failing_pair(td.wqlist, w, r == kStagedQpFail, e, q, P, qpushed);
     }
   }
-  if ((threadIdx.x & 63) == 0) td.wcount[w] = pushed;
+  if ((threadIdx.x & 63) == 0) td.wcount[w] = hand_counts(pushed, qpushed);
 }
 
 // Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
-// m.fallback and stores the total in m.status[4].  One block: a chunk of waves per thread,
-// a block-wide scan of the chunk sums, then each thread copies its chunk.  The counts and
-// segments were just written by waves on every XCD, so each load is a round trip to
-// memory: a thread reads its counts kCompactRegs at a time with the loads in flight
-// together, and loads the first two entries of every segment of a group before it stores
-// any (most segments hold none or one).
+// m.fallback and stores the total in m.status[4], and its pair segments into m.qlist, qp by
+// qp in qp_of_rank order, with the total in m.status[8] and the mid-surface pairs (the last
+// class) in m.status[9].  One block: a chunk of waves per thread, one block-wide scan of
+// the chunk sums of every class at once (wave-level shuffle scans, then the wave totals:
+// one barrier), then each thread copies its chunks.  The counts were just written by waves
+// on every XCD, so each load is a round trip to memory: a thread reads its packed count
+// words kCompactRegs at a time with the loads in flight together.
 constexpr int kCompactBlock = 1024;
+constexpr int kCompactWaves = kCompactBlock / 64;
 constexpr int kCompactRegs = 8;
+constexpr int kCompactClasses = 1 + kMaxQp;  // elements, then pairs by qp rank
 
-// Block-wide exclusive offset of this thread's `sum` (inclusive Hillis-Steele scan in s_sum);
-// the block's total is left in s_sum[kCompactBlock - 1].
-__device__ __forceinline__ int compact_offset(int* s_sum, int sum) {
-  const int t = threadIdx.x;
-  s_sum[t] = sum;
+// Count of class c (0: elements, 1 + r: pairs of qp rank r) in a packed word.
+__device__ __forceinline__ int class_count(unsigned long long w, int c, int P) {
+  return c == 0 ? count_elems(w) : count_pairs(w, qp_of_rank(c - 1, P));
+}
+
+// Block-wide exclusive offsets of this thread's sums of classes 0 .. nc - 1 and the block's
+// totals; s_wave: [kCompactWaves][kCompactClasses].
+__device__ __forceinline__ void compact_offsets(int* s_wave, int nc,
+                                                const int (&sum)[kCompactClasses],
+                                                int (&pos)[kCompactClasses],
+                                                int (&total)[kCompactClasses]) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  int inc[kCompactClasses];
+#pragma unroll
+  for (int c = 0; c < kCompactClasses; c++) inc[c] = sum[c];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+    for (int c = 0; c < kCompactClasses; c++) {
+      if (c >= nc) break;
+      const int v = __shfl_up(inc[c], d);
+      if (lane >= d) inc[c] += v;
+    }
+  }
+  if (lane == 63) {
+#pragma unroll
+    for (int c = 0; c < kCompactClasses; c++) s_wave[wv * kCompactClasses + c] = inc[c];
+  }
   __syncthreads();
-  for (int d = 1; d < kCompactBlock; d <<= 1) {
-    const int v = (t >= d) ? s_sum[t - d] : 0;
-    __syncthreads();
-    s_sum[t] += v;
-    __syncthreads();
-  }
-  return s_sum[t] - sum;
-}
-
-// Counts of segments g0 .. g0 + kCompactRegs - 1 (zero at and past w1).
-__device__ __forceinline__ void compact_counts(const int* __restrict__ count, int g0, int w1,
-                                               int (&cnt)[kCompactRegs]) {
 #pragma unroll
-  for (int r = 0; r < kCompactRegs; r++) cnt[r] = (g0 + r < w1) ? count[g0 + r] : 0;
-}
-
-// Thread's segments [w0, w1) of `list` (per_wave entries apart) to out[pos ..): entry k of
-// segment w as emit(position, w, value).
-template <class Emit>
-__device__ __forceinline__ void compact_copy(const int* __restrict__ count,
-                                             const int* __restrict__ list, int per_wave, int w0,
-                                             int w1, int pos, Emit&& emit) {
-  for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
-    int cnt[kCompactRegs], v0[kCompactRegs], v1[kCompactRegs];
-    compact_counts(count, g0, w1, cnt);
-#pragma unroll
-    for (int r = 0; r < kCompactRegs; r++) {
-      const size_t base = (size_t)(g0 + r) * per_wave;
-      v0[r] = cnt[r] > 0 ? list[base] : 0;
-      v1[r] = cnt[r] > 1 ? list[base + 1] : 0;
+  for (int c = 0; c < kCompactClasses; c++) {
+    int before = 0, all = 0;
+    for (int w = 0; w < kCompactWaves; w++) {
+      const int v = c < nc ? s_wave[w * kCompactClasses + c] : 0;
+      before += w < wv ? v : 0;
+      all += v;
     }
-#pragma unroll
-    for (int r = 0; r < kCompactRegs; r++) {
-      const int w = g0 + r;
-      if (cnt[r] > 0) emit(pos, w, v0[r]);
-      if (cnt[r] > 1) emit(pos + 1, w, v1[r]);
-      for (int k = 2; k < cnt[r]; k++) emit(pos + k, w, list[(size_t)w * per_wave + k]);
-      pos += cnt[r];
-    }
+    pos[c] = before + inc[c] - sum[c];
+    total[c] = all;
   }
 }
 
-// Sum of the thread's counts [w0, w1).
-__device__ __forceinline__ int compact_sum(const int* __restrict__ count, int w0, int w1) {
-  int sum = 0;
-  for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
-    int cnt[kCompactRegs];
-    compact_counts(count, g0, w1, cnt);
-#pragma unroll
-    for (int r = 0; r < kCompactRegs; r++) sum += cnt[r];
-  }
-  return sum;
-}
-
-__global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(const int* __restrict__ wcount,
-                                                                   const int* __restrict__ wlist,
-                                                                   int n_waves, int per_wave,
-                                                                   int* __restrict__ out,
-                                                                   int* __restrict__ status) {
-  __shared__ int s_sum[kCompactBlock];
+// The element segments (class 0, 64 entries per wave, elist) and the pair segments of the P
+// qps in qp_of_rank order (classes 1 .. P, plist, pair_segment; none for P = 0) of waves
+// [0, n_waves) with packed counts cnt: each class concatenated in wave order, the pair classes one after
+// another.  emit_elem(at, w, v), emit_pair(at, w, v); totals to set_totals(elements, pairs,
+// mid-surface pairs) by one thread.
+template <class EmitE, class EmitP, class Totals>
+__device__ __forceinline__ void compact_all(int* s_wave, const unsigned long long* cnt,
+                                            const int* elist, const int* plist, int n_waves,
+                                            int P, EmitE&& emit_elem, EmitP&& emit_pair,
+                                            Totals&& set_totals) {
+  // P = 0: elements only
   const int t = threadIdx.x;
   const int chunk = (n_waves + kCompactBlock - 1) / kCompactBlock;
   const int w0 = min(n_waves, t * chunk), w1 = min(n_waves, w0 + chunk);
-  const int sum = compact_sum(wcount, w0, w1);
-  const int pos = compact_offset(s_sum, sum);
-  if (sum > 0)  // most chunks hold no hand-over
-    compact_copy(wcount, wlist, per_wave, w0, w1, pos, [&](int at, int, int v) { out[at] = v; });
-  if (t == kCompactBlock - 1) status[4] = s_sum[t];
+  const int nc = 1 + P;
+  int sum[kCompactClasses] = {};
+  for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
+    unsigned long long wd[kCompactRegs];
+#pragma unroll
+    for (int r = 0; r < kCompactRegs; r++) wd[r] = (g0 + r < w1) ? cnt[g0 + r] : 0ull;
+#pragma unroll
+    for (int r = 0; r < kCompactRegs; r++)
+#pragma unroll
+      for (int c = 0; c < kCompactClasses; c++)
+        if (c < nc) sum[c] += class_count(wd[r], c, P);
+  }
+  int pos[kCompactClasses], total[kCompactClasses];
+  compact_offsets(s_wave, nc, sum, pos, total);
+  int base[kCompactClasses];  // start of each class in its output list
+  base[0] = 0;
+  int acc = 0;
+#pragma unroll
+  for (int c = 1; c < kCompactClasses; c++) {
+    base[c] = acc;
+    acc += c < nc ? total[c] : 0;
+  }
+  int any = 0;
+#pragma unroll
+  for (int c = 0; c < kCompactClasses; c++) any |= sum[c];
+  if (any) {  // most chunks hold no hand-over
+    for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
+      unsigned long long wd[kCompactRegs];
+#pragma unroll
+      for (int r = 0; r < kCompactRegs; r++) wd[r] = (g0 + r < w1) ? cnt[g0 + r] : 0ull;
+#pragma unroll
+      for (int r = 0; r < kCompactRegs; r++) {
+        if (wd[r] == 0ull) continue;
+        const int w = g0 + r;
+        const int ne = count_elems(wd[r]);
+        for (int k = 0; k < ne; k++) emit_elem(pos[0] + k, w, elist[(size_t)w * 64 + k]);
+        pos[0] += ne;
+        for (int c = 1; c < nc; c++) {
+          const int q = qp_of_rank(c - 1, P);
+          const int n = count_pairs(wd[r], q);
+          for (int k = 0; k < n; k++)
+            emit_pair(base[c] + pos[c] + k, w, plist[pair_segment(w, q) + k]);
+          pos[c] += n;
+        }
+      }
+    }
+  }
+  if (t == kCompactBlock - 1) set_totals(total[0], acc, has_mid_qp(P) ? total[P] : 0);
+}
+
+__global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(
+    const unsigned long long* __restrict__ cnt, const int* __restrict__ wlist,
+    const int* __restrict__ wqlist, int n_waves, int P, int* __restrict__ out,
+    int* __restrict__ qout, int* __restrict__ status) {
+  __shared__ int s_wave[kCompactWaves * kCompactClasses];
+  compact_all(s_wave, cnt, wlist, wqlist, n_waves, P,
+              [&](int at, int, int v) { out[at] = v; }, [&](int at, int, int v) { qout[at] = v; },
+              [&](int ne, int np, int nm) {
+                status[4] = ne;
+                status[8] = np;
+                status[9] = nm;
+              });
 }
 
 __global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ ptr,
@@ -722,38 +847,41 @@ __global__ void __launch_bounds__(kStagedBlock) bf_batch_tier2_kernel(BatchDev b
   const int t0 = __builtin_amdgcn_readfirstlane(logical_block() * (kStagedBlock / 64) +
                                                 (int)(threadIdx.x >> 6));
   for (int t = t0; t < b.n_etiles; t += step) {
-    const int n = b.tcount[t];
+    const int n = count_elems(b.tcount[t]);
     if (n == 0) continue;
     const Tile tl = b.etiles[t];
     const int c = __builtin_amdgcn_readfirstlane(tl.c);
     const CaseDev& cd = b.cases[c];
     int* seg = b.tlist + (size_t)t * b.tile_cap;
     int pushed = 0;
+    int qpushed[kMaxQp] = {};
     for (int k0 = 0; k0 < n; k0 += per_wave) {
       const int i = k0 + slot;
       if (slot < per_wave && i < n) {
         const int e = seg[i];
-        const bool take = staged_element<double>(cd.m, cd.kp, cd.x, cd.y, cd.z, e, q, base,
-                                                 kC33NoiseMaxTier2);
-        TileSink{seg, &pushed}.push(take, e);
+        const int r = staged_element<double>(cd.m, cd.kp, cd.x, cd.y, cd.z, e, q, base,
+                                             kC33NoiseMaxTier2);
+        TileSink{seg, &pushed}.push(r != kStagedKept && q == 0, e);
+        push_failing_pair(b.tqlist, t, r == kStagedQpFail, e, q, P, qpushed);
       }
     }
-    if ((threadIdx.x & 63) == 0) b.tcount[t] = pushed;
+    if ((threadIdx.x & 63) == 0) b.tcount[t] = hand_counts(pushed, qpushed);
   }
 }
 
-// Concatenates the tiles' hand-over segments into (case, element) pairs, *htotal = count.
-__global__ void __launch_bounds__(kCompactBlock) bf_batch_compact_kernel(BatchDev b) {
-  __shared__ int s_sum[kCompactBlock];
-  const int t = threadIdx.x;
-  const int chunk = (b.n_etiles + kCompactBlock - 1) / kCompactBlock;
-  const int w0 = min(b.n_etiles, t * chunk), w1 = min(b.n_etiles, w0 + chunk);
-  const int sum = compact_sum(b.tcount, w0, w1);
-  const int pos = compact_offset(s_sum, sum);
-  if (sum > 0)  // most chunks hold no hand-over
-    compact_copy(b.tcount, b.tlist, b.tile_cap, w0, w1, pos,
-                 [&](int at, int w, int v) { b.hlist[at] = make_int2(b.etiles[w].c, v); });
-  if (t == kCompactBlock - 1) *b.htotal = s_sum[t];
+// Concatenates the tiles' hand-over segments into (case, element) pairs, *htotal = count,
+// and (pairs: the FP32-mixed path) their pair segments into (case, qp_pair) entries by qp
+// rank, ptotal[0] = count, ptotal[1] = mid-surface pairs at the end.
+__global__ void __launch_bounds__(kCompactBlock) bf_batch_compact_kernel(BatchDev b, int pairs) {
+  __shared__ int s_wave[kCompactWaves * kCompactClasses];
+  compact_all(s_wave, b.tcount, b.tlist, b.tqlist, b.n_etiles, pairs ? b.P : 0,
+              [&](int at, int w, int v) { b.hlist[at] = make_int2(b.etiles[w].c, v); },
+              [&](int at, int w, int v) { b.plist[at] = make_int2(b.etiles[w].c, v); },
+              [&](int ne, int np, int nm) {
+                *b.htotal = ne;
+                b.ptotal[0] = np;
+                b.ptotal[1] = nm;
+              });
 }
 
 // Node gather of a batch: one block per node tile.
@@ -802,6 +930,9 @@ __global__ void bf_status_reset(int* st) {
     st[4] = 0;
     st[5] = 0;
     st[6] = 0;
+    st[7] = 0;
+    st[8] = 0;
+    st[9] = 0;
   }
 }
 
@@ -837,8 +968,9 @@ struct bf_ctx {
   double* d_c33 = nullptr;
   unsigned char* d_flags = nullptr;
   int* d_fallback = nullptr;
-  int* d_wlist = nullptr;
-  int* d_wcount = nullptr;
+  int* d_qlist = nullptr;
+  int* last_qlist = nullptr;  // hand-over list of the last staged evaluation (own or
+                              // scratch): pairs (FP32-mixed) or elements (FP64)
   double* d_qsv = nullptr;
   int* d_qst = nullptr;
   int* d_redo = nullptr;
@@ -883,7 +1015,8 @@ struct bf_ctx {
   unsigned char* d_touched = nullptr;
   unsigned char* d_deferred = nullptr;
   int* d_twlist = nullptr;
-  int* d_twcount = nullptr;
+  unsigned long long* d_twcount = nullptr;
+  int* d_twqlist = nullptr;
 };
 
 namespace {
@@ -917,7 +1050,7 @@ void free_tiles(bf_ctx* c) {
   dfree(c->d_telem); dfree(c->d_tlslot); dfree(c->d_tcptr); dfree(c->d_tcent);
   dfree(c->d_tnode_ptr); dfree(c->d_tnode); dfree(c->d_tout); dfree(c->d_tpptr);
   dfree(c->d_tcomplete); dfree(c->d_tpartial); dfree(c->d_touched); dfree(c->d_deferred);
-  dfree(c->d_twlist); dfree(c->d_twcount);
+  dfree(c->d_twlist); dfree(c->d_twcount); dfree(c->d_twqlist);
   c->n_tiles = c->n_partials = c->tileT = 0;
 }
 
@@ -965,6 +1098,7 @@ int build_ctx_tiles(bf_ctx* c, int T) {
   TALLOC(c->d_deferred, (size_t)std::max(c->E, 1));
   TALLOC(c->d_twlist, nt * T);
   TALLOC(c->d_twcount, nt * T / 64);
+  TALLOC(c->d_twqlist, nt * T * kMaxQp);
 #undef TALLOC
   hipStream_t s = c->stream;
   if (th.n_tiles > 0) {
@@ -991,7 +1125,7 @@ int build_ctx_tiles(bf_ctx* c, int T) {
 void free_mesh(bf_ctx* c) {
   dfree(c->d_patch); dfree(c->d_meta); dfree(c->d_corner); dfree(c->d_ref); dfree(c->d_dA0);
   dfree(c->d_fib); dfree(c->d_gscale); dfree(c->d_delay); dfree(c->d_Fe); dfree(c->d_c33);
-  dfree(c->d_flags); dfree(c->d_fallback); dfree(c->d_wlist); dfree(c->d_wcount);
+  dfree(c->d_flags); dfree(c->d_fallback); dfree(c->d_qlist);
   dfree(c->d_qsv); dfree(c->d_qst); dfree(c->d_redo);
   dfree(c->d_v); dfree(c->d_winv); dfree(c->d_fixed);
   c->has_state = false;
@@ -1059,8 +1193,8 @@ int noalloc_scratch_init(int device) {
 // Byte offsets of the per-evaluation arrays of a mesh (E elements, NS patch slots) in the
 // static scratch, 256-byte aligned; returns the total.
 struct ScratchLayout {
-  size_t Fe, fallback, wlist, wcount, qsv, qst, redo, partial, touched, deferred, twlist,
-      twcount, total;
+  size_t Fe, fallback, qlist, qsv, qst, redo, partial, touched, deferred, twlist, twcount,
+      twqlist, total;
 };
 ScratchLayout noalloc_layout(size_t E, size_t NS, size_t n_nodes, size_t n_tiles, size_t T,
                              size_t n_partials) {
@@ -1070,8 +1204,7 @@ ScratchLayout noalloc_layout(size_t E, size_t NS, size_t n_nodes, size_t n_tiles
   size_t o = 0;
   L.Fe = o;       o += up(sizeof(double) * 3 * NS * En);
   L.fallback = o; o += up(sizeof(int) * En);
-  L.wlist = o;    o += up(sizeof(int) * (En + 64));
-  L.wcount = o;   o += up(sizeof(int) * En);
+  L.qlist = o;    o += up(sizeof(int) * kMaxQp * En);
   L.qsv = o;      o += up(sizeof(double) * 3 * kMaxQp * En);
   L.qst = o;      o += up(sizeof(int) * kMaxQp * En);
   L.redo = o;     o += up(sizeof(int) * En);
@@ -1079,7 +1212,8 @@ ScratchLayout noalloc_layout(size_t E, size_t NS, size_t n_nodes, size_t n_tiles
   L.touched = o;  o += up(std::max<size_t>(n_nodes, 1));
   L.deferred = o; o += up(En);
   L.twlist = o;   o += up(sizeof(int) * nt * T);
-  L.twcount = o;  o += up(sizeof(int) * nt * T / 64);
+  L.twcount = o;  o += up(sizeof(unsigned long long) * nt * T / 64);
+  L.twqlist = o;  o += up(sizeof(int) * nt * T * kMaxQp);
   L.total = o;
   return L;
 }
@@ -1120,8 +1254,8 @@ int bf_ctx_create(int device, bf_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->own_ev[0]) != hipSuccess || hipEventCreate(&c->own_ev[1]) != hipSuccess ||
       hipEventCreate(&c->own_ev[2]) != hipSuccess || hipEventCreate(&c->own_ev[3]) != hipSuccess ||
-      hipMalloc((void**)&c->d_status, 8 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_status, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      hipMalloc((void**)&c->d_status, kStatusWords * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_status, kStatusWords * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     bf_ctx_destroy(c);
     return BF_ERR_DEVICE;
   }
@@ -1281,8 +1415,7 @@ int bf_mesh_upload(bf_ctx* c, const bf_mesh_desc* m) {
   ALLOC(c->d_c33, (size_t)kMaxQp * En);
   ALLOC(c->d_flags, En);
   ALLOC(c->d_fallback, En);
-  ALLOC(c->d_wlist, En + 64);
-  ALLOC(c->d_wcount, En);
+  ALLOC(c->d_qlist, (size_t)kMaxQp * En);
   ALLOC(c->d_qsv, (size_t)3 * kMaxQp * En);
   ALLOC(c->d_qst, (size_t)kMaxQp * En);
   ALLOC(c->d_redo, En);
@@ -1416,8 +1549,7 @@ static DevMesh dev_mesh(const bf_ctx* c) {
   dm.status = c->d_status;
   dm.flags = c->d_flags;
   dm.fallback = c->d_fallback;
-  dm.wlist = c->d_wlist;
-  dm.wcount = c->d_wcount;
+  dm.qlist = c->d_qlist;
   dm.qsv = c->d_qsv;
   dm.qst = c->d_qst;
   dm.redo = c->d_redo;
@@ -1452,6 +1584,7 @@ static TileDev tile_dev(const bf_ctx* c) {
   td.deferred = c->d_deferred;
   td.wlist = c->d_twlist;
   td.wcount = c->d_twcount;
+  td.wqlist = c->d_twqlist;
   return td;
 }
 
@@ -1509,11 +1642,15 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     if (precision == BF_PREC_MIXED)
       hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3((n_waves + 3) / 4), dim3(kStagedBlock), 0, s,
                          dm, kp, td, n_waves, x, y, z);
-    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, td.wcount,
-                       td.wlist, n_waves, 64, dm.fallback, dm.status);
+    const bool pairs = precision == BF_PREC_MIXED;
+    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s,
+                       (const unsigned long long*)td.wcount, (const int*)td.wlist,
+                       (const int*)td.wqlist, n_waves, pairs ? kp.n_qp : 0, dm.fallback,
+                       dm.qlist, dm.status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
-                                         s));
+                                         pairs, s));
     c->fb_prec = precision;
+    c->last_qlist = pairs ? dm.qlist : dm.fallback;
   }
   BF_HIP(hipEventRecord(e2, s));
   if (staged) {
@@ -1543,12 +1680,14 @@ static int after_last_eval(bf_ctx* c) {
 
 static int collect_status(bf_ctx* c, int* elem, int* code, int* n_warn) {
   if (int rc = after_last_eval(c)) return rc;
-  BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, kStatusWords * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   const int* st = c->h_status;
   if (c->fb_prec >= 0)
     c->wide_hint[c->fb_prec] =
-        st[4] > 0 && split_fits(st[4], kWideSplitMin, c->kp.n_qp, 4 * c->n_cu);
+        c->fb_prec == BF_PREC_MIXED
+            ? st[8] > 0 && split_fits(st[8], st[9], c->kp.n_qp, true, kWideSplitMin, 4 * c->n_cu)
+            : st[4] > 0 && split_fits(st[4], 0, c->kp.n_qp, false, kWideSplitMin, 4 * c->n_cu);
   int el = -1, cd = 0;
   if (st[0] != INT_MAX) { el = st[0]; cd = BF_ELEM_NONFINITE; }
   else if (st[2] != INT_MAX) { el = st[2]; cd = BF_ELEM_CC3333; }
@@ -1605,8 +1744,7 @@ int bf_force_eval_device_noalloc(bf_ctx* c, const double* d_x, const double* d_y
   DevMesh dm = dev_mesh(c);
   dm.Fe = (double*)(base + L.Fe);
   dm.fallback = (int*)(base + L.fallback);
-  dm.wlist = (int*)(base + L.wlist);
-  dm.wcount = (int*)(base + L.wcount);
+  dm.qlist = (int*)(base + L.qlist);
   dm.qsv = (double*)(base + L.qsv);
   dm.qst = (int*)(base + L.qst);
   dm.redo = (int*)(base + L.redo);
@@ -1615,7 +1753,8 @@ int bf_force_eval_device_noalloc(bf_ctx* c, const double* d_x, const double* d_y
   td.touched = (unsigned char*)(base + L.touched);
   td.deferred = (unsigned char*)(base + L.deferred);
   td.wlist = (int*)(base + L.twlist);
-  td.wcount = (int*)(base + L.twcount);
+  td.wcount = (unsigned long long*)(base + L.twcount);
+  td.wqlist = (int*)(base + L.twqlist);
   // the touched marks start at zero (the node pass clears what the tile kernel sets; the
   // array is shared with other meshes' calls, so it is cleared here, by a kernel)
   if (c->n_nodes > 0)
@@ -1707,6 +1846,36 @@ int bf_last_fallback_count(bf_ctx* c, int* n) {
   const int rc = collect_status(c, nullptr, nullptr, nullptr);
   if (rc != BF_OK && rc != BF_ERR_ELEMENT) return rc;
   *n = c->h_status[4];
+  return BF_OK;
+}
+
+int bf_last_handed_qps(bf_ctx* c, unsigned char* mask, int* n_pairs) {
+  if (!c || !mask) return BF_ERR_ARG;
+  if (!c->has_mesh) return BF_ERR_STATE;
+  BF_HIP(hipSetDevice(c->device));
+  const int rc = collect_status(c, nullptr, nullptr, nullptr);
+  if (rc != BF_OK && rc != BF_ERR_ELEMENT) return rc;
+  std::memset(mask, 0, (size_t)c->E);
+  if (n_pairs) *n_pairs = 0;
+  if (c->fb_prec < 0 || !c->last_qlist) return BF_OK;
+  // FP32-mixed: a pair list; FP64: an element list, every qp solved
+  const bool pairs = c->fb_prec == BF_PREC_MIXED;
+  const int n = pairs ? c->h_status[8] : c->h_status[4];
+  if (n <= 0) return BF_OK;
+  std::vector<int> list((size_t)n);
+  BF_HIP(hipMemcpyAsync(list.data(), c->last_qlist, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  const unsigned all = (1u << c->kp.n_qp) - 1u;
+  int count = 0;
+  for (int v : list) {
+    const int e = pairs ? pair_elem(v) : v;
+    const int q = pairs ? pair_qp(v) : 0;
+    if (e < 0 || e >= c->E || q >= c->kp.n_qp) return BF_ERR_DEVICE;
+    const unsigned add = pairs ? (1u << q) : all;
+    count += __builtin_popcount(add & ~(unsigned)mask[e]);
+    mask[e] |= (unsigned char)add;
+  }
+  if (n_pairs) *n_pairs = count;
   return BF_OK;
 }
 
@@ -1899,8 +2068,10 @@ struct bf_batch {
   CaseDev* d_cases = nullptr;
   Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_gtiles = nullptr, *d_xtiles = nullptr,
        *d_ttiles = nullptr;
-  int *d_tlist = nullptr, *d_tcount = nullptr, *d_htotal = nullptr, *d_rtotal = nullptr;
-  int2 *d_hlist = nullptr, *d_rlist = nullptr;
+  int *d_tlist = nullptr, *d_htotal = nullptr, *d_rtotal = nullptr;
+  unsigned long long* d_tcount = nullptr;
+  int *d_tqlist = nullptr, *d_ptotal = nullptr;
+  int2 *d_hlist = nullptr, *d_rlist = nullptr, *d_plist = nullptr;
   int* d_cstat = nullptr;  // [2 n_cases]: per-case status words read by bf_batch_case_status
   int n_etiles = 0, n_ntiles = 0, n_gtiles = 0, n_xtiles = 0, n_ttiles = 0, tile_cap = 0;
   long long E_total = 0;
@@ -1925,6 +2096,7 @@ void batch_free_lists(bf_batch* b) {
   dfree(b->d_cases); dfree(b->d_etiles); dfree(b->d_ntiles); dfree(b->d_gtiles);
   dfree(b->d_xtiles); dfree(b->d_ttiles);
   dfree(b->d_tlist); dfree(b->d_tcount); dfree(b->d_hlist); dfree(b->d_htotal);
+  dfree(b->d_tqlist); dfree(b->d_plist); dfree(b->d_ptotal);
   dfree(b->d_rlist); dfree(b->d_rtotal); dfree(b->d_cstat);
   b->n_etiles = b->n_ntiles = b->n_gtiles = b->n_xtiles = b->n_ttiles = 0;
 }
@@ -1974,10 +2146,14 @@ int batch_build(bf_batch* b) {
   b->tile_cap = 64;
   int rc;
   if ((rc = dalloc(&b->d_tlist, (size_t)std::max<long long>(n_tw, 1) * 64)) != BF_OK) return rc;
+  if ((rc = dalloc(&b->d_tqlist, (size_t)std::max<long long>(n_tw, 1) * 64 * kMaxQp)) != BF_OK) {
+    batch_free_lists(b);
+    return rc;
+  }
   std::vector<CaseDev> cases((size_t)nc);
   std::vector<Tile> et, nt, gt, xt, tt;
   std::vector<long long> case_wave((size_t)nc, 0);  // first tile wave of each case
-  long long Et = 0, wave_base = 0;
+  long long Et = 0, E_pairs = 0, wave_base = 0;
   for (int i = 0; i < nc; i++) {
     bf_ctx* c = b->cases[i];
     CaseDev& cd = cases[i];
@@ -1985,7 +2161,8 @@ int batch_build(bf_batch* b) {
     cd.kp = dev_params(c);
     cd.td = tile_dev(c);
     cd.td.wlist = b->d_tlist + wave_base * 64;
-    cd.td.wcount = nullptr;  // set below, once the count array exists
+    cd.td.wcount = nullptr;  // set below, once the count arrays exist
+    cd.td.wqlist = b->d_tqlist + wave_base * 64 * kMaxQp;
     for (int t = 0; t < c->n_tiles; t++) {
       tt.push_back(Tile{i, t, 0, 0});
       for (int w = 0; w < c->tileT / 64; w++) et.push_back(Tile{i, t, w, 0});
@@ -2012,6 +2189,7 @@ int batch_build(bf_batch* b) {
     for (int g0 = 0; g0 < c->n_ghost; g0 += 256) gt.push_back(Tile{i, g0, std::min(256, c->n_ghost - g0), 0});
     for (int e0 = 0; e0 < c->E; e0 += 64) xt.push_back(Tile{i, e0, std::min(64, c->E - e0), 0});
     Et += c->E;
+    E_pairs += (long long)c->E * c->kp.n_qp;
   }
   b->E_total = Et;
   b->n_etiles = (int)et.size();
@@ -2030,11 +2208,15 @@ int batch_build(bf_batch* b) {
   BALLOC(b->d_tcount, (size_t)std::max(b->n_etiles, 1));
   BALLOC(b->d_hlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_htotal, 1);
+  BALLOC(b->d_plist, (size_t)std::max<long long>(E_pairs, 1));
+  BALLOC(b->d_ptotal, 2);
   BALLOC(b->d_rlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_rtotal, 2);
   BALLOC(b->d_cstat, 2 * cases.size());
 #undef BALLOC
-  for (int i = 0; i < nc; i++) cases[i].td.wcount = b->d_tcount + case_wave[i];
+  for (int i = 0; i < nc; i++) {
+    cases[i].td.wcount = b->d_tcount + case_wave[i];
+  }
   hipStream_t s = b->stream;
   BF_HIP(hipMemcpyAsync(b->d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, s));
   if (!et.empty())
@@ -2071,6 +2253,9 @@ BatchDev batch_dev(const bf_batch* b) {
   d.tcount = b->d_tcount;
   d.hlist = b->d_hlist;
   d.htotal = b->d_htotal;
+  d.tqlist = b->d_tqlist;
+  d.plist = b->d_plist;
+  d.ptotal = b->d_ptotal;
   d.rlist = b->d_rlist;
   d.rtotal = b->d_rtotal;
   return d;
@@ -2101,8 +2286,10 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
       // one wave per tile wave (its segment), persistent over the work list
       hipLaunchKernelGGL(bf_batch_tier2_kernel, dim3(b->n_cu * 2), dim3(kStagedBlock), 0, s, d);
     }
-    hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d);
-    BF_HIP(launch_batch_exact_list(d, b->n_cu, s));
+    const bool pairs = precision == BF_PREC_MIXED;
+    hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d,
+                       pairs ? 1 : 0);
+    BF_HIP(launch_batch_exact_list(d, b->n_cu, pairs, s));
   }
   BF_HIP(launch_batch_tile_fixup(d, step, s));
   return kernel_launch_check();

@@ -35,11 +35,12 @@ struct DevMesh {
   const double* delay;
   double* Fe;          // [NS][E][3] element contributions: slot rows of (x, y, z)
   double* c33;         // [n_qp][E]
-  int* status;         // [8] first error elem, code, first warn elem, warn count, fallback count
+  int* status;         // [kStatusWords] first error elem, code, first warn elem, warn count,
+                       // fallback count, ... (below)
   unsigned char* flags;  // [E] per-element status bits of the last evaluation
   int* fallback;       // [E] elements handed from a staged kernel to the plain kernel
-  int* wlist;          // [E + 64] per-wave segments of the staged kernel's hand-over list
-  int* wcount;         // [E] per-wave hand-over counts
+  int* qlist;          // [kMaxQp E] their (element, qp) pairs the reference order solves:
+                       // qp_pair(e, q)
   double* qsv;         // [kMaxQp][E][3] reference-order path: S^00, S^11, S^01 per qp
   int* qst;            // [kMaxQp][E] reference-order path: status bits per qp
   int* redo;           // [E] elements whose CC3333 guard tripped: redone with coupled qps
@@ -64,12 +65,23 @@ struct TileDev {
                                 // evaluations: the node pass clears what the tiles set)
   unsigned char* deferred;      // [E] 1: handed over, its force lands in the element buffer
   int* wlist;                   // [n_tiles * T] hand-over segments, 64 entries per wave
-  int* wcount;                  // [n_tiles * T / 64]
+  int* wqlist;                  // [n_tiles * T * kMaxQp] (element, qp) pair segments,
+                                // 64 entries per wave and qp (pair_segment)
+  unsigned long long* wcount;   // [n_tiles * T / 64] packed segment counts (HandSink)
 };
 
+// status[4]: elements handed over (m.fallback)
 // status[5]: number of entries in redo
 // status[6]: work counter of the hand-over solve
-// status[7]: the narrow hand-over solve's list length, when the wide solve runs first
+// status[7]: the narrow hand-over solve's pair count, when the wide solve runs first
+// status[8]: (element, qp) pairs handed over (m.qlist, ordered by qp_of_rank)
+// status[9]: how many of them, at the end of the list, are mid-surface pairs (has_mid_qp)
+constexpr int kStatusWords = 16;
+
+// An (element, qp) pair of the hand-over solve.
+__host__ __device__ inline int qp_pair(int e, int q) { return (e << 3) | q; }
+__host__ __device__ inline int pair_elem(int v) { return v >> 3; }
+__host__ __device__ inline int pair_qp(int v) { return v & 7; }
 
 // GammaOfTimeDelayed (src/active_strain.c:1156-1179) for a per-element delay.
 __device__ __host__ inline double ramp_gamma(const RampParams& p, double delay) {
@@ -118,30 +130,56 @@ __device__ __forceinline__ void lane_group(int P, int lane, int& slot, int& q, i
   base = slot * P;
 }
 
-// Where a staged kernel lists the elements it hands to the reference-order kernel: every
-// wave owns a segment of `per_wave` entries and writes its count, with no atomics (device-
-// scope atomics on one counter serialise across the chip and cost more than the kernel's
-// arithmetic).  A compaction pass concatenates the segments.  push() is called by every
-// active lane of a wave that holds at least one element; lanes with `take` set append e.
-struct WaveSink {
-  int* list;    // [n_waves * per_wave]
-  int* count;   // [n_waves]
-  int per_wave;
+// Where a staged kernel lists what it hands to the reference-order kernels: every wave owns
+// a segment of 64 elements and one segment of 64 (element, qp) pairs per qp (pair_segment),
+// and writes their counts packed into one word (hand_count), with no atomics (device-scope
+// atomics on one counter serialise across the chip and cost more than the kernel's
+// arithmetic).  A compaction pass concatenates the segments, the pairs qp by qp.  push() is
+// called by every active lane of a wave that holds at least one element; a lane with `take`
+// appends its element e and the pairs of the qps set in qmask (qps below P, wave-uniform).
+__host__ __device__ inline size_t pair_segment(int wave, int q) {
+  return ((size_t)wave * kMaxQp + q) * 64;
+}
+// Packed counts of a wave's segments: bits 0-6 elements, bits 7 + 7 q .. 13 + 7 q the pairs
+// of qp q (each at most 64).
+__host__ __device__ inline unsigned long long hand_count(int n_elem) {
+  return (unsigned long long)n_elem;
+}
+__host__ __device__ inline unsigned long long hand_count_pairs(int q, int n) {
+  return (unsigned long long)n << (7 + 7 * q);
+}
+__host__ __device__ inline int count_elems(unsigned long long w) { return (int)(w & 127u); }
+__host__ __device__ inline int count_pairs(unsigned long long w, int q) {
+  return (int)((w >> (7 + 7 * q)) & 127u);
+}
+
+struct HandSink {
+  int* elist;                // [n_waves * 64]
+  int* plist;                // [n_waves * kMaxQp * 64]
+  unsigned long long* count; // [n_waves]
   int wave;
-  __device__ void push(bool take, int e) const {
+  __device__ void push(bool take, unsigned qmask, int e, int P) const {
     const unsigned long long act = __ballot(true);
-    const unsigned long long mask = __ballot(take);
     const int lane = (int)(threadIdx.x & 63u);
     const int leader = __ffsll((long long)act) - 1;
-    if (lane == leader) count[wave] = __popcll(mask);
-    if (take) list[(size_t)wave * per_wave + __popcll(mask & ((1ull << lane) - 1ull))] = e;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long be = __ballot(take);
+    if (take) elist[(size_t)wave * 64 + __popcll(be & below)] = e;
+    unsigned long long word = hand_count(__popcll(be));
+    for (int q = 0; q < P; q++) {
+      const bool tq = take && ((qmask >> q) & 1u) != 0u;
+      const unsigned long long bal = __ballot(tq);
+      if (tq) plist[pair_segment(wave, q) + __popcll(bal & below)] = qp_pair(e, q);
+      word |= hand_count_pairs(q, __popcll(bal));
+    }
+    if (lane == leader) count[wave] = word;
   }
 };
 
-// The batched form: a wave works through one tile of a case in passes of floor(64 / P)
-// elements and appends to the tile's segment; `pushed` (wave-uniform) counts what earlier
-// passes appended.  Every lane active in a pass was active in all earlier passes of the
-// tile, so its copy of `pushed` is current.
+// The form of a wave that works through one segment in passes of floor(64 / P) elements
+// (the FP32-mixed path's second tier) and appends to a segment; `pushed` (wave-uniform)
+// counts what earlier passes appended.  Every lane active in a pass was active in all
+// earlier passes of the segment, so its copy of `pushed` is current.
 struct TileSink {
   int* list;    // tile segment
   int* pushed;
@@ -152,6 +190,17 @@ struct TileSink {
     *pushed += __popcll(mask);
   }
 };
+
+// Hand-over solve order of the qps: qp by qp, the mid-surface point (theta = 0, odd n_qp)
+// last: on shells with a flat reference its blocks of exactly zero terms are skipped and its
+// solve costs a fraction of the others', so the expensive qps start first and the cheap
+// ones fill the tail.
+__host__ __device__ inline int qp_of_rank(int r, int P) {
+  const int mid = P >> 1;
+  if (!(P & 1)) return r;
+  return r < mid ? r : (r == P - 1 ? mid : r + 1);
+}
+__host__ __device__ inline bool has_mid_qp(int P) { return P > 1 && (P & 1); }
 
 // ---- batches: many meshes (cases) evaluated by the same launches ----
 // One case as the batch kernels see it: its mesh, options and resident state.
@@ -192,9 +241,12 @@ struct BatchDev {
   const Tile* ttiles;     // element tiles of the single-pass staged kernel: (case, tile)
   int n_ttiles;
   int* tlist;             // [n_etiles * tile_cap] hand-over segments
-  int* tcount;            // [n_etiles]
+  unsigned long long* tcount;  // [n_etiles] packed segment counts (HandSink)
   int2* hlist;            // compacted hand-over list: (case, element)
   int* htotal;
+  int* tqlist;            // [n_etiles * kMaxQp * 64] (element, qp) pair segments
+  int2* plist;            // compacted pair list: (case, qp_pair(element, qp)), by qp rank
+  int* ptotal;            // [2]: pairs, mid-surface pairs at the end of the list
   int2* rlist;            // elements whose CC3333 guard tripped: (case, element)
   int* rtotal;            // [2]: rlist count, work counter of the hand-over solve
 };
@@ -299,29 +351,43 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 // guard tripped with the qps coupled.  status[5] must be zero on entry.
 hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const double* x,
                                 const double* y, const double* z, int n_cu, hipStream_t s);
-// The hand-over solve splits each (element, qp) of a list over K lanes of a wave, K from
-// the list length (largest first): 60, 30, 15 (the wide solve kernel), 8, 6, 4, 2, 1.  A
-// split fits when its items fill at most one round of the n_waves persistent waves, the
-// expensive qps (all but the mid-surface point of an odd rule) counted whole and the
-// mid-surface point's at 0.4 of one (measured on flat reference shells, where its zero blocks
-// are skipped).  A wave holds split_groups(K) groups of K lanes.
+// The hand-over solve splits each (element, qp) of its list over K lanes of a wave, K from
+// the list length (largest first): 60, 30, 15 (the wide solve kernel), 8, 6, 4, 2, 1.  The
+// list holds elements, every qp of each solved (qp by qp in qp_of_rank order), or (element,
+// qp) pairs in that order (`pairs`).  A split fits when its work items fill at most one
+// round of the n_waves persistent waves, a wave holding split_groups(K) groups of K lanes
+// and an item one group's worth of consecutive entries of one qp; the mid-surface items
+// (has_mid_qp; for a pair list its last N_mid entries) count at 0.4 of one (measured on
+// flat reference shells, where their zero blocks are skipped; the waves the expensive items
+// leave free absorb them).
 constexpr int kWideSplitMin = 15;
 __host__ __device__ constexpr int split_groups(int K) { return 64 / K; }
-__host__ __device__ inline bool split_fits(int N, int K, int P, int n_waves) {
-  const bool mid = P > 1 && (P & 1);
-  const int pe = mid ? P - 1 : P;
-  const long long chunks = (N + split_groups(K) - 1) / split_groups(K);
-  return chunks * (10 * pe + (mid ? 4 : 0)) <= 10LL * n_waves;
+__host__ __device__ inline bool split_fits(int N, int N_mid, int P, bool pairs, int K,
+                                           int n_waves) {
+  const long long G = split_groups(K);
+  long long ce, cm;
+  if (pairs) {
+    ce = (N - N_mid + G - 1) / G;
+    cm = (N_mid + G - 1) / G;
+  } else {
+    const long long c = (N + G - 1) / G;
+    const int mid = has_mid_qp(P) ? 1 : 0;
+    ce = (P - mid) * c;
+    cm = mid * c;
+  }
+  return 10 * ce + 4 * cm <= 10LL * n_waves;
 }
 
-// Re-evaluates, in the reference's operation order, the elements a staged kernel listed in
-// m.fallback[0 .. m.status[4]) (persistent grids of 4 n_cu waves, count read on the device;
-// the solve takes its work items from status[6], which must be zero on entry).  `wide` also
+// Re-evaluates, in the reference's operation order, what a staged kernel handed over and
+// forms the forces of the elements it listed in m.fallback[0 .. m.status[4]): with `pairs`
+// the (element, qp) pairs of m.qlist[0 .. m.status[8]), else every qp of those elements
+// (persistent grids of 4 n_cu waves, counts read on the device; the solve takes its work
+// items from status[6], which must be zero on entry).  `wide` also
 // launches the solve kernel for short lists (split of kWideSplitMin lanes or more); without
 // it the narrow kernel takes every list -- the sums, and so the results, are the same.
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, int n_cu, bool wide,
-                                         hipStream_t s);
+                                         bool pairs, hipStream_t s);
 // Batch forms: the reference-order path over the element tiles (plain FP64 batches) and
 // over the compacted hand-over list b.hlist[0 .. *b.htotal).  b.rtotal[0..1] must be zero
 // on entry (the prologue clears them).
@@ -333,7 +399,7 @@ hipError_t launch_gather_step(const int* ptr, const unsigned* ent, const double*
                               double* z, double* v, const double* winv,
                               const unsigned char* fixed, double dt, double damp, hipStream_t s);
 hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s);
-hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, hipStream_t s);
+hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, bool pairs, hipStream_t s);
 // Node pass of the single-pass staged path: each node's force from its tile partial sums
 // (tile order) or the value its tile wrote, then the forces of handed-over elements
 // (element-force buffer, ascending element order) for nodes that have any; step != 0 also

@@ -160,14 +160,24 @@ int bf_update_thickness(bf_ctx *ctx, double *thickness);
  * number of elements that raised the non-fatal CC3333 condition.  Synchronises. */
 int bf_last_error(bf_ctx *ctx, int *elem, int *code, int *n_warn);
 
-/* Number of elements the last staged evaluation recomputed in the reference's operation
- * order; bf_set_fallback(ctx, 0) disables that hand-over (default on).  The count read
- * here (and by bf_force_eval, bf_last_error) also steers the next evaluation of that
- * precision: after a short list the hand-over solve adds a kernel that spreads each
- * element over more lanes.  The results do not depend on it.  No reference counterpart:
- * the reference has a single arithmetic path. */
+/* Number of elements the last staged evaluation handed to the reference's operation order
+ * (their forces are formed in that order: BF_PREC_FP64_FAST solves every through-thickness
+ * point of such an element in that order, BF_PREC_MIXED only the points that fail its
+ * conditioning tests and keeps the staged stresses of the others); bf_set_fallback(ctx, 0)
+ * disables that hand-over (default on).  The count read here (and by bf_force_eval,
+ * bf_last_error) also steers the next evaluation of that precision: after a short list the
+ * hand-over solve adds a kernel that spreads each (element, qp) over more lanes.  The
+ * results do not depend on it.  No reference counterpart: the reference has a single
+ * arithmetic path. */
 int bf_last_fallback_count(bf_ctx *ctx, int *n);
 int bf_set_fallback(bf_ctx *ctx, int on);
+
+/* The (element, qp) pairs the last staged evaluation solved in the reference's operation
+ * order: mask[e] bit q is set when through-thickness point q of element e was (elements
+ * whose CC3333 guard tripped are redone with all their points in that order as well; their
+ * flags carry BF_ELEM_CC3333).  *n_pairs: the number of pairs.  Synchronises.  No reference
+ * counterpart: instrumentation of the hand-over. */
+int bf_last_handed_qps(bf_ctx *ctx, unsigned char *mask, int *n_pairs);
 
 /* Wall-clock duration of the last bf_force_eval* kernels on the device, in ms
  * (HIP events on the context stream): element kernel and node-gather kernel. */

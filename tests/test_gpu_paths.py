@@ -44,24 +44,21 @@ def check_all(m, p, x, y, z, tol_exact=0.0, tol_fast=1e-12, tol_mixed=1e-5, omp=
         nf = fe.fallback_count()
         assert np.array_equal(fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST), fast)
         assert fe.fallback_count() == nf
-        # handed-over elements run in the reference order: bit for bit the oracle's C33
-        handed = (fe.elem_flags() & L.FLAG_FALLBACK) != 0
-        assert int(handed.sum()) == nf
-        assert np.array_equal(fe.c33()[handed], c33[handed])
+        # the (element, qp) pairs handed over run in the reference order: bit for bit the
+        # oracle's C33
+        pf = check_handed_qps(fe, nf, c33)
         mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
         nm = fe.fallback_count()
         assert np.array_equal(fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED), mixed)
         assert fe.fallback_count() == nm
-        handed = (fe.elem_flags() & L.FLAG_FALLBACK) != 0
-        assert int(handed.sum()) == nm
-        assert np.array_equal(fe.c33()[handed], c33[handed])
+        pm = check_handed_qps(fe, nm, c33)
         fe.set_fallback(False)
         fast0 = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
         mixed0 = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
         waves = 4 * torch.cuda.get_device_properties(0).multi_processor_count
-        print(f"  fast {rel(fast, ref):.3e} ({nf} handed over, split "
-              f"{split_lanes(nf, p.n_qp, waves)}) mixed {rel(mixed, ref):.3e} ({nm} handed "
-              f"over, split {split_lanes(nm, p.n_qp, waves)}); without hand-over: fast "
+        print(f"  fast {rel(fast, ref):.3e} ({nf} elements / {pf} qps handed over, split "
+              f"{split_lanes(pf, waves)}) mixed {rel(mixed, ref):.3e} ({nm} / {pm} handed "
+              f"over, split {split_lanes(pm, waves)}); without hand-over: fast "
               f"{rel(fast0, ref):.3e} mixed {rel(mixed0, ref):.3e}", flush=True)
         assert rel(fast, ref) <= tol_fast
         assert rel(mixed, ref) <= tol_mixed
@@ -89,25 +86,40 @@ def test_membrane_large_staged_paths():
     check_all(m, p, *xyz, omp=16)
 
 
-def split_lanes(n, n_qp, n_waves):
-    """Lanes per (element, qp) the hand-over solve picks for a list of n elements: the
-    largest of 60, 30, 15 (the wide kernel), 8, 6, 4, 2 whose items fit one round of the
-    persistent grid, the mid-surface qp of an odd rule weighted 0.4; else 1."""
-    mid = n_qp > 1 and n_qp % 2 == 1
-    pe = n_qp - 1 if mid else n_qp
+def split_lanes(n, n_waves):
+    """Lanes per (element, qp) pair the hand-over solve picks for a list of n pairs: the
+    largest of 60, 30, 15 (the wide kernel), 8, 6, 4, 2 whose pairs fit one round of the
+    persistent grid (64 // k groups per wave); else 1."""
     for k in (60, 30, 15, 8, 6, 4, 2):
-        chunks = -(-n // (60 // k))
-        if chunks * (10 * pe + (4 if mid else 0)) <= 10 * n_waves:
+        if -(-n // (64 // k)) <= n_waves:
             return k
     return 1
+
+
+def check_handed_qps(fe, n_elements, c33):
+    """After a staged evaluation: the elements flagged as handed over are the counted ones,
+    each has at least one (element, qp) pair solved in the reference order and no other
+    element has one, and those pairs -- all points of an element whose CC3333 guard tripped,
+    which is redone in that order -- hold the oracle's C33 bit for bit.  Returns the pair
+    count."""
+    flags = fe.elem_flags()
+    handed = (flags & L.FLAG_FALLBACK) != 0
+    assert int(handed.sum()) == n_elements
+    hq = fe.handed_qps()
+    assert not hq[~handed].any()
+    assert hq[handed].any(axis=1).all()
+    n_pairs = int(hq.sum())
+    hq[handed & ((flags & L.FLAG_CC3333) != 0)] = True
+    assert np.array_equal(fe.c33()[hq], c33[hq])
+    return n_pairs
 
 
 @pytest.mark.parametrize("sigma", [4e-4, 5.5e-4, 6.5e-4, 7e-4, 1e-3, 1.2e-3, 1.5e-3])
 def test_membrane_handover_sizes(sigma):
     """Hand-over lists from a few dozen to 19k of the 98 304 elements (current z noise
     N(0, sigma), seed 11), for the FP64 and the mixed path: on a 256-CU device the solve
-    splits each (element, qp) over 60, 30, 15 (the wide kernel), 8, 6, 4, 2 or 1 lanes by the
-    list length, and every handed-over element stays bit for bit the oracle's."""
+    splits each (element, qp) pair over 60, 30, 15 (the wide kernel), 8, 6, 4, 2 or 1 lanes by
+    the list length, and every handed-over pair stays bit for bit the oracle's."""
     m, p, (x, y, z) = configs.membrane_large()
     rng = np.random.default_rng(11)
     z = m.z0.copy()

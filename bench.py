@@ -28,15 +28,42 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBPS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # FP64 operations of one (element, qp, C33 iteration) of the reference's ElemC33Solve as
-# written: 17 per term of the 6561-term CC^{2222} sum, 9 x 81 x 5 for Ce_inv, 4 x 81 x 8 for
-# the stress entries read later, about 1e3 for the 3x3 inverses and the update.
-FLOPS_REF_QP_ITER = 17 * 6561 + 9 * 81 * 5 + 4 * 81 * 8 + 1000
+# written: 17 per term of the 6561-term CC^{2222} sum (81 blocks of 81 terms), 9 x 81 x 5 for
+# Ce_inv, 4 x 81 x 8 for the stress entries read later, about 1e3 for the 3x3 inverses and
+# the update.  At the mid-surface point of a flat reference shell the solve skips the 76
+# blocks whose outer factor is an exact zero, so only 5 of the 81 blocks execute there.
+FLOPS_CC_TERM = 17
+FLOPS_REF_QP_REST = 9 * 81 * 5 + 4 * 81 * 8 + 1000
+FLOPS_REF_QP_ITER = FLOPS_CC_TERM * 6561 + FLOPS_REF_QP_REST
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "membrane_256x192_hbm_traffic.md")
+TRAFFIC_PROFILE_REL = "profiles/membrane_256x192_hbm_traffic.md"
+
+
+def flops_handover(n_elem, n_qp, iters, mid_blocks=5):
+    """Reference-order FP64 operations the hand-over solve executes for n_elem elements of a
+    flat reference shell (every qp solved): the outer qps run all 81 CC blocks, the
+    mid-surface point of an odd rule mid_blocks of them."""
+    mid = 1 if (n_qp > 1 and n_qp % 2 == 1) else 0
+    outer = n_qp - mid
+    per_mid = FLOPS_CC_TERM * 81 * mid_blocks + FLOPS_REF_QP_REST
+    return n_elem * iters * (outer * FLOPS_REF_QP_ITER + mid * per_mid)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def profiled_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (calibrated against a 256 MiB copy), or None when the summary has no row."""
+    summary (calibrated against a 256 MiB copy), or None when the summary has no row.  The
+    counters are read by a separate profiled run of the same workload, not by this one."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
@@ -56,20 +83,17 @@ def log(msg):
 def byte_model(m, n_qp):
     """Algorithmic bytes of one force evaluation (compulsory traffic in this layout).
 
-    element kernel reads: coordinates 24 B/node; per element the 12-slot patch (48 B,
-    nen slots for irregular elements), packed meta (4 B), geom0 cache a1,a2,a3,a3_1,a3_2
-    (120 B), dA0 (8 B), fibre (24 B), gamma_scale (8 B).
-    gather kernel writes: Fint 24 B/node.
-    Overhead of the two-pass deterministic assembly (not algorithmic): the element-force
-    buffer written and read back (2 x 24 B per element slot), the node's (element, slot)
-    list (4 B per slot) and ranges (4 B per node), read by the gather.
+    element phase reads: coordinates 24 B/node; per element the 12-slot patch (48 B, nen
+    slots for irregular elements), packed meta (4 B), geom0 cache a1,a2,a3,a3_1,a3_2
+    (120 B), dA0 (8 B), fibre (24 B), gamma_scale (8 B); it writes Fint, 24 B/node.
+    Not algorithmic: the tiling's own index arrays (each lane's element, each slot's local
+    node, the tile's node list and contribution list) and the partial sums of nodes shared
+    by several tiles, which the node pass adds.
     """
     nen = m.nen().astype(np.int64)
     per_elem = int((4 * nen).sum()) + m.n_elmt * (4 + 120 + 8 + 24 + 8)
-    elem_kernel = 24 * m.n_nodes + per_elem
-    path = elem_kernel + 24 * m.n_nodes
-    overhead = int((2 * 3 * 8 * nen).sum()) + int((4 * nen).sum()) + 4 * (m.n_v + 1)
-    return elem_kernel, path, overhead
+    elem_kernel = 24 * m.n_nodes + per_elem + 24 * m.n_nodes
+    return elem_kernel, elem_kernel
 
 
 def run_gpu(fe, lib, L, prec, steps, warmup):
@@ -94,15 +118,16 @@ def run_gpu(fe, lib, L, prec, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     n = C.c_longlong()
-    te, th, tg = C.c_double(), C.c_double(), C.c_double()
+    te, th, tg, ts = C.c_double(), C.c_double(), C.c_double(), C.c_double()
     L.check("timing_read", lib.bf_timing_read_split(ctx, C.byref(n), C.byref(te), C.byref(th),
                                                     C.byref(tg)))
+    L.check("timing_read_solve", lib.bf_timing_read_solve(ctx, None, C.byref(ts)))
     L.check("timing", lib.bf_timing_enable(ctx, 0))
     e, code, warn = fe.last_error()
     if code == L.BF_ELEM_NONFINITE:
         raise RuntimeError(f"non-finite element force at element {e}")
     k = max(n.value, 1)
-    return wall, n.value, te.value / k, th.value / k, tg.value / k
+    return wall, n.value, te.value / k, th.value / k, tg.value / k, ts.value / k
 
 
 def run_batched(args, L, log):
@@ -124,7 +149,7 @@ def run_batched(args, L, log):
     E = sum(m.n_elmt for m, _, _ in cases)
     q = cases[0][1].n_qp
     elem_bytes = sum(byte_model(m, q)[0] for m, _, _ in cases)
-    path_bytes = sum(byte_model(m, q)[1] for m, _, _ in cases)
+    path_bytes = elem_bytes
     log(f"batched: {len(cases)} membranes, {E} elements, setup {time.perf_counter() - t0:.1f} s")
     tuned = b.tune(L.BF_PREC_MIXED, reps=3)
     log(f"batched: tuned launch {tuned}, {b.info()}")
@@ -214,7 +239,7 @@ def run_ellipsoid(args, L, log):
             forces[name] = fe.finternal_act(x, y, z, precision=prec)
             handed = fe.fallback_count() if prec != L.BF_PREC_FP64 else 0
             steps = args.steps if name != "fp64_ref_order" else max(3, args.steps // 10)
-            wall, _, t_elem, t_hand, t_gather = run_gpu(fe, lib, L, prec, steps, args.warmup)
+            wall, _, t_elem, t_hand, t_gather, _ = run_gpu(fe, lib, L, prec, steps, args.warmup)
             out[name] = dict(steps=steps, ms_per_step=1e3 * wall / steps,
                              elem_qp_per_s=E * q * steps / wall, kernel_ms_element=t_elem,
                              kernel_ms_handover=t_hand, kernel_ms_gather=t_gather,
@@ -418,8 +443,9 @@ def main():
     handed64 = fe.fallback_count()
     fmx = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
     handedmx = fe.fallback_count()
+    pairsmx = int(fe.handed_qps().sum())
     fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64)
-    elem_bytes, path_bytes, overhead_bytes = byte_model(m, q)
+    elem_bytes, path_bytes = byte_model(m, q)
 
     res = {}
     modes = [(n, v) for n, v in (("fp64", L.BF_PREC_FP64_FAST), ("fp64_ref_order", L.BF_PREC_FP64),
@@ -427,7 +453,7 @@ def main():
     for name, prec in modes:
         steps = args.steps if name != "fp64_ref_order" else max(3, args.steps // 10)
         log(f"timing {name}: warmup {args.warmup}, steps {steps}")
-        wall, n, t_elem, t_hand, t_gather = run_gpu(fe, lib, L, prec, steps, args.warmup)
+        wall, n, t_elem, t_hand, t_gather, t_solve = run_gpu(fe, lib, L, prec, steps, args.warmup)
         ms = 1e3 * wall / steps
         res[name] = dict(
             steps=steps,
@@ -435,42 +461,51 @@ def main():
             elem_qp_per_s=E * q * steps / wall,
             kernel_ms_element=t_elem,
             kernel_ms_handover=t_hand,
+            kernel_ms_handover_solve=t_solve,
             kernel_ms_gather=t_gather,
             timed_launches=n,
             gbps_element_kernel=elem_bytes / (t_elem * 1e-3) / 1e9,
             gbps_path=path_bytes / ((t_elem + t_hand + t_gather) * 1e-3) / 1e9,
         )
         log(f"{name}: {ms:.4f} ms/step, element kernel {t_elem:.4f} ms, hand-over "
-            f"{t_hand:.4f} ms, gather {t_gather:.4f} ms")
+            f"{t_hand:.4f} ms (solve {t_solve:.4f} ms), node pass {t_gather:.4f} ms")
 
     cpu = None
     parity = {}
     if args.cpu:
         from oracle import bf_oracle as O
-        log("CPU oracle baseline (single thread, full workload, geom0 prebuilt)")
+        nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        nth = args.cpu_threads or min(nproc or 1, 64)
+        log(f"CPU oracle baseline ({nth} threads and one thread, full workload, geom0 prebuilt)")
         cached = O.Cached(m, p, omp=True)
-        fref = np.zeros(3 * m.n_nodes)
         xs, ys, zs = (np.ascontiguousarray(a) for a in (x, y, z))
+        fomp = np.zeros(3 * m.n_nodes)
+        cached.eval(xs, ys, zs, fomp, nthreads=nth)   # warm-up (thread pool, pages)
+        reps = []
+        for _ in range(5):
+            fomp = np.zeros(3 * m.n_nodes)
+            t0 = time.perf_counter()
+            cached.eval(xs, ys, zs, fomp, nthreads=nth)
+            reps.append(time.perf_counter() - t0)
+        tomp = float(np.median(reps))
+        fref = np.zeros(3 * m.n_nodes)
         t0 = time.perf_counter()
         st = cached.eval(xs, ys, zs, fref, nthreads=1)
         t1 = time.perf_counter() - t0
-        nth = args.cpu_threads or min(os.cpu_count() or 1, 64)
-        fomp = np.zeros(3 * m.n_nodes)
-        t0 = time.perf_counter()
-        cached.eval(xs, ys, zs, fomp, nthreads=nth)
-        tomp = time.perf_counter() - t0
         cached.close()
-        cpu = {"value": E * q / t1, "unit": "elem-QP/s", "cores": 1, "kind": "port",
-               "sample": f"one full FInternalAct of the {args.nx}x{args.ny} membrane "
-                         f"({E} elements x {q} qp, {t1:.1f} s), oracle -O3 x86-64-v3, "
-                         "geom0 prebuilt",
-               "omp": {"value": E * q / tomp, "threads": nth, "seconds": tomp}}
+        cpu = {"value": E * q / tomp, "unit": "elem-QP/s", "cores": nth, "kind": "port",
+               "sample": f"full FInternalAct of the {args.nx}x{args.ny} membrane ({E} elements x "
+                         f"{q} qp), oracle -O3 x86-64-v3 with OpenMP over elements on {nth} host "
+                         f"threads (median of 5 after a warm-up: {tomp:.3f} s), geom0 prebuilt",
+               "host": {"cpu_model": cpu_model(), "nproc": nproc, "os_cpu_count": os.cpu_count()},
+               "single_thread": {"value": E * q / t1, "seconds": t1, "cores": 1}}
         nrm = np.linalg.norm(fref)
         parity = {"fp64_rel_err": float(np.linalg.norm(f64 - fref) / nrm),
                   "fp64_ref_order_rel_err": float(np.linalg.norm(fexact - fref) / nrm),
                   "fp64_ref_order_bitwise_equal": bool(np.array_equal(fexact, fref)),
                   "mixed_rel_err": float(np.linalg.norm(fmx - fref) / nrm),
                   "elements_handed_to_ref_order": {"fp64": handed64, "mixed": handedmx},
+                  "qps_solved_in_ref_order": {"fp64": handed64 * q, "mixed": pairsmx},
                   "omp_bitwise_equal": bool(np.array_equal(fomp, fref)),
                   "oracle_status": list(st)}
         log(f"cpu {t1:.2f} s single, {tomp:.2f} s x{nth}; parity {parity}")
@@ -494,13 +529,13 @@ def main():
         return
     r64 = res["fp64"]
     # dominant kernel of the headline step
-    per_elem = elem_bytes / max(E, 1)
-    if r64["kernel_ms_handover"] >= max(r64["kernel_ms_element"], r64["kernel_ms_gather"]):
-        dom, dom_ms = "bf_exact_solve_queue_kernel", r64["kernel_ms_handover"]
+    per_elem = (elem_bytes - 48 * m.n_nodes) / max(E, 1)
+    if r64["kernel_ms_handover_solve"] >= max(r64["kernel_ms_element"], r64["kernel_ms_gather"]):
+        dom, dom_ms = "bf_exact_solve_queue_kernel", r64["kernel_ms_handover_solve"]
         dom_bytes = int(per_elem * handed64)
-        dom_flops = handed64 * q * p.c33_iters * FLOPS_REF_QP_ITER
+        dom_flops = flops_handover(handed64, q, p.c33_iters)
     else:
-        dom, dom_ms = "bf_element_kernel<double>", r64["kernel_ms_element"]
+        dom, dom_ms = "bf_tile_kernel<double>", r64["kernel_ms_element"]
         dom_bytes, dom_flops = elem_bytes, None
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roof = {
@@ -510,18 +545,26 @@ def main():
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBPS,
         "traffic": profiled_traffic(dom),
+        "traffic_source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary {TRAFFIC_PROFILE_REL} "
+                          "(separate profiled runs of this workload; the FP64 and FP32-mixed "
+                          "dispatches of the kernel are listed apart there)",
         "kernel": dom,
         "algorithmic_bytes_per_launch": dom_bytes,
         "kernel_ms": dom_ms,
+        "kernel_ms_source": "HIP events on the evaluation stream around the hand-over solve "
+                            "kernels (bf_timing_read_solve), averaged over the timed steps",
     }
     if dom_flops is not None:
         tf = dom_flops / (dom_ms * 1e-3) / 1e12
-        roof.update(binding="valu-fp64", note="reference-order recomputation of the elements "
-                    "handed over (solve, finish and redo kernels, timed together): bound by the "
-                    "FP64 vector ALU and the sequential CC3333 sum, far right of the HBM ridge; "
-                    "flops are the reference's operation count (zero blocks included)",
+        roof.update(binding="valu-fp64", note="reference-order C33 solve of the elements handed "
+                    "over (every qp; the CC2222 sum's 6561 ordered additions are one dependent "
+                    "chain per (element, qp, iteration)): bound by the FP64 vector ALU and that "
+                    "chain, far right of the HBM ridge",
                     valu={"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                          "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": dom_flops})
+                          "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": dom_flops,
+                          "flops_counted": "the reference's operation count of the blocks the "
+                                           "solve executes (all 81 CC blocks at the outer qps, "
+                                           "5 at the mid-surface point of this flat reference)"})
     staged_gbps = r64["gbps_element_kernel"]
     out = {
         "metric": METRIC,
@@ -549,8 +592,9 @@ def main():
         "roofline": roof,
         "roofline_staged_kernel": {
             "bound": "hbm", "achieved": staged_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": staged_gbps / HBM_PEAK_GBPS, "traffic": profiled_traffic("bf_element_kernel<double>"),
-            "kernel": "bf_element_kernel<double>", "algorithmic_bytes_per_launch": elem_bytes,
+            "frac": staged_gbps / HBM_PEAK_GBPS, "traffic": profiled_traffic("bf_tile_kernel<double>"),
+            "traffic_source": TRAFFIC_PROFILE_REL,
+            "kernel": "bf_tile_kernel<double>", "algorithmic_bytes_per_launch": elem_bytes,
             "kernel_ms": r64["kernel_ms_element"]},
         "cpu_baseline": cpu,
         "fp32_mixed": dict(res["mixed"], speedup_vs_fp64=res["mixed"]["elem_qp_per_s"] / r64["elem_qp_per_s"],
@@ -562,7 +606,7 @@ def main():
                                speedup_vs_fp64=res["fp64_ref_order"]["elem_qp_per_s"] / r64["elem_qp_per_s"])
                           if "fp64_ref_order" in res else None,
         "fp64_detail": r64,
-        "bytes": {"element_kernel": elem_bytes, "path": path_bytes, "overhead_element_buffer": overhead_bytes},
+        "bytes": {"element_phase": elem_bytes, "path": path_bytes},
         "parity": parity,
         "batched": batched,
         "served_sweep": served,

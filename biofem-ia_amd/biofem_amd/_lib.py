@@ -103,6 +103,7 @@ SIGNATURES = {
     "bf_timing_enable": (C.c_int, [_vp, C.c_int]),
     "bf_timing_read": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp, _dp]),
     "bf_timing_read_split": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp, _dp, _dp]),
+    "bf_timing_read_solve": (C.c_int, [_vp, C.POINTER(C.c_longlong), _dp]),
     "bf_state_init": (C.c_int, [_vp, _dp, C.POINTER(C.c_ubyte), _dp, C.c_double, C.c_double]),
     "bf_step": (C.c_int, [_vp, C.c_int, C.c_int]),
     "bf_state_download": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),

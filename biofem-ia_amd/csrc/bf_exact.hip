@@ -911,7 +911,8 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, int n_cu, bool wide,
-                                         bool pairs, hipStream_t s) {
+                                         bool pairs, hipEvent_t ev_solve0, hipEvent_t ev_solve1,
+                                         hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
   // persistent grids of one wave per SIMD (the solve's occupancy; the list length is known
   // on the device only).  With `wide` the wide solve kernel runs first and leaves the
@@ -919,6 +920,10 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
   // reads the list length itself.
   const int* list = pairs ? m.qlist : m.fallback;
   const int full = pairs ? 8 : 4;
+  if (ev_solve0) {
+    const hipError_t e = hipEventRecord(ev_solve0, s);
+    if (e != hipSuccess) return e;
+  }
   if (wide)
     hipLaunchKernelGGL(bf_exact_solve_wide_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
                        z, list, (const int*)&m.status[full], pairs ? 1 : 0, &m.status[6],
@@ -926,6 +931,10 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
   hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
                      z, list, (const int*)&m.status[wide ? 7 : full], pairs ? 1 : 0,
                      &m.status[6], 4 * n_cu);
+  if (ev_solve1) {
+    const hipError_t e = hipEventRecord(ev_solve1, s);
+    if (e != hipSuccess) return e;
+  }
   // 64-lane blocks: a short hand-over list still spreads over the CUs
   hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x, y, z,
                      (const int*)m.fallback, (const int*)&m.status[4], 0,

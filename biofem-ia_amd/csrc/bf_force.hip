@@ -941,18 +941,19 @@ struct bf_ctx {
   int device = 0;
   int n_cu = 256;  // compute units of the device (persistent grid sizes)
   hipStream_t stream = nullptr;
-  // per evaluation: before the element kernel, after it, after the hand-over kernel,
-  // after the gather
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // per evaluation: before the element kernel, after it, after the hand-over kernels,
+  // after the gather; before and after the hand-over solve kernels
+  static constexpr int kEv = 6;
+  hipEvent_t ev[kEv] = {};
   bool timing = false;
-  hipEvent_t own_ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  // timing ring: 4 events per evaluation, harvested when full or on bf_timing_read
+  hipEvent_t own_ev[kEv] = {};
+  // timing ring: kEv events per evaluation, harvested when full or on bf_timing_read
   static constexpr int kRing = 64;
   std::vector<hipEvent_t> ring;
   int ring_n = 0;
   bool ring_on = false;
   long long t_count = 0;
-  double t_elem = 0.0, t_handover = 0.0, t_gather = 0.0;
+  double t_elem = 0.0, t_handover = 0.0, t_gather = 0.0, t_solve = 0.0;
   // mesh
   int n_v = 0, n_nodes = 0, E = 0, NS = 12, n_elmt_base = 0, nnz = 0, n_ghost = 0;
   bool has_mesh = false, has_params = false, has_delay = false, has_cap = false;
@@ -1154,15 +1155,17 @@ int upload_tables(int device) {
 
 int harvest_ring(bf_ctx* c) {
   for (int i = 0; i < c->ring_n; i++) {
-    hipEvent_t* r = &c->ring[4 * i];
+    hipEvent_t* r = &c->ring[bf_ctx::kEv * i];
     BF_HIP(hipEventSynchronize(r[3]));
-    float a = 0.f, h = 0.f, b = 0.f;
+    float a = 0.f, h = 0.f, b = 0.f, v = 0.f;
     BF_HIP(hipEventElapsedTime(&a, r[0], r[1]));
     BF_HIP(hipEventElapsedTime(&h, r[1], r[2]));
     BF_HIP(hipEventElapsedTime(&b, r[2], r[3]));
+    BF_HIP(hipEventElapsedTime(&v, r[4], r[5]));
     c->t_elem += a;
     c->t_handover += h;
     c->t_gather += b;
+    c->t_solve += v;
     c->t_count++;
   }
   c->ring_n = 0;
@@ -1251,15 +1254,15 @@ int bf_ctx_create(int device, bf_ctx** out) {
   int rc = upload_tables(device);
   if (rc == BF_OK) rc = noalloc_scratch_init(device);
   if (rc != BF_OK) { delete c; return rc; }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->own_ev[0]) != hipSuccess || hipEventCreate(&c->own_ev[1]) != hipSuccess ||
-      hipEventCreate(&c->own_ev[2]) != hipSuccess || hipEventCreate(&c->own_ev[3]) != hipSuccess ||
+  bool events = true;
+  for (auto& e : c->own_ev) events = events && hipEventCreate(&e) == hipSuccess;
+  if (!events || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->d_status, kStatusWords * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_status, kStatusWords * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     bf_ctx_destroy(c);
     return BF_ERR_DEVICE;
   }
-  for (int i = 0; i < 4; i++) c->ev[i] = c->own_ev[i];
+  for (int i = 0; i < bf_ctx::kEv; i++) c->ev[i] = c->own_ev[i];
   c->kp.fallback = 1;
   *out = c;
   return BF_OK;
@@ -1612,10 +1615,10 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
       int rc = harvest_ring(c);
       if (rc != BF_OK) return rc;
     }
-    ev = &c->ring[4 * c->ring_n];
+    ev = &c->ring[bf_ctx::kEv * c->ring_n];
     c->ring_n++;
   }
-  for (int i = 0; i < 4; i++) c->ev[i] = ev[i];  // bf_last_timing reads the latest set
+  for (int i = 0; i < bf_ctx::kEv; i++) c->ev[i] = ev[i];  // bf_last_timing reads the latest set
   hipEvent_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
   double* fout = step ? c->d_fint : fint;
   const int acc = step ? 0 : accumulate;
@@ -1648,9 +1651,13 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
                        (const int*)td.wqlist, n_waves, pairs ? kp.n_qp : 0, dm.fallback,
                        dm.qlist, dm.status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
-                                         pairs, s));
+                                         pairs, ev[4], ev[5], s));
     c->fb_prec = precision;
     c->last_qlist = pairs ? dm.qlist : dm.fallback;
+  }
+  if (c->fb_prec < 0) {  // no hand-over solve: an empty interval
+    BF_HIP(hipEventRecord(ev[4], s));
+    BF_HIP(hipEventRecord(ev[5], s));
   }
   BF_HIP(hipEventRecord(e2, s));
   if (staged) {
@@ -1901,7 +1908,7 @@ int bf_timing_enable(bf_ctx* c, int on) {
   if (!c) return BF_ERR_ARG;
   BF_HIP(hipSetDevice(c->device));
   if (on && c->ring.empty()) {
-    c->ring.assign(4 * bf_ctx::kRing, nullptr);
+    c->ring.assign(bf_ctx::kEv * bf_ctx::kRing, nullptr);
     for (auto& e : c->ring) BF_HIP(hipEventCreate(&e));
   }
   if (!on && c->ring_on) {
@@ -1911,7 +1918,7 @@ int bf_timing_enable(bf_ctx* c, int on) {
   c->ring_on = on != 0;
   c->ring_n = 0;
   c->t_count = 0;
-  c->t_elem = c->t_handover = c->t_gather = 0.0;
+  c->t_elem = c->t_handover = c->t_gather = c->t_solve = 0.0;
   return BF_OK;
 }
 
@@ -1936,6 +1943,16 @@ int bf_timing_read_split(bf_ctx* c, long long* count, double* ms_element,
   if (ms_element) *ms_element = c->t_elem;
   if (ms_handover) *ms_handover = c->t_handover;
   if (ms_gather) *ms_gather = c->t_gather;
+  return BF_OK;
+}
+
+int bf_timing_read_solve(bf_ctx* c, long long* count, double* ms_solve) {
+  if (!c) return BF_ERR_ARG;
+  BF_HIP(hipSetDevice(c->device));
+  int rc = harvest_ring(c);
+  if (rc != BF_OK) return rc;
+  if (count) *count = c->t_count;
+  if (ms_solve) *ms_solve = c->t_solve;
   return BF_OK;
 }
 

@@ -385,9 +385,12 @@ __host__ __device__ inline bool split_fits(int N, int N_mid, int P, bool pairs, 
 // items from status[6], which must be zero on entry).  `wide` also
 // launches the solve kernel for short lists (split of kWideSplitMin lanes or more); without
 // it the narrow kernel takes every list -- the sums, and so the results, are the same.
+// ev_solve0 / ev_solve1 (may be null) are recorded on s right before and after the solve
+// kernels.
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, int n_cu, bool wide,
-                                         bool pairs, hipStream_t s);
+                                         bool pairs, hipEvent_t ev_solve0, hipEvent_t ev_solve1,
+                                         hipStream_t s);
 // Batch forms: the reference-order path over the element tiles (plain FP64 batches) and
 // over the compacted hand-over list b.hlist[0 .. *b.htotal).  b.rtotal[0..1] must be zero
 // on entry (the prologue clears them).

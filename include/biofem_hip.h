@@ -194,6 +194,10 @@ int bf_timing_read(bf_ctx *ctx, long long *count, double *ms_element, double *ms
  * No reference counterpart (instrumentation). */
 int bf_timing_read_split(bf_ctx *ctx, long long *count, double *ms_element,
                          double *ms_handover, double *ms_gather);
+/* Total milliseconds of the hand-over solve kernels alone (the reference-order C33 solve of
+ * the handed-over (element, qp) items, inside ms_handover) over the same evaluations.  No
+ * reference counterpart (instrumentation). */
+int bf_timing_read_solve(bf_ctx *ctx, long long *count, double *ms_solve);
 
 /* Device pointers owned by the context, for callers that keep state resident:
  * coordinates and force buffers of n_nodes (3 * n_nodes for forces). */

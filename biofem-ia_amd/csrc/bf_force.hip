@@ -503,8 +503,12 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
     }
   }
   if (elem_thread) {
-    const int w = t * (T >> 6) + (tid >> 6);
-    HandSink{td.wlist, td.wqlist, td.wcount, w}.push(live && handed, qfail, e, P);
+    if (sizeof(R) == 8) {  // straight into the hand-over list: no compaction pass
+      push_hand_list(td, live && handed, e);
+    } else {  // per-wave segments, which the second tier re-evaluates
+      const int w = t * (T >> 6) + (tid >> 6);
+      HandSink{td.wlist, td.wqlist, td.wcount, w}.push(live && handed, qfail, e, P);
+    }
   }
   lds_barrier();  // region A is read: the slot forces take its place
   if (elem_thread) {
@@ -804,6 +808,7 @@ __global__ void __launch_bounds__(256) bf_batch_prologue(BatchDev b, int ghosts)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     b.rtotal[0] = 0;
     b.rtotal[1] = 0;
+    *b.htotal = 0;  // the FP64 tile kernel appends to hlist
   }
   if (!ghosts) return;
   for (int t = blockIdx.x; t < b.n_gtiles; t += gridDim.x) {
@@ -1606,7 +1611,11 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     return BF_ERR_ARG;
   const KParams kp = dev_params(c);
   const DevMesh dm = view ? *view : dev_mesh(c);
-  const TileDev td = tview ? *tview : tile_dev(c);
+  TileDev td = tview ? *tview : tile_dev(c);
+  td.list_count = dm.status + 4;  // status[4], zeroed by bf_status_reset
+  td.list_e = dm.fallback;
+  td.list_ce = nullptr;
+  td.list_case = 0;
   const bool staged = precision != BF_PREC_FP64;
   hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
   hipEvent_t* ev = c->own_ev;
@@ -1646,10 +1655,11 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
       hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3((n_waves + 3) / 4), dim3(kStagedBlock), 0, s,
                          dm, kp, td, n_waves, x, y, z);
     const bool pairs = precision == BF_PREC_MIXED;
-    hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s,
-                       (const unsigned long long*)td.wcount, (const int*)td.wlist,
-                       (const int*)td.wqlist, n_waves, pairs ? kp.n_qp : 0, dm.fallback,
-                       dm.qlist, dm.status);
+    if (pairs)  // the FP64 tile kernel listed its elements itself
+      hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s,
+                         (const unsigned long long*)td.wcount, (const int*)td.wlist,
+                         (const int*)td.wqlist, n_waves, kp.n_qp, dm.fallback, dm.qlist,
+                         dm.status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
                                          pairs, ev[4], ev[5], s));
     c->fb_prec = precision;
@@ -2233,6 +2243,10 @@ int batch_build(bf_batch* b) {
 #undef BALLOC
   for (int i = 0; i < nc; i++) {
     cases[i].td.wcount = b->d_tcount + case_wave[i];
+    cases[i].td.list_count = b->d_htotal;
+    cases[i].td.list_e = nullptr;
+    cases[i].td.list_ce = b->d_hlist;
+    cases[i].td.list_case = i;
   }
   hipStream_t s = b->stream;
   BF_HIP(hipMemcpyAsync(b->d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, s));
@@ -2304,8 +2318,8 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
       hipLaunchKernelGGL(bf_batch_tier2_kernel, dim3(b->n_cu * 2), dim3(kStagedBlock), 0, s, d);
     }
     const bool pairs = precision == BF_PREC_MIXED;
-    hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d,
-                       pairs ? 1 : 0);
+    if (pairs)  // the FP64 tile kernel listed its elements itself
+      hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d, 1);
     BF_HIP(launch_batch_exact_list(d, b->n_cu, pairs, s));
   }
   BF_HIP(launch_batch_tile_fixup(d, step, s));

@@ -68,6 +68,14 @@ struct TileDev {
   int* wqlist;                  // [n_tiles * T * kMaxQp] (element, qp) pair segments,
                                 // 64 entries per wave and qp (pair_segment)
   unsigned long long* wcount;   // [n_tiles * T / 64] packed segment counts (HandSink)
+  // The FP64 path lists its handed-over elements directly in the hand-over list, at offsets
+  // from one device-scope atomic per wave on *list_count (the list's order does not matter:
+  // every listed element is solved and finished on its own): list_e (single context,
+  // elements) or list_ce (batch, (list_case, element)).
+  int* list_count;
+  int* list_e;
+  int2* list_ce;
+  int list_case;
 };
 
 // status[4]: elements handed over (m.fallback)
@@ -175,6 +183,22 @@ struct HandSink {
     if (lane == leader) count[wave] = word;
   }
 };
+
+// Appends e of every lane with `take` to the FP64 path's hand-over list (TileDev list_*):
+// one atomic per wave that lists anything.  Called by every active lane of the wave.
+__device__ inline void push_hand_list(const TileDev& td, bool take, int e) {
+  const unsigned long long bal = __ballot(take);
+  if (bal == 0ull) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int leader = __ffsll((long long)bal) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(td.list_count, __popcll(bal));
+  base = __shfl(base, leader);
+  if (!take) return;
+  const int at = base + __popcll(bal & ((1ull << lane) - 1ull));
+  if (td.list_ce) td.list_ce[at] = make_int2(td.list_case, e);
+  else td.list_e[at] = e;
+}
 
 // The form of a wave that works through one segment in passes of floor(64 / P) elements
 // (the FP32-mixed path's second tier) and appends to a segment; `pushed` (wave-uniform)

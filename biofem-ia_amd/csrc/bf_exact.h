@@ -232,17 +232,50 @@ BF_HD double cc_sum_serial(const CcIn& c) {
   }
   return CC;
 }
+// ---- ElemElasCGDefTens :1389-1430: entry (i, j) of Ce_inv.Cont, one sum over m, p, q, l
+// in that order (i, j may differ between the lanes that call it) ----
+BF_HD double ce_entry(const Mat& FaCont, const Mat& gm0, const Mat& Ci, int i, int j) {
+  double Fr[3], Fcj[3];
+  row3(FaCont, i, Fr);
+  col3(FaCont, j, Fcj);
+  const double Fo[3] = {opaque(Fr[0]), opaque(Fr[1]), opaque(Fr[2])};
+  double acc = 0.0;
+#pragma unroll
+  for (int mm = 0; mm < 3; mm++)
+#pragma unroll
+    for (int pp = 0; pp < 3; pp++)
+#pragma unroll
+      for (int qq = 0; qq < 3; qq++)
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+          acc += Fo[mm] * gm0.m[mm][pp] * Ci.m[pp][qq] * Fcj[l] * gm0.m[l][qq];
+  return acc;
+}
+
 struct CcSerial {
   BF_HD double operator()(const CcIn& c) const { return cc_sum_serial(c); }
-  // the qp's other sums: every entry is formed by this lane
-  BF_HD bool mine(int) const { return true; }
-  BF_HD double share(double v, int) const { return v; }
+  // the nine entries of Ce_inv.Cont, one after the other
+  BF_HD void ce_inv(const Mat& FaCont, const Mat& gm0, const Mat& Ci, Mat& Ch) const {
+#pragma unroll 1
+    for (int i = 0; i < 3; i++)
+#pragma unroll 1
+      for (int j = 0; j < 3; j++) {
+        const double v = ce_entry(FaCont, gm0, Ci, i, j);
+        if (i == 0) {
+          if (j == 0) Ch.m[0][0] = v; else if (j == 1) Ch.m[0][1] = v; else Ch.m[0][2] = v;
+        } else if (i == 1) {
+          if (j == 0) Ch.m[1][0] = v; else if (j == 1) Ch.m[1][1] = v; else Ch.m[1][2] = v;
+        } else {
+          if (j == 0) Ch.m[2][0] = v; else if (j == 1) Ch.m[2][1] = v; else Ch.m[2][2] = v;
+        }
+      }
+  }
 };
 
 // One through-thickness point: ElemUpdateG, ElemActDefGrad, ElemCGDefTens and the C33
-// loop for qp `qp`.  cc_sum forms CC^{2222}; lanes that solve the same qp together also
-// divide the nine Ce_inv entries among themselves (cc_sum.mine(entry): this lane forms it,
-// cc_sum.share(value, entry): the owner's value), each entry still one sum in order.  ModElemC33 returns at the first qp whose |CC3333| fails the guard
+// loop for qp `qp`.  cc_sum forms CC^{2222} and the nine Ce_inv entries (cc_sum.ce_inv):
+// lanes that solve the same qp together divide both among themselves, each entry still
+// one sum in the reference's order.  ModElemC33 returns at the first qp whose |CC3333| fails the guard
 // (:1674-1681), so in iteration `it` this qp keeps its C33 if it or any earlier qp failed:
 // skip(fail, it) receives this qp's guard result and answers that question (sequentially
 // from a bit mask, or across the lanes that hold the element's other qps).
@@ -312,41 +345,7 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
   for (int it = 0; it < in.iters; it++) {
     // ---- ElemElasCGDefTens :1389-1430: sum over m, p, q, l per entry ----
     Mat Ch, CeiCov, CeCont, CeCov;
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) Ch.m[i][j] = 0.0;
-#pragma unroll 1
-    for (int i = 0; i < 3; i++) {
-      double Fr[3];
-      row3(FaCont, i, Fr);
-#pragma unroll 1
-      for (int j = 0; j < 3; j++) {
-        if (!cc_sum.mine(3 * i + j)) continue;
-        double Fcj[3];
-        col3(FaCont, j, Fcj);
-        const double Fo[3] = {opaque(Fr[0]), opaque(Fr[1]), opaque(Fr[2])};
-        double acc = 0.0;
-#pragma unroll
-        for (int mm = 0; mm < 3; mm++)
-#pragma unroll
-          for (int pp = 0; pp < 3; pp++)
-#pragma unroll
-            for (int qq = 0; qq < 3; qq++)
-#pragma unroll
-              for (int l = 0; l < 3; l++)
-                acc += Fo[mm] * gm0.m[mm][pp] * Ci.m[pp][qq] * Fcj[l] * gm0.m[l][qq];
-        if (i == 0) {
-          if (j == 0) Ch.m[0][0] = acc; else if (j == 1) Ch.m[0][1] = acc; else Ch.m[0][2] = acc;
-        } else if (i == 1) {
-          if (j == 0) Ch.m[1][0] = acc; else if (j == 1) Ch.m[1][1] = acc; else Ch.m[1][2] = acc;
-        } else {
-          if (j == 0) Ch.m[2][0] = acc; else if (j == 1) Ch.m[2][1] = acc; else Ch.m[2][2] = acc;
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) Ch.m[i][j] = cc_sum.share(Ch.m[i][j], 3 * i + j);
+    cc_sum.ce_inv(FaCont, gm0, Ci, Ch);
     lower2(gm0, Ch, CeiCov);
     inv(CeiCov, CeCont);
     lower2(gm0, CeCont, CeCov);

@@ -210,8 +210,9 @@ __device__ __forceinline__ void exact_solve(const DevMesh& m, const KParams& kp,
 // it back to lane 0 for the next round.  These are the additions of cc_sum_serial in its
 // order, so the sum is bit for bit the serial one, and every addition reads its term from
 // the adding lane's registers (a dependent FP64 add is ~5 cycles there, ~16 when its term
-// comes from another lane through LDS).  Everything else of the iteration is formed by all
-// K lanes alike.  A wave holds split_groups(K) = 64 / K groups.
+// comes from another lane through LDS).  The nine Ce_inv entries of the iteration are divided
+// among the K lanes the same way (ce_inv); everything else is formed by all K lanes alike.
+// A wave holds split_groups(K) = 64 / K groups.
 
 // Lane i receives lane i - 1's v (lane 0 keeps its own): DPP wave_shr:1, the whole-wave
 // shift of GFX9's data-parallel primitives.
@@ -255,10 +256,21 @@ struct CcSplit {
     }
     return __shfl(CC, lead);
   }
-  // Ce_inv entry k is formed by lane k % K of the group
-  __device__ bool mine(int k) const { return k % K == j; }
-  __device__ double share(double v, int k) const {
-    return __shfl(v, (int)(threadIdx.x & 63) - j + k % K);
+  // The nine Ce_inv entries: in round r lane j forms entry r K + j (all lanes at once, each
+  // entry one sum in the reference's order), then every lane of the group takes each entry
+  // from the lane that formed it.
+  __device__ void ce_inv(const exact::Mat& FaCont, const exact::Mat& gm0, const exact::Mat& Ci,
+                         exact::Mat& Ch) const {
+    constexpr int R = (9 + K - 1) / K;
+    const int lead = (int)(threadIdx.x & 63) - j;
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int k = r * K + j;
+      v[r] = k < 9 ? exact::ce_entry(FaCont, gm0, Ci, k / 3, k % 3) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) Ch.m[k / 3][k % 3] = __shfl(v[k / K], lead + k % K);
   }
 };
 

@@ -252,8 +252,43 @@ BF_HD double ce_entry(const Mat& FaCont, const Mat& gm0, const Mat& Ci, int i, i
   return acc;
 }
 
+// ---- ElemTotStress :1506-1528: entry (i, j) of the total stress, one sum over w, p, z, s in
+// that order (i, j may differ between the lanes that call it) ----
+BF_HD double tot_entry(const Mat& Fi, const Mat& G, const Mat& Se, int i, int j) {
+  double Gi[3], Gj[3], Sei[3];
+  col3(G, i, Gi);
+  col3(G, j, Gj);
+  col3(Se, i, Sei);
+  const double half = opaque(0.5);
+  double acc = 0.0;
+#pragma unroll
+  for (int w = 0; w < 3; w++)
+#pragma unroll
+    for (int pp = 0; pp < 3; pp++)
+#pragma unroll
+      for (int z = 0; z < 3; z++)
+#pragma unroll
+        for (int s = 0; s < 3; s++)
+          acc += half * Fi.m[w][pp] * Fi.m[z][s] * Gj[s] *
+                 (Se.m[pp][z] * Gi[w] + Sei[pp] * G.m[w][z]);
+  return acc;
+}
+// The total-stress entries a C33 iteration reads: S^22, and in the last iteration (whose
+// values are the ones kept) the in-plane S^00, S^11, S^01 as well.
+struct TotOut {
+  double S00, S11, S01, S22;
+};
+
 struct CcSerial {
   BF_HD double operator()(const CcIn& c) const { return cc_sum_serial(c); }
+  BF_HD void stresses(const Mat& Fi, const Mat& G, const Mat& Se, bool last, TotOut& o) const {
+    if (last) {
+      o.S00 = tot_entry(Fi, G, Se, 0, 0);
+      o.S11 = tot_entry(Fi, G, Se, 1, 1);
+      o.S01 = tot_entry(Fi, G, Se, 0, 1);
+    }
+    o.S22 = tot_entry(Fi, G, Se, 2, 2);
+  }
   // the nine entries of Ce_inv.Cont, one after the other
   BF_HD void ce_inv(const Mat& FaCont, const Mat& gm0, const Mat& Ci, Mat& Ch) const {
 #pragma unroll 1
@@ -273,10 +308,10 @@ struct CcSerial {
 };
 
 // One through-thickness point: ElemUpdateG, ElemActDefGrad, ElemCGDefTens and the C33
-// loop for qp `qp`.  cc_sum forms CC^{2222} and the nine Ce_inv entries (cc_sum.ce_inv):
-// lanes that solve the same qp together divide both among themselves, each entry still
-// one sum in the reference's order.  ModElemC33 returns at the first qp whose |CC3333| fails the guard
-// (:1674-1681), so in iteration `it` this qp keeps its C33 if it or any earlier qp failed:
+// loop for qp `qp`.  cc_sum forms CC^{2222}, the nine Ce_inv entries (cc_sum.ce_inv) and
+// the total-stress entries (cc_sum.stresses): lanes that solve the same qp together divide
+// these among themselves, each entry still one sum in the reference's order.  ModElemC33
+// returns at the first qp whose |CC3333| fails the guard (:1674-1681), so in iteration `it` this qp keeps its C33 if it or any earlier qp failed:
 // skip(fail, it) receives this qp's guard result and answers that question (sequentially
 // from a bit mask, or across the lanes that hold the element's other qps).
 struct QpOut {
@@ -364,27 +399,15 @@ BF_HD QpOut integrate_qp(const ElemIn& in, int qp, Skip&& skip, CcSum&& cc_sum,
       }
     // ---- ElemTotStress :1506-1528, the four entries later steps read: S^22 in every
     // iteration, the in-plane ones only in the last, whose values are the ones kept ----
-    auto tot = [&](int i, int j) {
-      const double half = opaque(0.5);
-      double acc = 0.0;
-#pragma unroll
-      for (int w = 0; w < 3; w++)
-#pragma unroll
-        for (int pp = 0; pp < 3; pp++)
-#pragma unroll
-          for (int z = 0; z < 3; z++)
-#pragma unroll
-            for (int s = 0; s < 3; s++)
-              acc += half * Fi.m[w][pp] * Fi.m[z][s] * G.m[s][j] *
-                     (Se.m[pp][z] * G.m[w][i] + Se.m[pp][i] * G.m[w][z]);
-      return acc;
-    };
-    if (it == in.iters - 1) {
-      S00 = tot(0, 0);
-      S11 = tot(1, 1);
-      S01 = tot(0, 1);
+    TotOut st;
+    const bool last = it == in.iters - 1;
+    cc_sum.stresses(Fi, G, Se, last, st);
+    if (last) {
+      S00 = st.S00;
+      S11 = st.S11;
+      S01 = st.S01;
     }
-    const double S22 = tot(2, 2);
+    const double S22 = st.S22;
     // ---- ElemElsTangMatTens :1548-1585 (alpha, beta; entries formed on demand) ----
     const double Je2 = Je * Je;
     const double alpha = in.K * (Je2 - Je) - in.mu;

@@ -224,6 +224,24 @@ __device__ __forceinline__ double from_lane_below(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// Lane lead of each K-lane group receives lane lead + K - 1's v (other lanes receive
+// anything): when K divides 16 a DPP row_ror:(17 - K), which rotates each row of 16 lanes so
+// that lane i receives lane (i + K - 1) mod 16, else a cross-lane permute.
+template <int K>
+__device__ __forceinline__ double from_group_last(double v, int lead) {
+  static_assert(K > 1 && K <= 64, "a group holds 2 to 64 lanes");
+  if constexpr (16 % K == 0) {
+    constexpr int ctrl = 0x120 + (17 - K);
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), ctrl, 0xF,
+                                               0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), ctrl, 0xF,
+                                               0xF, false);
+    return __hiloint2double(hi, lo);
+  } else {
+    return __shfl(v, lead + K - 1);
+  }
+}
+
 template <int K>
 struct CcSplit {
   int j;  // lane within the group
@@ -240,9 +258,8 @@ struct CcSplit {
         act = exact::cc_block_active(c, k);
       }
       if (__ballot(act) == 0ull) continue;
+      // t is read only by lanes with an active block, which form all of it
       double t[81];
-#pragma unroll
-      for (int i = 0; i < 81; i++) t[i] = 0.0;
       if (act) exact::cc_block_terms(c, k, [&](int i, double v) { t[i] = v; });
 #pragma unroll 1
       for (int sl = 0; sl < K; sl++) {
@@ -252,7 +269,7 @@ struct CcSplit {
         }
         if (sl + 1 < K) CC = from_lane_below(CC);
       }
-      CC = __shfl(CC, lead + K - 1);  // back to lane 0 (every lane of the group holds it)
+      CC = from_group_last<K>(CC, lead);  // back to lane 0 of the group
     }
     return __shfl(CC, lead);
   }
@@ -271,6 +288,28 @@ struct CcSplit {
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) Ch.m[k / 3][k % 3] = __shfl(v[k / K], lead + k % K);
+  }
+  // The total-stress entries: S^22 by every lane; in the last iteration the four entries
+  // S^00, S^11, S^01, S^22 are divided among the lanes like the Ce_inv entries.
+  __device__ void stresses(const exact::Mat& Fi, const exact::Mat& G, const exact::Mat& Se,
+                           bool last, exact::TotOut& o) const {
+    if (!last) {
+      o.S22 = exact::tot_entry(Fi, G, Se, 2, 2);
+      return;
+    }
+    constexpr int R = (4 + K - 1) / K;
+    const int lead = (int)(threadIdx.x & 63) - j;
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int k = r * K + j;  // 0: S^00, 1: S^11, 2: S^01, 3: S^22
+      const int ti = k == 1 ? 1 : (k == 3 ? 2 : 0), tj = k == 0 ? 0 : (k == 3 ? 2 : 1);
+      v[r] = k < 4 ? exact::tot_entry(Fi, G, Se, ti, tj) : 0.0;
+    }
+    o.S00 = __shfl(v[0 / K], lead + 0 % K);
+    o.S11 = __shfl(v[1 / K], lead + 1 % K);
+    o.S01 = __shfl(v[2 / K], lead + 2 % K);
+    o.S22 = __shfl(v[3 / K], lead + 3 % K);
   }
 };
 

@@ -817,6 +817,7 @@ hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s) {
 // element-force buffer (written by the reference-order finish or the FP64 second tier), and
 // its touched mark is cleared for the next evaluation.  Ghost rows are zeroed when not
 // accumulating.  Returns whether f holds the force (always for real nodes when need_f).
+constexpr int kNodeChunk = 8;  // loads in flight per node and chunk
 __device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __restrict__ ptr,
                                                 const unsigned* __restrict__ ent,
                                                 const double* __restrict__ Fe, int E, int n_v,
@@ -836,11 +837,26 @@ __device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __
     f[0] = accumulate ? fo[0] : 0.0;
     f[1] = accumulate ? fo[1] : 0.0;
     f[2] = accumulate ? fo[2] : 0.0;
-    for (int j = p0; j < p1; j++) {
-      const double* pr = td.partial + 3 * (size_t)j;
-      f[0] += pr[0];
-      f[1] += pr[1];
-      f[2] += pr[2];
+    // partials in chunks whose loads are all in flight before the first add
+    for (int j0 = p0; j0 < p1; j0 += kNodeChunk) {
+      double px[kNodeChunk], py[kNodeChunk], pz[kNodeChunk];
+#pragma unroll
+      for (int k = 0; k < kNodeChunk; k++) {
+        px[k] = py[k] = pz[k] = 0.0;
+        if (j0 + k < p1) {
+          const double* pr = td.partial + 3 * (size_t)(j0 + k);
+          px[k] = pr[0];
+          py[k] = pr[1];
+          pz[k] = pr[2];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kNodeChunk; k++)
+        if (j0 + k < p1) {
+          f[0] += px[k];
+          f[1] += py[k];
+          f[2] += pz[k];
+        }
     }
     write = true;
   } else if (comp || accumulate) {
@@ -853,14 +869,34 @@ __device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __
   }
   if (tch) {
     td.touched[n] = 0;
-    for (int j = ptr[n]; j < ptr[n + 1]; j++) {
-      const unsigned v = ent[j];
-      const int e = (int)(v >> 5);
-      if (!td.deferred[e]) continue;
-      const double* fe = Fe + 3 * ((size_t)(v & 31u) * E + e);
-      f[0] += fe[0];
-      f[1] += fe[1];
-      f[2] += fe[2];
+    // entries in chunks: the chunk's entries, then their deferred flags, then the rows of
+    // the deferred ones are each loaded together; the rows are added in entry order
+    const int j1 = ptr[n + 1];
+    for (int j0 = ptr[n]; j0 < j1; j0 += kNodeChunk) {
+      unsigned v[kNodeChunk];
+#pragma unroll
+      for (int k = 0; k < kNodeChunk; k++) v[k] = j0 + k < j1 ? ent[j0 + k] : 0u;
+      bool d[kNodeChunk];
+#pragma unroll
+      for (int k = 0; k < kNodeChunk; k++) d[k] = j0 + k < j1 && td.deferred[v[k] >> 5] != 0;
+      double fx[kNodeChunk], fy[kNodeChunk], fz[kNodeChunk];
+#pragma unroll
+      for (int k = 0; k < kNodeChunk; k++) {
+        fx[k] = fy[k] = fz[k] = 0.0;
+        if (d[k]) {
+          const double* fe = Fe + 3 * ((size_t)(v[k] & 31u) * E + (v[k] >> 5));
+          fx[k] = fe[0];
+          fy[k] = fe[1];
+          fz[k] = fe[2];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kNodeChunk; k++)
+        if (d[k]) {
+          f[0] += fx[k];
+          f[1] += fy[k];
+          f[2] += fz[k];
+        }
     }
     write = true;
   }

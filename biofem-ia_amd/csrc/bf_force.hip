@@ -249,8 +249,8 @@ constexpr int kStagedBlock = 256;
 // then the chain-only inputs, which the qp results [4][P][T] replace once every chain thread
 // holds its inputs -- and, once phase 3 has read all of it, the slot forces [3][NS][T];
 // then the qp status bits [P][T] (ints), the tile's local slot codes [NS][T] (ushort), its
-// contribution lists (cent, ushort, at most NS T entries; cptr, lmax + 1 ints) and a touched
-// flag per local node (bytes).
+// contribution lists (cent, ushort, at most NS T entries; cptr, lmax + 1 ints), a touched
+// flag per local node (bytes), and each local node's global index and output code (ints).
 // Every global load that depends on the tile or element index alone is issued before the
 // first barrier, and the barriers order LDS only: the global stores of a phase drain while
 // the next one runs.
@@ -266,7 +266,7 @@ __host__ __device__ inline size_t tile_region_a(int T, int P, int NS, int lmax) 
 }
 __host__ __device__ inline size_t tile_smem_doubles(int T, int P, int NS, int lmax) {
   return tile_region_a(T, P, NS, lmax) + ((size_t)P * T + 1) / 2 + ((size_t)NS * T + 3) / 4 +
-         ((size_t)NS * T + 3) / 4 + ((size_t)lmax + 2) / 2 + ((size_t)lmax + 7) / 8;
+         ((size_t)NS * T + 3) / 4 + ((size_t)lmax + 2) / 2 + ((size_t)lmax + 7) / 8 + (size_t)lmax;
 }
 
 // Workgroup barrier that orders LDS only (global stores stay in flight across it).
@@ -297,53 +297,94 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
   unsigned short* ce = sc + (size_t)((NS * T + 3) / 4) * 4;  // contributions of the tile
   int* cp = (int*)(ce + (size_t)((NS * T + 3) / 4) * 4);    // [lmax + 1] ranges in ce
   unsigned char* tf = (unsigned char*)(cp + ((lm + 2) / 2) * 2);  // [lmax] touched flags
+  int* sn = (int*)(tf + ((lm + 7) / 8) * 8);            // [lmax] global node
+  int* so = sn + lm;                                    // [lmax] output code (TileDev out)
   const int E = kp.E;
   const int q = tid / T, i = tid - q * T;  // q: wave-uniform
   const bool elem_thread = q == 0;
+  // ---- 0. loads, in the order they are needed: a wave waits for its loads in issue
+  //      order, so the tile's node lists and contributions go first, then the node
+  //      coordinates, then what depends on the element index (read in phases 1 and 2) ----
+  const int j0 = td.node_ptr[t], L = td.node_ptr[t + 1] - j0;
+  const int cb = td.cptr[j0], nc = td.cptr[j0 + L] - cb;
+  unsigned short cv[kMaxNen];  // the element threads' slot codes (before the branch-free part)
+  if (elem_thread) {
+    const unsigned short* ls = td.lslot + (size_t)t * NS * T + i;
+#pragma unroll
+    for (int a = 0; a < kMaxNen; a++) cv[a] = ls[(size_t)(a < NS ? a : NS - 1) * T];
+  }
   const int e = td.elem[(size_t)t * T + i];
   const bool live = e >= 0;
   const bool chain = live && q < P;
-  // ---- loads that depend on the tile and the element only ----
-  double rv[15];
-  if (chain) {
+  // first pass: local nodes tid and tid + NT, contributions tid + r NT (r < 4) -- all of
+  // them when n_qp >= 2 and NS <= 4 n_qp; later passes below.  The loads are branch-free
+  // (indices clamped into the tile's ranges, L >= 1 and nc >= 1; dead lanes read element 0),
+  // so that each wait covers exactly the loads it needs.
+  const int la = tid, lb = tid + NT;
+  const bool ha = la < L, hb = lb < L;
+  const int ia = j0 + (ha ? la : L - 1), ib = j0 + (hb ? lb : L - 1);
+  const int na = td.node[ia], nb = td.node[ib];
+  const int ca = td.cptr[ia + 1], cbn = td.cptr[ib + 1];
+  const int oa = td.out[ia], ob = td.out[ib];
+  const double xa = x[na], ya = y[na], za = z[na];
+  const double xb = x[nb], yb = y[nb], zb = z[nb];
+  unsigned short cv4[4];
 #pragma unroll
-    for (int k = 0; k < 15; k++) rv[k] = m.ref[(size_t)k * E + e];
+  for (int r = 0; r < 4; r++) {
+    const int k = tid + r * NT;
+    cv4[r] = td.cent[cb + (k < nc ? k : nc - 1)];
   }
-  int meta = 0;
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0, gs = 1.0, dly = 0.0, A0 = 0.0;
-  int csl[3] = {-1, -1, -1};
-  if (elem_thread) {
-    const unsigned short* ls = td.lslot + (size_t)t * NS * T + i;
-    unsigned short cv[kMaxNen];  // all loads in flight before the first store
+  const int el = live ? e : 0;
+  double rv[15];
 #pragma unroll
-    for (int a = 0; a < kMaxNen; a++) cv[a] = a < NS ? ls[(size_t)a * T] : kTileIdle;
+  for (int k = 0; k < 15; k++) rv[k] = m.ref[(size_t)k * E + el];
+  // read by the element threads only, loaded by every thread (no branch)
+  int meta = m.meta[el];
+  const double f0 = m.fib[el], f1 = m.fib[(size_t)E + el], f2 = m.fib[(size_t)2 * E + el];
+  const double gs = m.gscale ? m.gscale[el] : 1.0;
+  const double dly = kp.has_delay ? m.delay[el] : 0.0;
+  const double A0 = m.dA0[el];
+  if (!live || !elem_thread) meta = 0;
+  if (ha) {
+    sx[la] = xa;
+    sy[la] = ya;
+    sz[la] = za;
+    cp[la + 1] = ca - cb;
+    tf[la] = 0;
+    sn[la] = na;
+    so[la] = oa;
+  }
+  if (hb) {
+    sx[lb] = xb;
+    sy[lb] = yb;
+    sz[lb] = zb;
+    cp[lb + 1] = cbn - cb;
+    tf[lb] = 0;
+    sn[lb] = nb;
+    so[lb] = ob;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    if (tid + r * NT < nc) ce[tid + r * NT] = cv4[r];
+  if (elem_thread) {
 #pragma unroll
     for (int a = 0; a < kMaxNen; a++)
       if (a < NS) sc[(size_t)a * T + i] = cv[a];
-    if (live) {
-      meta = m.meta[e];
-      f0 = m.fib[e];
-      f1 = m.fib[(size_t)E + e];
-      f2 = m.fib[(size_t)2 * E + e];
-      gs = m.gscale ? m.gscale[e] : 1.0;
-      dly = kp.has_delay ? m.delay[e] : 0.0;
-      A0 = m.dA0[e];
-      if ((meta & 3) == kCap)
-        for (int k = 0; k < 3; k++) csl[k] = m.corner[(size_t)(3 + k) * E + e];
-    }
   }
-  const int j0 = td.node_ptr[t], L = td.node_ptr[t + 1] - j0;
-  const int cb = td.cptr[j0], nc = td.cptr[j0 + L] - cb;
-  for (int l = tid; l < L; l += NT) {
+  for (int l = tid + 2 * NT; l < L; l += NT) {
     const int n = td.node[j0 + l];
-    const int c = td.cptr[j0 + l + 1];
     sx[l] = x[n];
     sy[l] = y[n];
     sz[l] = z[n];
-    cp[l + 1] = c - cb;
+    cp[l + 1] = td.cptr[j0 + l + 1] - cb;
     tf[l] = 0;
+    sn[l] = n;
+    so[l] = td.out[j0 + l];
   }
-  for (int k = tid; k < nc; k += NT) ce[k] = td.cent[cb + k];
+  for (int k = tid + 4 * NT; k < nc; k += NT) ce[k] = td.cent[cb + k];
+  int csl[3] = {-1, -1, -1};  // LV cap: the patch slots of the CST corners
+  if (elem_thread && live && (meta & 3) == kCap)
+    for (int k = 0; k < 3; k++) csl[k] = m.corner[(size_t)(3 + k) * E + e];
   if (tid == 0) cp[0] = 0;
   lds_barrier();
   const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
@@ -406,6 +447,7 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
     act.gam = eb[(size_t)kTbGam * T + i];
   }
   lds_barrier();  // every chain thread holds its inputs: the qp results take their place
+  double c33_keep = 0.0;  // stored after phase 4, off the path of the phases between
   if (chain) {
     RefGeom rg;
     rg.a1 = mk(rv[0], rv[1], rv[2]);
@@ -428,8 +470,7 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
     o[(size_t)2 * P * T] = (double)sq.S01;
     o[(size_t)3 * P * T] = sq.c33;
     qs[q * T + i] = sq.status | c33_range_bit(sq.c33, sizeof(R) == 4);
-    // handed-over elements get their C33 from the reference-order solve later
-    m.c33[(size_t)q * E + e] = sq.c33;
+    c33_keep = sq.c33;
   }
   lds_barrier();
   // ---- 3. resultants, hand-over, force coefficients ----
@@ -455,10 +496,12 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
     handed = kp.fallback && needs_exact_bits(st, finite);
     td.deferred[e] = handed ? 1 : 0;
     if (handed) {  // its nodes are marked in LDS; phase 4 writes the marks
-      for (int a = 0; a < nen; a++) {
-        const unsigned short code = sc[(size_t)a * T + i];
-        if (code != kTileIdle) tf[code] = 1;  // kTileIdle: a slot without force (LV cap)
-      }
+      unsigned short code[kMaxNen];
+#pragma unroll
+      for (int a = 0; a < kMaxNen; a++) code[a] = a < nen ? sc[(size_t)a * T + i] : kTileIdle;
+#pragma unroll
+      for (int a = 0; a < kMaxNen; a++)
+        if (code[a] != kTileIdle) tf[code[a]] = 1;  // kTileIdle: a slot without force (LV cap)
       // FP64: every qp goes to the reference-order solve (no pairs are listed: the solve
       // takes the element list).  FP32-mixed (the second tier re-evaluates the element in
       // FP64 first): the failing qps are listed, the others keep their staged stresses, from
@@ -502,9 +545,14 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       v2 = -2.0 * pref0 * hh * M2;
     }
   }
+  // FP64: straight into the hand-over list, no compaction pass; the list atomic is issued
+  // here and its offset used after phase 4
+  unsigned long long hbal = 0ull;
+  int hbase = 0;
   if (elem_thread) {
-    if (sizeof(R) == 8) {  // straight into the hand-over list: no compaction pass
-      push_hand_list(td, live && handed, e);
+    if (sizeof(R) == 8) {
+      hbal = __ballot(live && handed);
+      hbase = hand_list_reserve(td, hbal);
     } else {  // per-wave segments, which the second tier re-evaluates
       const int w = t * (T >> 6) + (tid >> 6);
       HandSink{td.wlist, td.wqlist, td.wcount, w}.push(live && handed, qfail, e, P);
@@ -552,15 +600,34 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
   for (int l = tid; l < L; l += NT) {
     const int c0 = cp[l], c1 = cp[l + 1];
     double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int c = c0; c < c1; c++) {
-      const unsigned v = ce[c];
-      const size_t k = (size_t)(v >> 8) * T + (v & 0xFFu);
-      ax += sf[k];
-      ay += sf[(size_t)NS * T + k];
-      az += sf[(size_t)2 * NS * T + k];
+    // contributions in chunks: the chunk's entries, then their slot forces, each read
+    // together; added in entry order
+    constexpr int kCh = 8;
+    for (int cc = c0; cc < c1; cc += kCh) {
+      unsigned v[kCh];
+#pragma unroll
+      for (int r = 0; r < kCh; r++) v[r] = cc + r < c1 ? ce[cc + r] : 0u;
+      double fx[kCh], fy[kCh], fz[kCh];
+#pragma unroll
+      for (int r = 0; r < kCh; r++) {
+        fx[r] = fy[r] = fz[r] = 0.0;
+        if (cc + r < c1) {
+          const size_t k = (size_t)(v[r] >> 8) * T + (v[r] & 0xFFu);
+          fx[r] = sf[k];
+          fy[r] = sf[(size_t)NS * T + k];
+          fz[r] = sf[(size_t)2 * NS * T + k];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kCh; r++)
+        if (cc + r < c1) {
+          ax += fx[r];
+          ay += fy[r];
+          az += fz[r];
+        }
     }
-    const int o = td.out[j0 + l];
-    const int n = td.node[j0 + l];
+    const int o = so[l];
+    const int n = sn[l];
     if (tf[l] && n < n_v) td.touched[n] = 1;
     if (o == -1) {
       double* fo = fint + 3 * (size_t)n;
@@ -576,6 +643,9 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       pr[2] = az;
     }
   }
+  if (sizeof(R) == 8 && elem_thread) hand_list_write(td, hbal, hbase, live && handed, e);
+  // handed-over elements get their C33 from the reference-order solve later
+  if (chain) m.c33[(size_t)q * E + e] = c33_keep;
 }
 
 template <class R>

@@ -185,15 +185,21 @@ struct HandSink {
 };
 
 // Appends e of every lane with `take` to the FP64 path's hand-over list (TileDev list_*):
-// one atomic per wave that lists anything.  Called by every active lane of the wave.
-__device__ inline void push_hand_list(const TileDev& td, bool take, int e) {
-  const unsigned long long bal = __ballot(take);
+// one atomic per wave that lists anything, in two calls by every active lane of the wave,
+// so that other work can run while the atomic is in flight: hand_list_reserve issues it
+// (bal = __ballot(take)) and returns the offset in the wave's first taking lane;
+// hand_list_write spreads the offset and writes the entries.
+__device__ inline int hand_list_reserve(const TileDev& td, unsigned long long bal) {
+  int base = 0;
+  if (bal != 0ull && (int)(threadIdx.x & 63u) == __ffsll((long long)bal) - 1)
+    base = atomicAdd(td.list_count, __popcll(bal));
+  return base;
+}
+__device__ inline void hand_list_write(const TileDev& td, unsigned long long bal, int base,
+                                       bool take, int e) {
   if (bal == 0ull) return;
   const int lane = (int)(threadIdx.x & 63u);
-  const int leader = __ffsll((long long)bal) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(td.list_count, __popcll(bal));
-  base = __shfl(base, leader);
+  base = __shfl(base, __ffsll((long long)bal) - 1);
   if (!take) return;
   const int at = base + __popcll(bal & ((1ull << lane) - 1ull));
   if (td.list_ce) td.list_ce[at] = make_int2(td.list_case, e);

@@ -155,11 +155,12 @@ BF_HD M33<R> abs_mul(const M33<R>& A, const M33<R>& B) {
 // closed forms evaluated on element-wise absolute values (Ha = |Fa_inv.Cov| |gm0.Cont|,
 // H2a = Ha Ha, |Ce_inv.Cont|, |Se|, |alpha|, |beta|), an upper bound because the staged
 // derivation only regroups terms.  Returns the bound on the resulting change of C33,
-// (|dC33| rel.err(CC) + err(S22) / |CC|).  Elements where it exceeds the fast paths'
-// budget are ill-conditioned: the reference's own C33 is noise-dominated there.
+// (|dC33| rel.err(CC) + err(S22) / |CC|), from inv_aCC = 1 / |CC|.  Elements where it
+// exceeds the fast paths' budget are ill-conditioned: the reference's own C33 is
+// noise-dominated there.
 template <class R>
 BF_HD R c33_update_noise(const M33<R>& Ha, const M33<R>& H2a, const M33<R>& Ch,
-                         const M33<R>& Se, R alpha, R beta, R S22, R CC) {
+                         const M33<R>& Se, R alpha, R beta, R S22, R inv_aCC) {
   const R eps = R(1.1102230246251565e-16);  // 2^-53, the reference's FP64
   const R hx = Ha.m[0][2], hy = Ha.m[1][2], hz = Ha.m[2][2];
   const R Px = H2a.m[0][2], Py = H2a.m[1][2], Pz = H2a.m[2][2];
@@ -189,8 +190,7 @@ BF_HD R c33_update_noise(const M33<R>& Ha, const M33<R>& H2a, const M33<R>& Ch,
     SHz[p] = fabs(Se.m[p][0]) * hx + fabs(Se.m[p][1]) * hy + fabs(Se.m[p][2]) * hz;
   const R S22abs = R(0.5) * (hx * SHz[0] + hy * SHz[1] + hz * SHz[2] + fabs(Se.m[0][2]) * Px +
                              fabs(Se.m[1][2]) * Py + fabs(Se.m[2][2]) * Pz);
-  const R aCC = fabs(CC);
-  return (fabs(S22) * (R(6569) * eps * CCabs) / aCC + R(89) * eps * S22abs) / aCC;
+  return (fabs(S22) * (R(6569) * eps * CCabs) * inv_aCC + R(89) * eps * S22abs) * inv_aCC;
 }
 
 // Gram matrix of three vectors g_i . g_j (ComputeMetricTensor, active_strain.c:769-779)
@@ -437,8 +437,12 @@ BF_HD StagedQp<R> staged_qp(const V3<double>& a1c, const V3<double>& a2c,
     const R c00 = gm.a00, c01 = gm.a01, c11 = gm.a11;
     R c33 = R(1);
     R ci00 = gmI.a00, ci01 = gmI.a01, ci11 = gmI.a11, ci22 = R(1);
+    // after the first update C.Cov is block-diagonal: its in-plane inverse no longer
+    // depends on C33
+    const R rD = R(1) / (c00 * c11 - c01 * c01);
     R S00 = 0, S11 = 0, S01 = 0;
     for (int it = 0; it < iters; it++) {
+      const bool last = it == iters - 1;
       // ---- Ce^-1 = T Ci T^T (ElemElasCGDefTens :1394-1417) ----
       M33<R> TC;  // T * Ci, Ci block-diagonal
 #pragma unroll
@@ -467,18 +471,24 @@ BF_HD StagedQp<R> staged_qp(const V3<double>& a1c, const V3<double>& a2c,
       Se.a01 = mu * (G.a01 - Ch.a01) + kj * Ch.a01;
       Se.a02 = mu * (G.a02 - Ch.a02) + kj * Ch.a02;
       Se.a12 = mu * (G.a12 - Ch.a12) + kj * Ch.a12;
-      // ---- S = 1/2 (H^T Se H + Se H2) (ElemTotStress :1509-1528) ----
-      const M33<R> SeH = mul(full(Se), H);
+      // ---- S = 1/2 (H^T Se H + Se H2) (ElemTotStress :1509-1528): S^22 in every
+      //      iteration, the in-plane entries in the last (the ones kept) ----
       const M33<R> SeF = full(Se);
-      // S^ij = 1/2 (sum_p H[p][i] SeH[p][j] + sum_p Se[p][i] H2[p][j])
-      S00 = R(0.5) * (H.m[0][0] * SeH.m[0][0] + H.m[1][0] * SeH.m[1][0] + H.m[2][0] * SeH.m[2][0] +
-                      SeF.m[0][0] * H2.m[0][0] + SeF.m[1][0] * H2.m[1][0] + SeF.m[2][0] * H2.m[2][0]);
-      S11 = R(0.5) * (H.m[0][1] * SeH.m[0][1] + H.m[1][1] * SeH.m[1][1] + H.m[2][1] * SeH.m[2][1] +
-                      SeF.m[0][1] * H2.m[0][1] + SeF.m[1][1] * H2.m[1][1] + SeF.m[2][1] * H2.m[2][1]);
-      S01 = R(0.5) * (H.m[0][0] * SeH.m[0][1] + H.m[1][0] * SeH.m[1][1] + H.m[2][0] * SeH.m[2][1] +
-                      SeF.m[0][0] * H2.m[0][1] + SeF.m[1][0] * H2.m[1][1] + SeF.m[2][0] * H2.m[2][1]);
-      const R S22 = R(0.5) * (H.m[0][2] * SeH.m[0][2] + H.m[1][2] * SeH.m[1][2] + H.m[2][2] * SeH.m[2][2] +
-                              SeF.m[0][2] * H2.m[0][2] + SeF.m[1][2] * H2.m[1][2] + SeF.m[2][2] * H2.m[2][2]);
+      // S^ij = 1/2 (sum_p H[p][i] SeH[p][j] + sum_p Se[p][i] H2[p][j]), SeH = Se H
+      auto seh = [&](int p, int j) {
+        return SeF.m[p][0] * H.m[0][j] + SeF.m[p][1] * H.m[1][j] + SeF.m[p][2] * H.m[2][j];
+      };
+      auto tot = [&](int i, int j) {
+        return R(0.5) * (H.m[0][i] * seh(0, j) + H.m[1][i] * seh(1, j) + H.m[2][i] * seh(2, j) +
+                         SeF.m[0][i] * H2.m[0][j] + SeF.m[1][i] * H2.m[1][j] +
+                         SeF.m[2][i] * H2.m[2][j]);
+      };
+      if (last) {
+        S00 = tot(0, 0);
+        S11 = tot(1, 1);
+        S01 = tot(0, 1);
+      }
+      const R S22 = tot(2, 2);
       // ---- CC^{2222} closed form (ElemElsTangMatTens + ElemTotTangMatTens) ----
       const R alpha = K * JJ - mu;
       const R beta = R(0.5) * K * Je * (R(2) * Je - R(1));
@@ -499,8 +509,9 @@ BF_HD StagedQp<R> staged_qp(const V3<double>& a1c, const V3<double>& a2c,
       const R CC = R(0.25) * (R(2) * (beta - alpha) * b * b + R(4) * beta * b * Pr -
                               R(4) * alpha * BhP * hr + R(2) * beta * Pr * Pr -
                               alpha * BPP * Ch.a22 - alpha * Pr * Pr);
+      const R rCC = R(1) / CC;
       {
-        const R nz = c33_update_noise(Ha, H2a, full(Ch), SeF, alpha, beta, S22, CC);
+        const R nz = c33_update_noise(Ha, H2a, full(Ch), SeF, alpha, beta, S22, fabs(rCC));
         if (!(nz <= R(noise_max))) out.status |= kStatIllCond;
         out.noise = fmaxf(out.noise, (float)nz);
       }
@@ -508,7 +519,7 @@ BF_HD StagedQp<R> staged_qp(const V3<double>& a1c, const V3<double>& a2c,
       const bool fail = !(fabs(CC) > R(1e-14));  // error ignored by the caller (:2086)
       if (fail) out.status |= kStatCC3333;
       if (!skip(fail, it)) {
-        R dC = -S22 / CC;
+        R dC = -S22 * rCC;
         const R cold = c33;
         R trial = cold + dC;
         // det(C.Cov) of the block-diagonal C: C00 (C11 C33) - C01 (C01 C33)
@@ -529,15 +540,12 @@ BF_HD StagedQp<R> staged_qp(const V3<double>& a1c, const V3<double>& a2c,
         }
         c33 = trial;
       }
-      // ---- C_inv.Cont = INV(C.Cov) (:2101), block-diagonal cofactor form ----
-      {
-        const R det = c00 * (c33 * c11) - c01 * (c33 * c01);
-        const R c = R(1) / det;
-        ci00 = c * (c33 * c11);
-        ci01 = -c * (c33 * c01);
-        ci11 = c * (c33 * c00);
-        ci22 = c * (c11 * c00 - c01 * c01);
-      }
+      // ---- C_inv.Cont = INV(C.Cov) (:2101), block-diagonal: the in-plane inverse of
+      //      the in-plane block, 1 / C33 ----
+      ci00 = c11 * rD;
+      ci01 = -c01 * rD;
+      ci11 = c00 * rD;
+      ci22 = R(1) / c33;
     }
     out.c33 = (double)c33;
     out.S00 = S00;
@@ -787,7 +795,7 @@ BF_HD StagedQp<float> staged_qp_mixed(const V3<double>& a1c, const V3<double>& a
                               R(4) * alpha * BhP * hr + R(2) * beta * Pr * Pr -
                               alpha * BPP * Ch.m[2][2] - alpha * Pr * Pr);
       {
-        const R nz = c33_update_noise(Ha, H2a, Ch, Se, alpha, beta, S22, CC);
+        const R nz = c33_update_noise(Ha, H2a, Ch, Se, alpha, beta, S22, R(1) / fabs(CC));
         if (!(nz <= R(kC33NoiseMax))) out.status |= kStatIllCond;
         out.noise = fmaxf(out.noise, (float)nz);
       }

@@ -335,16 +335,16 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
     cv4[r] = td.cent[cb + (k < nc ? k : nc - 1)];
   }
   const int el = live ? e : 0;
-  double rv[15];
-#pragma unroll
-  for (int k = 0; k < 15; k++) rv[k] = m.ref[(size_t)k * E + el];
-  // read by the element threads only, loaded by every thread (no branch)
+  // read in phase 1 by the element threads only, loaded by every thread (no branch)
   int meta = m.meta[el];
   const double f0 = m.fib[el], f1 = m.fib[(size_t)E + el], f2 = m.fib[(size_t)2 * E + el];
   const double gs = m.gscale ? m.gscale[el] : 1.0;
   const double dly = kp.has_delay ? m.delay[el] : 0.0;
   const double A0 = m.dA0[el];
   if (!live || !elem_thread) meta = 0;
+  double rv[15];  // read in phase 2
+#pragma unroll
+  for (int k = 0; k < 15; k++) rv[k] = m.ref[(size_t)k * E + el];
   if (ha) {
     sx[la] = xa;
     sy[la] = ya;

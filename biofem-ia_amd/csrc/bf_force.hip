@@ -526,23 +526,29 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
         atomicMin(&m.status[2], e);
         atomicAdd(&m.status[3], 1);
       }
-      // the kept rows: the qp results took the chain-input rows only
-      const V3<double> a1 = ld3(kTbA1), a2 = ld3(kTbA2);
-      A3 = ld3(kTbA3);
-      const double* gg = eb + (size_t)kTbG * T + i;
-      const double pref0 = eb[(size_t)kTbPref * T + i];
-      const double hh = 0.5 * kp.h0;
-      // LV cap: CST membrane only (:1807-1851)
-      const double M0 = kind == kCap ? 0.0 : r.M0, M1 = kind == kCap ? 0.0 : r.M1,
-                   M2 = kind == kCap ? 0.0 : r.M2;
-      const double b0 = gg[0] * M0 + gg[(size_t)T] * M1 + 2.0 * gg[(size_t)2 * T] * M2;
-      const double b1 = gg[(size_t)3 * T] * M0 + gg[(size_t)4 * T] * M1 +
-                        2.0 * gg[(size_t)5 * T] * M2;
-      U0 = scale(pref0, add(add(scale(r.N0, a1), scale(r.N2, a2)), scale(-hh * b0, A3)));
-      U1 = scale(pref0, add(add(scale(r.N2, a1), scale(r.N1, a2)), scale(-hh * b1, A3)));
-      v0 = -pref0 * hh * M0;
-      v1 = -pref0 * hh * M1;
-      v2 = -2.0 * pref0 * hh * M2;
+    }
+    // the force coefficients, formed by every lane of the wave alike (no divergent branch)
+    // and zero for a handed-over element, whose force comes later; the kept rows: the qp
+    // results took the chain-input rows only
+    const V3<double> a1 = ld3(kTbA1), a2 = ld3(kTbA2);
+    A3 = ld3(kTbA3);
+    const double* gg = eb + (size_t)kTbG * T + i;
+    const double pref0 = eb[(size_t)kTbPref * T + i];
+    const double hh = 0.5 * kp.h0;
+    // LV cap: CST membrane only (:1807-1851)
+    const double M0 = kind == kCap ? 0.0 : r.M0, M1 = kind == kCap ? 0.0 : r.M1,
+                 M2 = kind == kCap ? 0.0 : r.M2;
+    const double b0 = gg[0] * M0 + gg[(size_t)T] * M1 + 2.0 * gg[(size_t)2 * T] * M2;
+    const double b1 = gg[(size_t)3 * T] * M0 + gg[(size_t)4 * T] * M1 +
+                      2.0 * gg[(size_t)5 * T] * M2;
+    U0 = scale(pref0, add(add(scale(r.N0, a1), scale(r.N2, a2)), scale(-hh * b0, A3)));
+    U1 = scale(pref0, add(add(scale(r.N2, a1), scale(r.N1, a2)), scale(-hh * b1, A3)));
+    v0 = -pref0 * hh * M0;
+    v1 = -pref0 * hh * M1;
+    v2 = -2.0 * pref0 * hh * M2;
+    if (handed) {
+      U0 = U1 = A3 = mk(0.0, 0.0, 0.0);
+      v0 = v1 = v2 = 0.0;
     }
   }
   // FP64: straight into the hand-over list, no compaction pass; the list atomic is issued
@@ -566,14 +572,16 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       sf[(size_t)(NS + a) * T + i] = f.y;
       sf[(size_t)(2 * NS + a) * T + i] = f.z;
     };
-    if (adds && kind == kRegular) {  // the Loop tables as immediates
+    // a handed-over or empty lane takes its kind's branch with zero coefficients (its slot
+    // forces are signed zeros, which leave every node sum as it is)
+    if (kind == kRegular) {  // the Loop tables as immediates
 #pragma unroll
       for (int a = 0; a < 12; a++) {
         const double s = kLoopNab[0][a] * v0 + kLoopNab[1][a] * v1 + kLoopNab[2][a] * v2;
         put(a, add(add(scale(kLoopNa[0][a], U0), scale(kLoopNa[1][a], U1)), scale(s, A3)));
       }
       for (int a = 12; a < NS; a++) put(a, mk(0.0, 0.0, 0.0));
-    } else if (adds && kind == kIrregular) {
+    } else if (kind == kIrregular) {
       const double(*tb)[kMaxNen] = c_IN[val];
       for (int a = 0; a < NS; a++) {
         V3<double> f = mk(0.0, 0.0, 0.0);

@@ -475,6 +475,8 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
   lds_barrier();
   // ---- 3. resultants, hand-over, force coefficients ----
   bool handed = false;
+  int st_keep = 0;           // the element's status bits and finiteness, stored after
+  bool finite_keep = true;   // phase 4 with the hand-over mark
   unsigned qfail = 0u;  // qps of a handed-over element that the reference order solves
   V3<double> U0 = mk(0.0, 0.0, 0.0), U1 = U0, A3 = U0;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
@@ -494,14 +496,15 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
     const bool finite = isfinite(r.N0) && isfinite(r.N1) && isfinite(r.N2) &&
                         isfinite(r.M0) && isfinite(r.M1) && isfinite(r.M2);
     handed = kp.fallback && needs_exact_bits(st, finite);
-    td.deferred[e] = handed ? 1 : 0;
+    st_keep = st;
+    finite_keep = finite;
     if (handed) {  // its nodes are marked in LDS; phase 4 writes the marks
-      unsigned short code[kMaxNen];
+      unsigned short code[kMaxNen];  // read branch-free (rows clamped), then masked
 #pragma unroll
-      for (int a = 0; a < kMaxNen; a++) code[a] = a < nen ? sc[(size_t)a * T + i] : kTileIdle;
+      for (int a = 0; a < kMaxNen; a++) code[a] = sc[(size_t)(a < NS ? a : NS - 1) * T + i];
 #pragma unroll
       for (int a = 0; a < kMaxNen; a++)
-        if (code[a] != kTileIdle) tf[code[a]] = 1;  // kTileIdle: a slot without force (LV cap)
+        if (a < nen && code[a] != kTileIdle) tf[code[a]] = 1;  // kTileIdle: a slot without force (LV cap)
       // FP64: every qp goes to the reference-order solve (no pairs are listed: the solve
       // takes the element list).  FP32-mixed (the second tier re-evaluates the element in
       // FP64 first): the failing qps are listed, the others keep their staged stresses, from
@@ -518,13 +521,6 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
           d[2] = s2;
           m.qst[(size_t)k * E + e] = 0;
         }
-      }
-    } else {
-      if (!finite) atomicMin(&m.status[0], e);
-      m.flags[e] = (unsigned char)((st & (kStatCC3333 | kStatClamp)) | (finite ? 0 : kStatNonFinite));
-      if (st & kStatCC3333) {
-        atomicMin(&m.status[2], e);
-        atomicAdd(&m.status[3], 1);
       }
     }
     // the force coefficients, formed by every lane of the wave alike (no divergent branch)
@@ -652,6 +648,18 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
     }
   }
   if (sizeof(R) == 8 && elem_thread) hand_list_write(td, hbal, hbase, live && handed, e);
+  if (elem_thread && live) {
+    td.deferred[e] = handed ? 1 : 0;
+    if (!handed) {
+      if (!finite_keep) atomicMin(&m.status[0], e);
+      m.flags[e] = (unsigned char)((st_keep & (kStatCC3333 | kStatClamp)) |
+                                   (finite_keep ? 0 : kStatNonFinite));
+      if (st_keep & kStatCC3333) {
+        atomicMin(&m.status[2], e);
+        atomicAdd(&m.status[3], 1);
+      }
+    }
+  }
   // handed-over elements get their C33 from the reference-order solve later
   if (chain) m.c33[(size_t)q * E + e] = c33_keep;
 }

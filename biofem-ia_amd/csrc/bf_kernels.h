@@ -189,10 +189,17 @@ struct HandSink {
 // so that other work can run while the atomic is in flight: hand_list_reserve issues it
 // (bal = __ballot(take)) and returns the offset in the wave's first taking lane;
 // hand_list_write spreads the offset and writes the entries.
+// The counter's address goes through an empty asm statement so that the compiler sees a
+// per-lane address: its rewrite of a uniform-address atomic into a wave reduction reads the
+// returned value right away, which would wait for the atomic here.
 __device__ inline int hand_list_reserve(const TileDev& td, unsigned long long bal) {
   int base = 0;
-  if (bal != 0ull && (int)(threadIdx.x & 63u) == __ffsll((long long)bal) - 1)
-    base = atomicAdd(td.list_count, __popcll(bal));
+  const int lane = (int)(threadIdx.x & 63u);
+  if (bal != 0ull && lane == __ffsll((long long)bal) - 1) {
+    int zero = 0;
+    asm volatile("" : "+v"(zero));
+    base = atomicAdd(td.list_count + zero, __popcll(bal));
+  }
   return base;
 }
 __device__ inline void hand_list_write(const TileDev& td, unsigned long long bal, int base,

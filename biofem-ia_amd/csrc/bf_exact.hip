@@ -242,11 +242,49 @@ __device__ __forceinline__ double from_group_last(double v, int lead) {
   }
 }
 
+// LDS copy of the factor matrices Fi, G, Ch (27 doubles) of each split group of a one-wave
+// block (wide splits: at most 4 groups): a round's lane reads the factors of its block by
+// computed address instead of selecting them from registers with lane-varying indices.
+// Measured on the 256 x 192 membrane's 3 676 FP32-mixed pairs (K = 15): 136 us with register
+// selects, 117 us with this array; sized for 32 groups, although 4 are used, because the
+// 4-group size measured 131 us on the same device (the 184k-triangle ellipsoid's 11 pairs,
+// K = 60: 9 us faster with 4).  At K <= 8 the register selects are faster (200 against
+// 215 us for the FP64 list of the same membrane).
+constexpr int kCcLdsGroups = 32;
+template <int K>
+constexpr bool cc_factors_from_lds() { return K >= kWideSplitMin; }
+__device__ __forceinline__ double* cc_factor_lds() {
+  __shared__ double s_fac[kCcLdsGroups * 27];
+  return s_fac;
+}
+
 template <int K>
 struct CcSplit {
   int j;  // lane within the group
   __device__ double operator()(const exact::CcIn& c) const {
+    constexpr bool lds = cc_factors_from_lds<K>();
+    static_assert(!lds || 64 / K <= kCcLdsGroups, "one LDS slot per group");
     const int lead = (threadIdx.x & 63) - j;
+    const double* Fl = nullptr;
+    const double* Gl = nullptr;
+    const double* Cl = nullptr;
+    if constexpr (lds) {
+      double* fac = cc_factor_lds() + (size_t)((threadIdx.x & 63) / K) * 27;
+      if (j == 0) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int col = 0; col < 3; col++) {
+            fac[3 * r + col] = c.Fi.m[r][col];
+            fac[9 + 3 * r + col] = c.G.m[r][col];
+            fac[18 + 3 * r + col] = c.Ch.m[r][col];
+          }
+      }
+      __syncthreads();  // one wave per block: orders the group leaders' writes before the reads
+      Fl = fac;
+      Gl = fac + 9;
+      Cl = fac + 18;
+    }
     double CC = 0.0;  // the running sum, valid in the lane whose block comes next
 #pragma unroll 1
     for (int b0 = 0; b0 < 81; b0 += K) {
@@ -254,7 +292,22 @@ struct CcSplit {
       exact::CcBlock k;
       bool act = false;
       if (b < 81) {
-        k = exact::cc_block_at(c, b);
+        if constexpr (lds) {  // the factors of block (g, h, m, n), as exact::cc_block_at forms them
+          const int gg = b / 27, h = (b / 9) % 3, mm = (b / 3) % 3, n = b % 3;
+          k.Figh = Fl[3 * gg + h];
+          k.Gm2 = Gl[3 * mm + 2];
+          k.Finm = Fl[3 * n + mm];
+          k.Gg2 = Gl[3 * gg + 2];
+          k.Ggn = Gl[3 * gg + n];
+          k.Chhn = Cl[3 * h + n];
+#pragma unroll
+          for (int r = 0; r < 3; r++) {
+            k.Chr[r] = Cl[3 * h + r];
+            k.Chc[r] = Cl[3 * r + n];
+          }
+        } else {
+          k = exact::cc_block_at(c, b);
+        }
         act = exact::cc_block_active(c, k);
       }
       if (__ballot(act) == 0ull) continue;
@@ -271,7 +324,9 @@ struct CcSplit {
       }
       CC = from_group_last<K>(CC, lead);  // back to lane 0 of the group
     }
-    return __shfl(CC, lead);
+    const double out = __shfl(CC, lead);
+    if constexpr (lds) __syncthreads();  // the factors are read before the next call's writes
+    return out;
   }
   // The nine Ce_inv entries: in round r lane j forms entry r K + j (all lanes at once, each
   // entry one sum in the reference's order), then every lane of the group takes each entry

@@ -254,6 +254,36 @@ BF_HD MidGeom mid_geom(int nen, const double* na0, const double* na1, const doub
   return G;
 }
 
+// mid_geom for coordinates read from HBM: every slot's coordinates are loaded before the
+// first sum (slots past nen read slot nen - 1, so the loads are branch-free and overlap
+// instead of forming one round trip per slot), then the same sums run in the same slot
+// order as mid_geom.
+template <class CoordFn>
+BF_HD MidGeom mid_geom_loaded(int nen, const double* na0, const double* na1, const double* nb0,
+                              const double* nb1, const double* nb2, CoordFn xyz) {
+  V3<double> X[kMaxNen];
+#pragma unroll
+  for (int i = 0; i < kMaxNen; i++) X[i] = xyz(i < nen ? i : nen - 1);
+  MidGeom G;
+  G.Aaa = mk(0.0, 0.0, 0.0);
+  G.Abb = G.Aaa;
+  G.Aab = G.Aaa;
+  G.a1 = G.Aaa;
+  G.a2 = G.Aaa;
+#pragma unroll
+  for (int i = 0; i < kMaxNen; i++) {
+    if (i < nen) {
+      G.Aaa.x += nb0[i] * X[i].x; G.Aaa.y += nb0[i] * X[i].y; G.Aaa.z += nb0[i] * X[i].z;
+      G.Abb.x += nb1[i] * X[i].x; G.Abb.y += nb1[i] * X[i].y; G.Abb.z += nb1[i] * X[i].z;
+      G.Aab.x += nb2[i] * X[i].x; G.Aab.y += nb2[i] * X[i].y; G.Aab.z += nb2[i] * X[i].z;
+      G.a1.x += na0[i] * X[i].x; G.a1.y += na0[i] * X[i].y; G.a1.z += na0[i] * X[i].z;
+      G.a2.x += na1[i] * X[i].x; G.a2.y += na1[i] * X[i].y; G.a2.z += na1[i] * X[i].z;
+    }
+  }
+  geom_tail(G);
+  return G;
+}
+
 // Loop box-spline derivatives at the element centre of a regular patch, the verbatim
 // 4-decimal values of src/active_strain.c:69-88 (Na_center, Nab_center).
 constexpr double kLoopNa[2][12] = {

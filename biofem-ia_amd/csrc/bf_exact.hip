@@ -24,11 +24,11 @@ __global__ void __launch_bounds__(256) bf_geom0_kernel(DevMesh m, int E,
   const int kind = meta & 3, val = (meta >> 8) & 255, nen = (meta >> 16) & 255;
   MidGeom G;
   auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
-  if (kind == kRegular) {
-    G = mid_geom(12, c_Na[0], c_Na[1], c_Nab[0], c_Nab[1], c_Nab[2], xyz);
+  if (kind == kRegular) {  // the same sums (c_Na / c_Nab hold the same values)
+    G = mid_geom_regular(xyz);
   } else if (kind == kIrregular) {
     const double(*t)[kMaxNen] = c_IN[val];
-    G = mid_geom(nen, t[0], t[1], t[2], t[3], t[4], xyz);
+    G = mid_geom_loaded(nen, t[0], t[1], t[2], t[3], t[4], xyz);
   } else {
     G = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),
                  ld3(x, y, z, m.corner[(size_t)2 * E + e]));
@@ -80,13 +80,16 @@ __device__ __forceinline__ ExactRows exact_inputs(const DevMesh& m, const KParam
   const int val = (meta >> 8) & 255;
   r.nen = (meta >> 16) & 255;
   auto xyz = [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); };
+  // full (the solve kernels, where registers are scarce): the slot loop; otherwise every
+  // slot's coordinates are loaded first (the same sums: c_Na / c_Nab hold kLoopNa / kLoopNab)
   if (r.kind == kRegular) {
     r.na0 = c_Na[0]; r.na1 = c_Na[1]; r.nb0 = c_Nab[0]; r.nb1 = c_Nab[1]; r.nb2 = c_Nab[2];
-    in.cur = mid_geom(12, r.na0, r.na1, r.nb0, r.nb1, r.nb2, xyz);
+    in.cur = full ? mid_geom(12, r.na0, r.na1, r.nb0, r.nb1, r.nb2, xyz) : mid_geom_regular(xyz);
   } else if (r.kind == kIrregular) {
     const double(*t)[kMaxNen] = c_IN[val];
     r.na0 = t[0]; r.na1 = t[1]; r.nb0 = t[2]; r.nb1 = t[3]; r.nb2 = t[4];
-    in.cur = mid_geom(r.nen, r.na0, r.na1, r.nb0, r.nb1, r.nb2, xyz);
+    in.cur = full ? mid_geom(r.nen, r.na0, r.na1, r.nb0, r.nb1, r.nb2, xyz)
+                  : mid_geom_loaded(r.nen, r.na0, r.na1, r.nb0, r.nb1, r.nb2, xyz);
   } else {
     r.na0 = r.na1 = r.nb0 = r.nb1 = r.nb2 = nullptr;
     in.cur = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),
@@ -450,14 +453,20 @@ __device__ __forceinline__ void exact_finish(const DevMesh& m, const KParams& kp
   const int E = kp.E;
   int status = 0;
   double Sv[kMaxQp][3];
+  int st[kMaxQp];
+#pragma unroll
+  for (int j = 0; j < kMaxQp; j++) {  // branch-free: qps past n_qp read the last one
+    const int jj = j < kp.n_qp ? j : kp.n_qp - 1;
+    const double* o = m.qsv + 3 * ((size_t)jj * E + e);
+    Sv[j][0] = o[0];
+    Sv[j][1] = o[1];
+    Sv[j][2] = o[2];
+    st[j] = m.qst[(size_t)jj * E + e];
+  }
 #pragma unroll
   for (int j = 0; j < kMaxQp; j++) {
     if (j < kp.n_qp) {
-      const double* o = m.qsv + 3 * ((size_t)j * E + e);
-      Sv[j][0] = o[0];
-      Sv[j][1] = o[1];
-      Sv[j][2] = o[2];
-      status |= m.qst[(size_t)j * E + e];
+      status |= st[j];
     } else {
       Sv[j][0] = Sv[j][1] = Sv[j][2] = 0.0;
     }

@@ -104,8 +104,8 @@ __device__ __forceinline__ int staged_element(const DevMesh& m, const KParams& k
   } else if (kind == kIrregular) {
     const double(*t)[kMaxNen] = c_IN[val];
     na0 = t[0]; na1 = t[1]; nb0 = t[2]; nb1 = t[3]; nb2 = t[4];
-    G = mid_geom(nen, na0, na1, nb0, nb1, nb2,
-                 [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); });
+    G = mid_geom_loaded(nen, na0, na1, nb0, nb1, nb2,
+                        [&](int i) { return ld3(x, y, z, m.patch[(size_t)i * E + e]); });
   } else {
     na0 = na1 = nb0 = nb1 = nb2 = nullptr;
     G = cap_geom(ld3(x, y, z, m.corner[e]), ld3(x, y, z, m.corner[(size_t)E + e]),

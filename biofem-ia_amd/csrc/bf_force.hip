@@ -738,63 +738,71 @@ __global__ void __launch_bounds__(kStagedBlock) bf_tile_tier2_kernel(DevMesh m, 
 // Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
 // m.fallback and stores the total in m.status[4], and its pair segments into m.qlist, qp by
 // qp in qp_of_rank order, with the total in m.status[8] and the mid-surface pairs (the last
-// class) in m.status[9].  One block: a chunk of waves per thread, one block-wide scan of
-// the chunk sums of every class at once (wave-level shuffle scans, then the wave totals:
-// one barrier), then each thread copies its chunks.  The counts were just written by waves
-// on every XCD, so each load is a round trip to memory: a thread reads its packed count
-// words kCompactRegs at a time with the loads in flight together.
-constexpr int kCompactBlock = 1024;
+// class) in m.status[9].  One thread per wave, blocks of kCompactBlock waves: every block
+// reads all the packed count words (loads in flight together) and forms the class totals
+// and the sums of the waves before its own, then a block-wide scan gives each wave its
+// offsets, and each thread copies its wave's segments with the loads of every class in
+// flight together.  The segments were just written by waves on every XCD, so each dependent
+// load is a round trip to memory; the copy takes one, whatever the number of segments.
+constexpr int kCompactBlock = 256;
 constexpr int kCompactWaves = kCompactBlock / 64;
-constexpr int kCompactRegs = 8;
 constexpr int kCompactClasses = 1 + kMaxQp;  // elements, then pairs by qp rank
+constexpr int kCompactChunk = 8;             // entries per class loaded together
 
 // Count of class c (0: elements, 1 + r: pairs of qp rank r) in a packed word.
 __device__ __forceinline__ int class_count(unsigned long long w, int c, int P) {
   return c == 0 ? count_elems(w) : count_pairs(w, qp_of_rank(c - 1, P));
 }
 
-// Block-wide exclusive offsets of this thread's sums of classes 0 .. nc - 1 and the block's
-// totals; s_wave: [kCompactWaves][kCompactClasses].
-__device__ __forceinline__ void compact_offsets(int* s_wave, int nc,
-                                                const int (&sum)[kCompactClasses],
-                                                int (&pos)[kCompactClasses],
-                                                int (&total)[kCompactClasses]) {
+// Block-wide sums (red) and exclusive scan (scan) of v[c], c < kCompactClasses;
+// s_wave: [2][kCompactWaves][kCompactClasses].
+__device__ __forceinline__ void compact_block_sums(int* s_wave, const int (&v)[kCompactClasses],
+                                                   const int (&u)[kCompactClasses],
+                                                   int (&scan)[kCompactClasses],
+                                                   int (&red)[kCompactClasses]) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  int inc[kCompactClasses];
+  int inc[kCompactClasses], tot[kCompactClasses];
 #pragma unroll
-  for (int c = 0; c < kCompactClasses; c++) inc[c] = sum[c];
+  for (int c = 0; c < kCompactClasses; c++) {
+    inc[c] = v[c];
+    tot[c] = u[c];
+  }
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
 #pragma unroll
     for (int c = 0; c < kCompactClasses; c++) {
-      if (c >= nc) break;
-      const int v = __shfl_up(inc[c], d);
-      if (lane >= d) inc[c] += v;
+      const int x = __shfl_up(inc[c], d);
+      if (lane >= d) inc[c] += x;
+      tot[c] += __shfl_xor(tot[c], d);
     }
   }
   if (lane == 63) {
 #pragma unroll
-    for (int c = 0; c < kCompactClasses; c++) s_wave[wv * kCompactClasses + c] = inc[c];
+    for (int c = 0; c < kCompactClasses; c++) {
+      s_wave[wv * kCompactClasses + c] = inc[c];
+      s_wave[(kCompactWaves + wv) * kCompactClasses + c] = tot[c];
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < kCompactClasses; c++) {
     int before = 0, all = 0;
+#pragma unroll
     for (int w = 0; w < kCompactWaves; w++) {
-      const int v = c < nc ? s_wave[w * kCompactClasses + c] : 0;
-      before += w < wv ? v : 0;
-      all += v;
+      before += w < wv ? s_wave[w * kCompactClasses + c] : 0;
+      all += s_wave[(kCompactWaves + w) * kCompactClasses + c];
     }
-    pos[c] = before + inc[c] - sum[c];
-    total[c] = all;
+    scan[c] = before + inc[c] - v[c];
+    red[c] = all;
   }
 }
 
 // The element segments (class 0, 64 entries per wave, elist) and the pair segments of the P
 // qps in qp_of_rank order (classes 1 .. P, plist, pair_segment; none for P = 0) of waves
-// [0, n_waves) with packed counts cnt: each class concatenated in wave order, the pair classes one after
-// another.  emit_elem(at, w, v), emit_pair(at, w, v); totals to set_totals(elements, pairs,
-// mid-surface pairs) by one thread.
+// [0, n_waves) with packed counts cnt: each class concatenated in wave order, the pair
+// classes one after another.  emit_elem(at, w, v), emit_pair(at, w, v); totals to
+// set_totals(elements, pairs, mid-surface pairs) by one thread.  Launched with
+// ceil(n_waves / kCompactBlock) blocks (at least one).
 template <class EmitE, class EmitP, class Totals>
 __device__ __forceinline__ void compact_all(int* s_wave, const unsigned long long* cnt,
                                             const int* elist, const int* plist, int n_waves,
@@ -802,22 +810,38 @@ __device__ __forceinline__ void compact_all(int* s_wave, const unsigned long lon
                                             Totals&& set_totals) {
   // P = 0: elements only
   const int t = threadIdx.x;
-  const int chunk = (n_waves + kCompactBlock - 1) / kCompactBlock;
-  const int w0 = min(n_waves, t * chunk), w1 = min(n_waves, w0 + chunk);
+  const int w0 = blockIdx.x * kCompactBlock, w = w0 + t;
   const int nc = 1 + P;
-  int sum[kCompactClasses] = {};
-  for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
-    unsigned long long wd[kCompactRegs];
+  const unsigned long long mine = w < n_waves ? cnt[w] : 0ull;
+  // class sums of every wave (all) and of the waves before this block (before)
+  int before[kCompactClasses] = {}, all[kCompactClasses] = {};
+  constexpr int kR = 16;  // count words in flight per thread
+  for (int g0 = t; g0 < n_waves; g0 += kR * kCompactBlock) {
+    unsigned long long wd[kR];
 #pragma unroll
-    for (int r = 0; r < kCompactRegs; r++) wd[r] = (g0 + r < w1) ? cnt[g0 + r] : 0ull;
+    for (int r = 0; r < kR; r++) {
+      const int g = g0 + r * kCompactBlock;
+      wd[r] = cnt[g < n_waves ? g : n_waves - 1];
+      if (g >= n_waves) wd[r] = 0ull;
+    }
 #pragma unroll
-    for (int r = 0; r < kCompactRegs; r++)
+    for (int r = 0; r < kR; r++) {
+      const bool pre = g0 + r * kCompactBlock < w0;
 #pragma unroll
-      for (int c = 0; c < kCompactClasses; c++)
-        if (c < nc) sum[c] += class_count(wd[r], c, P);
+      for (int c = 0; c < kCompactClasses; c++) {
+        const int k = c < nc ? class_count(wd[r], c, P) : 0;
+        all[c] += k;
+        before[c] += pre ? k : 0;
+      }
+    }
   }
-  int pos[kCompactClasses], total[kCompactClasses];
-  compact_offsets(s_wave, nc, sum, pos, total);
+  int own[kCompactClasses];
+#pragma unroll
+  for (int c = 0; c < kCompactClasses; c++) own[c] = c < nc ? class_count(mine, c, P) : 0;
+  int scan[kCompactClasses], sb[kCompactClasses], total[kCompactClasses];
+  compact_block_sums(s_wave, own, all, scan, total);
+  __syncthreads();
+  compact_block_sums(s_wave, before, before, sb, sb);  // sb: the block's sum of `before`
   int base[kCompactClasses];  // start of each class in its output list
   base[0] = 0;
   int acc = 0;
@@ -826,39 +850,45 @@ __device__ __forceinline__ void compact_all(int* s_wave, const unsigned long lon
     base[c] = acc;
     acc += c < nc ? total[c] : 0;
   }
-  int any = 0;
+  if (mine != 0ull) {  // most waves hold no hand-over
+    int v[kCompactClasses][kCompactChunk];
 #pragma unroll
-  for (int c = 0; c < kCompactClasses; c++) any |= sum[c];
-  if (any) {  // most chunks hold no hand-over
-    for (int g0 = w0; g0 < w1; g0 += kCompactRegs) {
-      unsigned long long wd[kCompactRegs];
+    for (int c = 0; c < kCompactClasses; c++) {
+      const int n = own[c];
+      const int* src = c == 0 ? elist + (size_t)w * 64
+                              : plist + pair_segment(w, c < nc ? qp_of_rank(c - 1, P) : 0);
 #pragma unroll
-      for (int r = 0; r < kCompactRegs; r++) wd[r] = (g0 + r < w1) ? cnt[g0 + r] : 0ull;
+      for (int k = 0; k < kCompactChunk; k++) v[c][k] = src[k < n ? k : (n > 0 ? n - 1 : 0)];
+    }
 #pragma unroll
-      for (int r = 0; r < kCompactRegs; r++) {
-        if (wd[r] == 0ull) continue;
-        const int w = g0 + r;
-        const int ne = count_elems(wd[r]);
-        for (int k = 0; k < ne; k++) emit_elem(pos[0] + k, w, elist[(size_t)w * 64 + k]);
-        pos[0] += ne;
-        for (int c = 1; c < nc; c++) {
-          const int q = qp_of_rank(c - 1, P);
-          const int n = count_pairs(wd[r], q);
-          for (int k = 0; k < n; k++)
-            emit_pair(base[c] + pos[c] + k, w, plist[pair_segment(w, q) + k]);
-          pos[c] += n;
+    for (int c = 0; c < kCompactClasses; c++) {
+      const int n = own[c];
+      const int at = base[c] + sb[c] + scan[c];
+#pragma unroll
+      for (int k = 0; k < kCompactChunk; k++) {
+        if (k < n) {
+          if (c == 0) emit_elem(at + k, w, v[c][k]);
+          else emit_pair(at + k, w, v[c][k]);
+        }
+      }
+      if (n > kCompactChunk) {  // rare: a long segment
+        const int* src = c == 0 ? elist + (size_t)w * 64
+                                : plist + pair_segment(w, qp_of_rank(c - 1, P));
+        for (int k = kCompactChunk; k < n; k++) {
+          if (c == 0) emit_elem(at + k, w, src[k]);
+          else emit_pair(at + k, w, src[k]);
         }
       }
     }
   }
-  if (t == kCompactBlock - 1) set_totals(total[0], acc, has_mid_qp(P) ? total[P] : 0);
+  if (blockIdx.x == 0 && t == 0) set_totals(total[0], acc, has_mid_qp(P) ? total[P] : 0);
 }
 
 __global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(
     const unsigned long long* __restrict__ cnt, const int* __restrict__ wlist,
     const int* __restrict__ wqlist, int n_waves, int P, int* __restrict__ out,
     int* __restrict__ qout, int* __restrict__ status) {
-  __shared__ int s_wave[kCompactWaves * kCompactClasses];
+  __shared__ int s_wave[2 * kCompactWaves * kCompactClasses];
   compact_all(s_wave, cnt, wlist, wqlist, n_waves, P,
               [&](int at, int, int v) { out[at] = v; }, [&](int at, int, int v) { qout[at] = v; },
               [&](int ne, int np, int nm) {
@@ -964,7 +994,7 @@ __global__ void __launch_bounds__(kStagedBlock) bf_batch_tier2_kernel(BatchDev b
 // and (pairs: the FP32-mixed path) their pair segments into (case, qp_pair) entries by qp
 // rank, ptotal[0] = count, ptotal[1] = mid-surface pairs at the end.
 __global__ void __launch_bounds__(kCompactBlock) bf_batch_compact_kernel(BatchDev b, int pairs) {
-  __shared__ int s_wave[kCompactWaves * kCompactClasses];
+  __shared__ int s_wave[2 * kCompactWaves * kCompactClasses];
   compact_all(s_wave, b.tcount, b.tlist, b.tqlist, b.n_etiles, pairs ? b.P : 0,
               [&](int at, int w, int v) { b.hlist[at] = make_int2(b.etiles[w].c, v); },
               [&](int at, int w, int v) { b.plist[at] = make_int2(b.etiles[w].c, v); },
@@ -1742,7 +1772,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
                          dm, kp, td, n_waves, x, y, z);
     const bool pairs = precision == BF_PREC_MIXED;
     if (pairs)  // the FP64 tile kernel listed its elements itself
-      hipLaunchKernelGGL(bf_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s,
+      hipLaunchKernelGGL(bf_compact_kernel, dim3(std::max(1, (n_waves + kCompactBlock - 1) / kCompactBlock)), dim3(kCompactBlock), 0, s,
                          (const unsigned long long*)td.wcount, (const int*)td.wlist,
                          (const int*)td.wqlist, n_waves, kp.n_qp, dm.fallback, dm.qlist,
                          dm.status);
@@ -2405,7 +2435,9 @@ int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
     }
     const bool pairs = precision == BF_PREC_MIXED;
     if (pairs)  // the FP64 tile kernel listed its elements itself
-      hipLaunchKernelGGL(bf_batch_compact_kernel, dim3(1), dim3(kCompactBlock), 0, s, d, 1);
+      hipLaunchKernelGGL(bf_batch_compact_kernel,
+                         dim3(std::max(1, (b->n_etiles + kCompactBlock - 1) / kCompactBlock)),
+                         dim3(kCompactBlock), 0, s, d, 1);
     BF_HIP(launch_batch_exact_list(d, b->n_cu, pairs, s));
   }
   BF_HIP(launch_batch_tile_fixup(d, step, s));

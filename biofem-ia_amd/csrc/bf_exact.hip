@@ -881,7 +881,11 @@ hipError_t launch_batch_gather_step(const BatchDev& b, hipStream_t s) {
 // element-force buffer (written by the reference-order finish or the FP64 second tier), and
 // its touched mark is cleared for the next evaluation.  Ghost rows are zeroed when not
 // accumulating.  Returns whether f holds the force (always for real nodes when need_f).
-constexpr int kNodeChunk = 8;  // loads in flight per node and chunk
+// Loads in flight per node and chunk: partials kNodeChunk; entries of a marked node
+// EntChunk (the single-mesh kernels use 24, the entries of a node interior to a regular
+// patch mesh, so one chunk; the batch kernel, with many more waves in flight, 8).
+constexpr int kNodeChunk = 8;
+template <int EntChunk>
 __device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __restrict__ ptr,
                                                 const unsigned* __restrict__ ent,
                                                 const double* __restrict__ Fe, int E, int n_v,
@@ -936,16 +940,16 @@ __device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __
     // entries in chunks: the chunk's entries, then their deferred flags, then the rows of
     // the deferred ones are each loaded together; the rows are added in entry order
     const int j1 = ptr[n + 1];
-    for (int j0 = ptr[n]; j0 < j1; j0 += kNodeChunk) {
-      unsigned v[kNodeChunk];
+    for (int j0 = ptr[n]; j0 < j1; j0 += EntChunk) {
+      unsigned v[EntChunk];
 #pragma unroll
-      for (int k = 0; k < kNodeChunk; k++) v[k] = j0 + k < j1 ? ent[j0 + k] : 0u;
-      bool d[kNodeChunk];
+      for (int k = 0; k < EntChunk; k++) v[k] = j0 + k < j1 ? ent[j0 + k] : 0u;
+      bool d[EntChunk];
 #pragma unroll
-      for (int k = 0; k < kNodeChunk; k++) d[k] = j0 + k < j1 && td.deferred[v[k] >> 5] != 0;
-      double fx[kNodeChunk], fy[kNodeChunk], fz[kNodeChunk];
+      for (int k = 0; k < EntChunk; k++) d[k] = j0 + k < j1 && td.deferred[v[k] >> 5] != 0;
+      double fx[EntChunk], fy[EntChunk], fz[EntChunk];
 #pragma unroll
-      for (int k = 0; k < kNodeChunk; k++) {
+      for (int k = 0; k < EntChunk; k++) {
         fx[k] = fy[k] = fz[k] = 0.0;
         if (d[k]) {
           const double* fe = Fe + 3 * ((size_t)(v[k] & 31u) * E + (v[k] >> 5));
@@ -955,7 +959,7 @@ __device__ __forceinline__ bool tile_node_final(const TileDev& td, const int* __
         }
       }
 #pragma unroll
-      for (int k = 0; k < kNodeChunk; k++)
+      for (int k = 0; k < EntChunk; k++)
         if (d[k]) {
           f[0] += fx[k];
           f[1] += fy[k];
@@ -998,7 +1002,7 @@ __global__ void __launch_bounds__(256) bf_tile_fixup_kernel(TileDev td, const in
   const int n = logical_block() * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   double f[3];
-  tile_node_final(td, ptr, ent, Fe, E, n_v, accumulate, fint, n, false, f);
+  tile_node_final<24>(td, ptr, ent, Fe, E, n_v, accumulate, fint, n, false, f);
 }
 
 __global__ void __launch_bounds__(256) bf_tile_fixup_step_kernel(
@@ -1009,7 +1013,7 @@ __global__ void __launch_bounds__(256) bf_tile_fixup_step_kernel(
   const int n = logical_block() * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   double f[3];
-  if (tile_node_final(td, ptr, ent, Fe, E, n_v, 0, fint, n, true, f))
+  if (tile_node_final<24>(td, ptr, ent, Fe, E, n_v, 0, fint, n, true, f))
     step_update(n, f, x, y, z, v, winv, fixed, dt, damp);
 }
 
@@ -1021,7 +1025,7 @@ __global__ void __launch_bounds__(256) bf_batch_tile_fixup_kernel(BatchDev b, in
   const int n = tl.b + i;
   double f[3];
   const bool adv = step && cd.v;
-  if (tile_node_final(cd.td, cd.ptr, cd.ent, cd.m.Fe, cd.kp.E, cd.n_v, 0, cd.fint, n, adv, f) &&
+  if (tile_node_final<8>(cd.td, cd.ptr, cd.ent, cd.m.Fe, cd.kp.E, cd.n_v, 0, cd.fint, n, adv, f) &&
       adv)
     step_update(n, f, cd.x, cd.y, cd.z, cd.v, cd.winv, cd.fixed, cd.dt, cd.damp);
 }

@@ -755,6 +755,9 @@ BF_HD StagedQp<float> staged_qp_mixed(const V3<double>& a1c, const V3<double>& a
     // ---- C33 iteration ----
     const double c00 = gm.a00, c01 = gm.a01, c11 = gm.a11;
     double c33 = 1.0;
+    // after the first update C.Cov is block-diagonal: its in-plane inverse does not depend
+    // on C33
+    const double rD = 1.0 / (c00 * c11 - c01 * c01);
     // E = C^-1 - G for the first iteration (C^-1 = gm^-1 in-plane block, 1 at [2][2])
     M33<R> E;
     {
@@ -768,6 +771,7 @@ BF_HD StagedQp<float> staged_qp_mixed(const V3<double>& a1c, const V3<double>& a
     }
     R S00 = 0, S11 = 0, S01 = 0;
     for (int it = 0; it < iters; it++) {
+      const bool last = it == iters - 1;
       // dC = T E T^T + Q  (Ce^-1 - G)
       const M33<R> TE = mul(T, E);
       M33<R> dC;
@@ -795,15 +799,22 @@ BF_HD StagedQp<float> staged_qp_mixed(const V3<double>& a1c, const V3<double>& a
           Ch.m[i][j] = G.m[i][j] + dC.m[i][j];
           Se.m[i][j] = -mu * dC.m[i][j] + K * JJ * Ch.m[i][j];
         }
-      const M33<R> SeH = mul(Se, H);
-      S00 = R(0.5) * (H.m[0][0] * SeH.m[0][0] + H.m[1][0] * SeH.m[1][0] + H.m[2][0] * SeH.m[2][0] +
-                      Se.m[0][0] * H2.m[0][0] + Se.m[1][0] * H2.m[1][0] + Se.m[2][0] * H2.m[2][0]);
-      S11 = R(0.5) * (H.m[0][1] * SeH.m[0][1] + H.m[1][1] * SeH.m[1][1] + H.m[2][1] * SeH.m[2][1] +
-                      Se.m[0][1] * H2.m[0][1] + Se.m[1][1] * H2.m[1][1] + Se.m[2][1] * H2.m[2][1]);
-      S01 = R(0.5) * (H.m[0][0] * SeH.m[0][1] + H.m[1][0] * SeH.m[1][1] + H.m[2][0] * SeH.m[2][1] +
-                      Se.m[0][0] * H2.m[0][1] + Se.m[1][0] * H2.m[1][1] + Se.m[2][0] * H2.m[2][1]);
-      const R S22 = R(0.5) * (H.m[0][2] * SeH.m[0][2] + H.m[1][2] * SeH.m[1][2] + H.m[2][2] * SeH.m[2][2] +
-                              Se.m[0][2] * H2.m[0][2] + Se.m[1][2] * H2.m[1][2] + Se.m[2][2] * H2.m[2][2]);
+      // S^ij = 1/2 (sum_p H[p][i] SeH[p][j] + sum_p Se[p][i] H2[p][j]), SeH = Se H: S^22 in
+      // every iteration, the in-plane entries in the last (the ones kept)
+      auto seh = [&](int p, int j) {
+        return Se.m[p][0] * H.m[0][j] + Se.m[p][1] * H.m[1][j] + Se.m[p][2] * H.m[2][j];
+      };
+      auto tot = [&](int i, int j) {
+        return R(0.5) * (H.m[0][i] * seh(0, j) + H.m[1][i] * seh(1, j) + H.m[2][i] * seh(2, j) +
+                         Se.m[0][i] * H2.m[0][j] + Se.m[1][i] * H2.m[1][j] +
+                         Se.m[2][i] * H2.m[2][j]);
+      };
+      if (last) {
+        S00 = tot(0, 0);
+        S11 = tot(1, 1);
+        S01 = tot(0, 1);
+      }
+      const R S22 = tot(2, 2);
       // CC^{2222}
       const R alpha = K * JJ - mu;
       const R beta = R(0.5) * K * Je * (R(2) * Je - R(1));
@@ -824,16 +835,17 @@ BF_HD StagedQp<float> staged_qp_mixed(const V3<double>& a1c, const V3<double>& a
       const R CC = R(0.25) * (R(2) * (beta - alpha) * b * b + R(4) * beta * b * Pr -
                               R(4) * alpha * BhP * hr + R(2) * beta * Pr * Pr -
                               alpha * BPP * Ch.m[2][2] - alpha * Pr * Pr);
+      const R rCC = R(1) / CC;
       {
-        const R nz = c33_update_noise(Ha, H2a, Ch, Se, alpha, beta, S22, R(1) / fabs(CC));
+        const R nz = c33_update_noise(Ha, H2a, Ch, Se, alpha, beta, S22, fabs(rCC));
         if (!(nz <= R(kC33NoiseMax))) out.status |= kStatIllCond;
         out.noise = fmaxf(out.noise, (float)nz);
       }
-      // ModElemC33 in FP64
+      // ModElemC33 in FP64 (the FP32 quotient's rounding is far below that of S^22 itself)
       const bool fail = !(fabs(CC) > R(1e-14));
       if (fail) out.status |= kStatCC3333;
       if (!skip(fail, it)) {
-        double dCd = -(double)S22 / (double)CC;
+        double dCd = -(double)(S22 * rCC);
         const double cold = c33;
         double trial = cold + dCd;
         if (c00 * (c11 * trial) - c01 * (c01 * trial) <= 0.0) {
@@ -853,20 +865,21 @@ BF_HD StagedQp<float> staged_qp_mixed(const V3<double>& a1c, const V3<double>& a
         }
         c33 = trial;
       }
-      // E = C^-1 - G = -C^-1 (C - g0) G with the block-diagonal C (FP64 inverse)
-      {
-        const double det = c00 * (c33 * c11) - c01 * (c33 * c01);
-        const double c = 1.0 / det;
-        const double Ci[3][3] = {{c * (c33 * c11), -c * (c33 * c01), 0.0},
-                                 {-c * (c33 * c01), c * (c33 * c00), 0.0},
-                                 {0.0, 0.0, c * (c11 * c00 - c01 * c01)}};
+      // E = C^-1 - G = -C^-1 (C - g0) G with the block-diagonal C (FP64 inverse), for the
+      // next iteration
+      if (!last) {
+        const double ci22 = 1.0 / c33;
+        const double Ci[2][2] = {{c11 * rD, -c01 * rD}, {-c01 * rD, c00 * rD}};
         const double Dm[3][3] = {{dg.a00, dg.a01, -gm0.a02},
                                  {dg.a01, dg.a11, -gm0.a12},
                                  {-gm0.a02, -gm0.a12, c33 - gm0.a22}};
         const double Gf[3][3] = {{Gd.a00, Gd.a01, Gd.a02}, {Gd.a01, Gd.a11, Gd.a12}, {Gd.a02, Gd.a12, Gd.a22}};
         double CD[3][3];
-        for (int i = 0; i < 3; i++)
-          for (int j = 0; j < 3; j++) CD[i][j] = Ci[i][0] * Dm[0][j] + Ci[i][1] * Dm[1][j] + Ci[i][2] * Dm[2][j];
+        for (int j = 0; j < 3; j++) {
+          CD[0][j] = Ci[0][0] * Dm[0][j] + Ci[0][1] * Dm[1][j];
+          CD[1][j] = Ci[1][0] * Dm[0][j] + Ci[1][1] * Dm[1][j];
+          CD[2][j] = ci22 * Dm[2][j];
+        }
         for (int i = 0; i < 3; i++)
           for (int j = 0; j < 3; j++)
             E.m[i][j] = (R)(-(CD[i][0] * Gf[0][j] + CD[i][1] * Gf[1][j] + CD[i][2] * Gf[2][j]));

@@ -547,14 +547,23 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       v0 = v1 = v2 = 0.0;
     }
   }
-  // FP64: straight into the hand-over list, no compaction pass; the list atomic is issued
-  // here and its offset used after phase 4
+  // FP64, and the single-context FP32-mixed path: straight into one list (the hand-over
+  // list; the second tier's input), no compaction pass; the list atomic is issued here and
+  // its offset used after phase 4.  Batches on the FP32-mixed path: per-wave segments.
+  const bool direct = sizeof(R) == 8 || td.wlist2 != nullptr;
+  int* hcount = td.list_count;
+  int* hlist = td.list_e;
+  if (sizeof(R) == 4 && td.wlist2) {  // this tile's region of the flat list
+    const int j = t / td.t1_tiles;
+    hcount += kT1Stride * j;
+    hlist += (size_t)j * td.t1_tiles * T;
+  }
   unsigned long long hbal = 0ull;
   int hbase = 0;
   if (elem_thread) {
-    if (sizeof(R) == 8) {
+    if (direct) {
       hbal = __ballot(live && handed);
-      hbase = hand_list_reserve(td, hbal);
+      hbase = hand_list_reserve(hcount, hbal);
     } else {  // per-wave segments, which the second tier re-evaluates
       const int w = t * (T >> 6) + (tid >> 6);
       HandSink{td.wlist, td.wqlist, td.wcount, w}.push(live && handed, qfail, e, P);
@@ -647,7 +656,7 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       pr[2] = az;
     }
   }
-  if (sizeof(R) == 8 && elem_thread) hand_list_write(td, hbal, hbase, live && handed, e);
+  if (direct && elem_thread) hand_list_write(td, hlist, hbal, hbase, live && handed, e);
   if (elem_thread && live) {
     td.deferred[e] = handed ? 1 : 0;
     if (!handed) {
@@ -700,39 +709,75 @@ __device__ __forceinline__ void push_failing_pair(int* list, int w, bool fail, i
   }
 }
 
-// Second tier of the FP32-mixed path: wave w re-evaluates the elements its tile wave handed
-// over (segment w of td.wlist, up to 64) with the FP64 staged chain, in passes of
-// floor(64 / n_qp) lane groups, and rewrites the segment in place with what that chain
-// cannot hold within the mixed budget either (the FP32 chain keeps the well-conditioned
-// elements: rounding bound 1e-11, C33 in [0.4, 2.5]; the FP64 chain the moderately
-// conditioned ones: bound 1e-8, C33 in [0.1, 10]), and its pair segment with their failing
-// qps.  The forces of the elements it keeps go to the element-force buffer (they stay
-// deferred: the node pass adds them).
-__global__ void __launch_bounds__(kStagedBlock) bf_tile_tier2_kernel(DevMesh m, KParams kp,
-                                                                     TileDev td, int n_waves,
-                                                                     const double* __restrict__ x,
-                                                                     const double* __restrict__ y,
-                                                                     const double* __restrict__ z) {
+// Second tier of the single-context FP32-mixed path: the elements the tile kernel handed
+// over (the regions of its flat list, TileDev::t1_tiles, taken in region order) are
+// re-evaluated with the FP64 staged chain,
+// floor(64 / n_qp) lane groups per pass, and what that chain cannot hold within the mixed
+// budget either is listed again (the FP32 chain keeps the well-conditioned elements:
+// rounding bound 1e-11, C33 in [0.4, 2.5]; the FP64 chain the moderately conditioned ones:
+// bound 1e-8, C33 in [0.1, 10]).  Wave k takes passes k, k + n_seg, ..: every wave has at
+// most one pass's latency unless more than n_seg passes are needed, and its own output
+// segment k (elements in td.wlist2, their failing qps in the pair segments of k, packed
+// counts in td.wcount2[k]) holds at most floor(64 / per_wave) passes, which n_seg is sized
+// for (kTier2Span).  The forces of the elements it keeps go to the element-force buffer
+// (they stay deferred: the node pass adds them).
+constexpr int kTier2Span = 60;  // min over n_qp <= 5 of per_wave * floor(64 / per_wave)
+__host__ __device__ inline int tier2_segments(int cap) { return (cap + kTier2Span - 1) / kTier2Span; }
+// Tiles per region of the FP32-mixed tile kernel's flat list, and its entries in all.
+inline int t1_region_tiles(int n_tiles) { return std::max(1, (n_tiles + kT1Regions - 1) / kT1Regions); }
+inline size_t t1_list_entries(size_t n_tiles, size_t T) {
+  return (size_t)kT1Regions * t1_region_tiles((int)n_tiles) * T;
+}
+
+__global__ void __launch_bounds__(64) bf_tile_tier2_kernel(DevMesh m, KParams kp, TileDev td,
+                                                           int n_seg,
+                                                           const double* __restrict__ x,
+                                                           const double* __restrict__ y,
+                                                           const double* __restrict__ z) {
   const int P = kp.n_qp, per_wave = 64 / P;
   int slot, q, base;
-  lane_group(P, threadIdx.x & 63, slot, q, base);
-  const int w = logical_block() * (kStagedBlock / 64) + (threadIdx.x >> 6);
-  if (w >= n_waves) return;
-  const int n = count_elems(td.wcount[w]);
-  if (n == 0) return;
-  int* seg = td.wlist + (size_t)w * 64;
-  int pushed = 0;
-  int qpushed[kMaxQp] = {};
-  for (int k0 = 0; k0 < n; k0 += per_wave) {
-    const int i = k0 + slot;
-    if (slot < per_wave && i < n) {
-      const int e = seg[i];
-      const int r = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
-      TileSink{seg, &pushed}.push(r != kStagedKept && q == 0, e);
-      push_failing_pair(td.wqlist, w, r == kStagedQpFail, e, q, P, qpushed);
-    }
+  const int lane = (int)(threadIdx.x & 63u);
+  lane_group(P, lane, slot, q, base);
+  const int k = (int)blockIdx.x;
+  const int cap = td.t1_tiles * td.T;  // entries per region
+  int rc[kT1Regions];
+  int total = 0;
+#pragma unroll
+  for (int j = 0; j < kT1Regions; j++) {
+    rc[j] = td.list_count[kT1Stride * j];
+    total += rc[j];
   }
-  if ((threadIdx.x & 63) == 0) td.wcount[w] = hand_counts(pushed, qpushed);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long word = 0ull;  // packed counts of segment k so far
+  for (int p = k; p * per_wave < total; p += n_seg) {
+    const int i = p * per_wave + slot;
+    const bool act = slot < per_wave && i < total;
+    int r = kStagedKept, e = 0;
+    if (act) {
+      int j = 0, off = i;  // region of entry i
+#pragma unroll
+      for (int jj = 0; jj + 1 < kT1Regions; jj++)
+        if (j == jj && off >= rc[jj]) {
+          off -= rc[jj];
+          j = jj + 1;
+        }
+      e = td.list_e[(size_t)j * cap + off];
+      r = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
+    }
+    const bool take_e = act && q == 0 && r != kStagedKept;
+    const unsigned long long be = __ballot(take_e);
+    if (take_e) td.wlist2[(size_t)k * 64 + count_elems(word) + __popcll(be & below)] = e;
+    unsigned long long add = hand_count(__popcll(be));
+#pragma unroll
+    for (int j = 0; j < kMaxQp; j++) {
+      const bool tq = j < P && act && q == j && r == kStagedQpFail;
+      const unsigned long long bq = __ballot(tq);
+      if (tq) td.wqlist[pair_segment(k, j) + count_pairs(word, j) + __popcll(bq & below)] = qp_pair(e, q);
+      add |= hand_count_pairs(j, __popcll(bq));
+    }
+    word += add;  // every field stays at most 64: no carries
+  }
+  if (lane == 0) td.wcount2[k] = word;
 }
 
 // Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
@@ -1055,6 +1100,7 @@ __global__ void bf_status_reset(int* st) {
     st[8] = 0;
     st[9] = 0;
   }
+  if (threadIdx.x < kT1Regions) st[kT1Base + kT1Stride * threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------------------ host side ----------
@@ -1218,9 +1264,11 @@ int build_ctx_tiles(bf_ctx* c, int T) {
   TALLOC(c->d_tpartial, 3 * (size_t)std::max(th.n_partials, 1));
   TALLOC(c->d_touched, (size_t)std::max(c->n_nodes, 1));
   TALLOC(c->d_deferred, (size_t)std::max(c->E, 1));
-  TALLOC(c->d_twlist, nt * T);
-  TALLOC(c->d_twcount, nt * T / 64);
-  TALLOC(c->d_twqlist, nt * T * kMaxQp);
+  // single context: the FP32-mixed tile kernel's flat list, then the second tier's segments
+  const size_t nseg2 = (size_t)tier2_segments((int)(nt * T));
+  TALLOC(c->d_twlist, t1_list_entries(nt, T) + nseg2 * 64);
+  TALLOC(c->d_twcount, nseg2);
+  TALLOC(c->d_twqlist, nseg2 * 64 * kMaxQp);
 #undef TALLOC
   hipStream_t s = c->stream;
   if (th.n_tiles > 0) {
@@ -1335,9 +1383,10 @@ ScratchLayout noalloc_layout(size_t E, size_t NS, size_t n_nodes, size_t n_tiles
   L.partial = o;  o += up(sizeof(double) * 3 * std::max<size_t>(n_partials, 1));
   L.touched = o;  o += up(std::max<size_t>(n_nodes, 1));
   L.deferred = o; o += up(En);
-  L.twlist = o;   o += up(sizeof(int) * nt * T);
-  L.twcount = o;  o += up(sizeof(unsigned long long) * nt * T / 64);
-  L.twqlist = o;  o += up(sizeof(int) * nt * T * kMaxQp);
+  const size_t nseg2 = (size_t)tier2_segments((int)(nt * T));
+  L.twlist = o;   o += up(sizeof(int) * (t1_list_entries(nt, T) + nseg2 * 64));
+  L.twcount = o;  o += up(sizeof(unsigned long long) * nseg2);
+  L.twqlist = o;  o += up(sizeof(int) * nseg2 * 64 * kMaxQp);
   L.total = o;
   return L;
 }
@@ -1378,7 +1427,7 @@ int bf_ctx_create(int device, bf_ctx** out) {
   bool events = true;
   for (auto& e : c->own_ev) events = events && hipEventCreate(&e) == hipSuccess;
   if (!events || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc((void**)&c->d_status, kStatusWords * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_status, kStatusAlloc * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_status, kStatusWords * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     bf_ctx_destroy(c);
     return BF_ERR_DEVICE;
@@ -1709,6 +1758,9 @@ static TileDev tile_dev(const bf_ctx* c) {
   td.wlist = c->d_twlist;
   td.wcount = c->d_twcount;
   td.wqlist = c->d_twqlist;
+  td.t1_tiles = t1_region_tiles(c->n_tiles);
+  td.wlist2 = c->d_twlist ? c->d_twlist + t1_list_entries(c->n_tiles, c->tileT) : nullptr;
+  td.wcount2 = c->d_twcount;
   return td;
 }
 
@@ -1728,8 +1780,11 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   const KParams kp = dev_params(c);
   const DevMesh dm = view ? *view : dev_mesh(c);
   TileDev td = tview ? *tview : tile_dev(c);
-  td.list_count = dm.status + 4;  // status[4], zeroed by bf_status_reset
-  td.list_e = dm.fallback;
+  // FP64: the hand-over list (status[4]); FP32-mixed: the second tier's input list (its
+  // region counters); both zeroed by bf_status_reset
+  const bool mixed = precision == BF_PREC_MIXED;
+  td.list_count = dm.status + (mixed ? kT1Base : 4);
+  td.list_e = mixed ? td.wlist : dm.fallback;
   td.list_ce = nullptr;
   td.list_case = 0;
   const bool staged = precision != BF_PREC_FP64;
@@ -1767,14 +1822,15 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   c->fb_prec = -1;
   if (c->E > 0 && staged && kp.fallback) {
     const int n_waves = td.n_tiles * (td.T / 64);
+    const int n_seg = tier2_segments(n_waves * 64);
     if (precision == BF_PREC_MIXED)
-      hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3((n_waves + 3) / 4), dim3(kStagedBlock), 0, s,
-                         dm, kp, td, n_waves, x, y, z);
+      hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3(n_seg), dim3(64), 0, s, dm, kp, td, n_seg, x,
+                         y, z);
     const bool pairs = precision == BF_PREC_MIXED;
     if (pairs)  // the FP64 tile kernel listed its elements itself
-      hipLaunchKernelGGL(bf_compact_kernel, dim3(std::max(1, (n_waves + kCompactBlock - 1) / kCompactBlock)), dim3(kCompactBlock), 0, s,
-                         (const unsigned long long*)td.wcount, (const int*)td.wlist,
-                         (const int*)td.wqlist, n_waves, kp.n_qp, dm.fallback, dm.qlist,
+      hipLaunchKernelGGL(bf_compact_kernel, dim3(std::max(1, (n_seg + kCompactBlock - 1) / kCompactBlock)), dim3(kCompactBlock), 0, s,
+                         (const unsigned long long*)td.wcount2, (const int*)td.wlist2,
+                         (const int*)td.wqlist, n_seg, kp.n_qp, dm.fallback, dm.qlist,
                          dm.status);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
                                          pairs, ev[4], ev[5], s));
@@ -1888,6 +1944,8 @@ int bf_force_eval_device_noalloc(bf_ctx* c, const double* d_x, const double* d_y
   td.wlist = (int*)(base + L.twlist);
   td.wcount = (unsigned long long*)(base + L.twcount);
   td.wqlist = (int*)(base + L.twqlist);
+  td.wlist2 = td.wlist + t1_list_entries(c->n_tiles, c->tileT);
+  td.wcount2 = td.wcount;
   // the touched marks start at zero (the node pass clears what the tile kernel sets; the
   // array is shared with other meshes' calls, so it is cleared here, by a kernel)
   if (c->n_nodes > 0)
@@ -2306,6 +2364,8 @@ int batch_build(bf_batch* b) {
     cd.td.wlist = b->d_tlist + wave_base * 64;
     cd.td.wcount = nullptr;  // set below, once the count arrays exist
     cd.td.wqlist = b->d_tqlist + wave_base * 64 * kMaxQp;
+    cd.td.wlist2 = nullptr;  // the batch's second tier rewrites its segments in place
+    cd.td.wcount2 = nullptr;
     for (int t = 0; t < c->n_tiles; t++) {
       tt.push_back(Tile{i, t, 0, 0});
       for (int w = 0; w < c->tileT / 64; w++) et.push_back(Tile{i, t, w, 0});

@@ -49,6 +49,7 @@ struct DevMesh {
 // see it, plus its per-evaluation arrays.
 struct TileDev {
   int n_tiles, NS, T, lmax;     // tiles, patch slots, elements (threads) per tile, node cap
+  int t1_tiles;                 // FP32-mixed single context: tiles per region of the flat list
   const int* elem;              // [n_tiles * T] element of each lane, -1 if none
   const unsigned short* lslot;  // [n_tiles][NS][T] local node of each slot, 0xFFFF unused
   const int* node_ptr;          // [n_tiles + 1] range of each tile's local nodes
@@ -68,6 +69,15 @@ struct TileDev {
   int* wqlist;                  // [n_tiles * T * kMaxQp] (element, qp) pair segments,
                                 // 64 entries per wave and qp (pair_segment)
   unsigned long long* wcount;   // [n_tiles * T / 64] packed segment counts (HandSink)
+  // Single-context FP32-mixed path: the tile kernel lists its handed-over elements in a flat
+  // list of kT1Regions regions (tiles [j t1_tiles, (j + 1) t1_tiles) append to region j of
+  // list_e = wlist, t1_tiles * T entries each, at offsets from their own counter
+  // list_count[kT1Stride j]: the tiles' atomics spread over eight addresses), and the second
+  // tier writes what it still hands over to its own segments, one per second-tier wave:
+  // elements in wlist2, pairs in wqlist, packed counts in wcount2.  Null in batches (their
+  // tiles keep per-wave segments).
+  int* wlist2;                  // [n_seg2 * 64]
+  unsigned long long* wcount2;  // [n_seg2]
   // The FP64 path lists its handed-over elements directly in the hand-over list, at offsets
   // from one device-scope atomic per wave on *list_count (the list's order does not matter:
   // every listed element is solved and finished on its own): list_e (single context,
@@ -85,6 +95,12 @@ struct TileDev {
 // status[8]: (element, qp) pairs handed over (m.qlist, ordered by qp_of_rank)
 // status[9]: how many of them, at the end of the list, are mid-surface pairs (has_mid_qp)
 constexpr int kStatusWords = 16;
+// The FP32-mixed single-context tile kernel's region counters (TileDev::t1_tiles), after the
+// status words in the same allocation, 256 bytes apart.
+constexpr int kT1Regions = 8;
+constexpr int kT1Stride = 64;
+constexpr int kT1Base = 64;
+constexpr int kStatusAlloc = kT1Base + kT1Regions * kT1Stride;
 
 // An (element, qp) pair of the hand-over solve.
 __host__ __device__ inline int qp_pair(int e, int q) { return (e << 3) | q; }
@@ -192,25 +208,25 @@ struct HandSink {
 // The counter's address goes through an empty asm statement so that the compiler sees a
 // per-lane address: its rewrite of a uniform-address atomic into a wave reduction reads the
 // returned value right away, which would wait for the atomic here.
-__device__ inline int hand_list_reserve(const TileDev& td, unsigned long long bal) {
+__device__ inline int hand_list_reserve(int* count, unsigned long long bal) {
   int base = 0;
   const int lane = (int)(threadIdx.x & 63u);
   if (bal != 0ull && lane == __ffsll((long long)bal) - 1) {
     int zero = 0;
     asm volatile("" : "+v"(zero));
-    base = atomicAdd(td.list_count + zero, __popcll(bal));
+    base = atomicAdd(count + zero, __popcll(bal));
   }
   return base;
 }
-__device__ inline void hand_list_write(const TileDev& td, unsigned long long bal, int base,
-                                       bool take, int e) {
+__device__ inline void hand_list_write(const TileDev& td, int* list_e, unsigned long long bal,
+                                       int base, bool take, int e) {
   if (bal == 0ull) return;
   const int lane = (int)(threadIdx.x & 63u);
   base = __shfl(base, __ffsll((long long)bal) - 1);
   if (!take) return;
   const int at = base + __popcll(bal & ((1ull << lane) - 1ull));
   if (td.list_ce) td.list_ce[at] = make_int2(td.list_case, e);
-  else td.list_e[at] = e;
+  else list_e[at] = e;
 }
 
 // The form of a wave that works through one segment in passes of floor(64 / P) elements

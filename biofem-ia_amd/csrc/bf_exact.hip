@@ -567,12 +567,12 @@ __device__ __forceinline__ int next_item(int* counter) {
   return __shfl(i, 0) + gridDim.x * (blockDim.x >> 6);
 }
 
-// The hand-over solve at split factor K over list[0 .. N) (HandItems): wave w takes item w
-// first, then items from the counter.
+// The hand-over solve at split factor K over the first N entries of a SegList (HandItems):
+// wave w takes item w first, then items from the counter.
 template <int K>
 __device__ void solve_list_k(const DevMesh& m, const KParams& kp, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
-                             const int* __restrict__ list, int N, bool pairs, int* counter) {
+                             SegList list, SegCount sc, int N, bool pairs, int* counter) {
   constexpr int NG = split_groups(K);
   const HandItems hi{N, kp.n_qp, NG, pairs};
   const int items = hi.n_items();
@@ -581,29 +581,44 @@ __device__ void solve_list_k(const DevMesh& m, const KParams& kp, const double* 
   for (int i = wave_id(); i < items; i = next_item(counter)) {
     int idx, q;
     if (g < NG && hi.at(i, g, idx, q)) {
-      const int v = list[idx];
+      const int v = seg_at(list, sc, idx);
       const int e = pairs ? pair_elem(v) : v;
       exact_solve_k<K>(m, kp, x, y, z, e, pairs ? pair_qp(v) : q, j);
     }
   }
 }
 
+// Mid-surface entries at the end of a hand-over list: the pair classes are the qp ranks, the
+// mid-surface point last (has_mid_qp); element lists have none.
+__device__ __forceinline__ int list_mid(const SegCount& sc, int P, bool pairs) {
+  int v = 0;  // sc.c[P - 1] by selects: a run-time index would put sc in scratch memory
+#pragma unroll
+  for (int j = 0; j < kRegions; j++) v = (j == P - 1) ? sc.c[j] : v;
+  return (pairs && has_mid_qp(P)) ? v : 0;
+}
+
 // Solve over the hand-over list: one-wave blocks, persistent grid of n_waves waves, split
-// factor from the list length.  pairs: list = m.qlist, *count pairs (m.status[9] of them
-// mid-surface pairs at the end when *count is the list's full length), else list =
-// m.fallback, *count elements.
+// factor from the list length.  pairs: the pair classes of m.qlist, else the element regions
+// of m.fallback.  narrow (not null): the wide kernel ran first and left here 0 if it took the
+// list.  The first lane also stores the list's length in the status words the host reads
+// (pairs: status[8], status[9]; elements: status[4] is stored by the finish kernel).
 __global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
     DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
-    int pairs, int* __restrict__ counter, int n_waves) {
-  const int N = count[0];
-  const int N_mid = (pairs && N > 0) ? m.status[9] : 0;
-  switch (split_factor(N, N_mid, kp.n_qp, pairs != 0, n_waves, false)) {
-    case 8: solve_list_k<8>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
-    case 6: solve_list_k<6>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
-    case 4: solve_list_k<4>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
-    case 2: solve_list_k<2>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
-    default: solve_list_k<1>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    const double* __restrict__ z, SegList list, const int* __restrict__ narrow, int pairs,
+    int* __restrict__ counter, int n_waves) {
+  const SegCount sc = seg_count(list);
+  const int N_mid = list_mid(sc, kp.n_qp, pairs != 0);
+  if (pairs && blockIdx.x == 0 && threadIdx.x == 0) {
+    m.status[8] = sc.total;
+    m.status[9] = N_mid;
+  }
+  const int N = narrow ? *narrow : sc.total;
+  switch (split_factor(N, N > 0 ? N_mid : 0, kp.n_qp, pairs != 0, n_waves, false)) {
+    case 8: solve_list_k<8>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
+    case 6: solve_list_k<6>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
+    case 4: solve_list_k<4>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
+    case 2: solve_list_k<2>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
+    default: solve_list_k<1>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
   }
 }
 
@@ -613,16 +628,17 @@ __global__ void __launch_bounds__(64) bf_exact_solve_queue_kernel(
 // split (then this kernel does nothing else).
 __global__ void __launch_bounds__(64) bf_exact_solve_wide_kernel(
     DevMesh m, KParams kp, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const int* __restrict__ list, const int* __restrict__ count,
-    int pairs, int* __restrict__ counter, int n_waves, int* __restrict__ narrow_count) {
-  const int N = *count;
-  const int N_mid = pairs ? m.status[9] : 0;
+    const double* __restrict__ z, SegList list, int pairs, int* __restrict__ counter,
+    int n_waves, int* __restrict__ narrow_count) {
+  const SegCount sc = seg_count(list);
+  const int N = sc.total;
+  const int N_mid = list_mid(sc, kp.n_qp, pairs != 0);
   const int K = N > 0 ? split_factor(N, N_mid, kp.n_qp, pairs != 0, n_waves, true) : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_count = K >= kWideSplitMin ? 0 : N;
   switch (K) {
-    case 60: solve_list_k<60>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
-    case 30: solve_list_k<30>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
-    case 15: solve_list_k<15>(m, kp, x, y, z, list, N, pairs != 0, counter); break;
+    case 60: solve_list_k<60>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
+    case 30: solve_list_k<30>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
+    case 15: solve_list_k<15>(m, kp, x, y, z, list, sc, N, pairs != 0, counter); break;
     default: break;  // the queue kernel's
   }
 }
@@ -633,21 +649,26 @@ struct RedoSink {
   __device__ void operator()(int e) const { list[atomicAdd(count, 1)] = e; }
 };
 
+// list.v == nullptr: elements 0 .. n_static - 1; else the list's entries, whose length the
+// first lane stores in status[4] (the count the host reads).
 __global__ void __launch_bounds__(256) bf_exact_finish_kernel(DevMesh m, KParams kp,
                                                               const double* __restrict__ x,
                                                               const double* __restrict__ y,
                                                               const double* __restrict__ z,
-                                                              const int* __restrict__ list,
-                                                              const int* __restrict__ count,
-                                                              int n_static,
+                                                              SegList list, int n_static,
                                                               unsigned char extra_flags) {
-  const long long N = (count ? *count : n_static) * (long long)kFinishLanes;
+  SegCount sc{};
+  if (list.v) {
+    sc = seg_count(list);
+    if (blockIdx.x == 0 && threadIdx.x == 0) m.status[4] = sc.total;
+  }
+  const long long N = (list.v ? sc.total : n_static) * (long long)kFinishLanes;
   const RedoSink redo{m.redo, &m.status[5]};
   // the stride is a multiple of kFinishLanes: an element's lanes stay together
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N;
        t += (long long)gridDim.x * blockDim.x) {
     const int idx = (int)(t / kFinishLanes), slice = (int)(t % kFinishLanes);
-    exact_finish(m, kp, x, y, z, list ? list[idx] : idx, extra_flags, redo, slice);
+    exact_finish(m, kp, x, y, z, list.v ? seg_at(list, sc, idx) : idx, extra_flags, redo, slice);
   }
 }
 
@@ -1066,34 +1087,31 @@ hipError_t launch_geom0(const DevMesh& m, int E, const double* x, const double* 
 
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, int n_cu, bool wide,
-                                         bool pairs, hipEvent_t ev_solve0, hipEvent_t ev_solve1,
-                                         hipStream_t s) {
+                                         bool pairs, const SegList& solve_list,
+                                         const SegList& elem_list, hipEvent_t ev_solve0,
+                                         hipEvent_t ev_solve1, hipStream_t s) {
   if (kp.E <= 0) return hipSuccess;
   // persistent grids of one wave per SIMD (the solve's occupancy; the list length is known
   // on the device only).  With `wide` the wide solve kernel runs first and leaves the
   // narrow one its count in status[7] (0 if it took the list); without it the narrow kernel
   // reads the list length itself.
-  const int* list = pairs ? m.qlist : m.fallback;
-  const int full = pairs ? 8 : 4;
   if (ev_solve0) {
     const hipError_t e = hipEventRecord(ev_solve0, s);
     if (e != hipSuccess) return e;
   }
   if (wide)
     hipLaunchKernelGGL(bf_exact_solve_wide_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
-                       z, list, (const int*)&m.status[full], pairs ? 1 : 0, &m.status[6],
-                       4 * n_cu, &m.status[7]);
+                       z, solve_list, pairs ? 1 : 0, &m.status[6], 4 * n_cu, &m.status[7]);
   hipLaunchKernelGGL(bf_exact_solve_queue_kernel, dim3(4 * n_cu), dim3(64), 0, s, m, kp, x, y,
-                     z, list, (const int*)&m.status[wide ? 7 : full], pairs ? 1 : 0,
-                     &m.status[6], 4 * n_cu);
+                     z, solve_list, wide ? (const int*)&m.status[7] : (const int*)nullptr,
+                     pairs ? 1 : 0, &m.status[6], 4 * n_cu);
   if (ev_solve1) {
     const hipError_t e = hipEventRecord(ev_solve1, s);
     if (e != hipSuccess) return e;
   }
   // 64-lane blocks: a short hand-over list still spreads over the CUs
   hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x, y, z,
-                     (const int*)m.fallback, (const int*)&m.status[4], 0,
-                     (unsigned char)kStatFallback);
+                     elem_list, 0, (unsigned char)kStatFallback);
   hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,
                      z, (unsigned char)kStatFallback);
   return hipGetLastError();
@@ -1107,8 +1125,7 @@ hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const doubl
                      kp, x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E);
   hipLaunchKernelGGL(bf_exact_finish_kernel,
                      dim3((unsigned)(((long long)kp.E * kFinishLanes + 255) / 256)), dim3(256), 0,
-                     s, m, kp, x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E,
-                     (unsigned char)0);
+                     s, m, kp, x, y, z, SegList{}, kp.E, (unsigned char)0);
   hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,
                      z, (unsigned char)0);
   return hipGetLastError();

@@ -1,19 +1,23 @@
 // bf_force.hip -- gfx950 kernels and C ABI of the element internal-force path.
 //
 // Pipeline of one force evaluation (replaces FInternalAct, src/active_strain.c:2226-2267):
-//   1. bf_element_kernel<R>: one group of n_qp lanes per element, one lane per
-//      through-thickness point.  Gathers the element's Loop patch from SoA coordinates, forms
-//      the current midsurface geometry (FP64), runs the staged active-strain constitutive
-//      chain of its qp in R, sums the qp resultants across the group in qp order, and writes
-//      the 3*nen nodal contributions to the slot rows of the element-force buffer.  Elements
-//      outside the well-conditioned regime are listed instead (per-wave segments, no
-//      atomics).
-//   2. bf_compact_kernel + the reference-order solve / finish / redo kernels over the list:
-//      the listed elements recomputed in the reference's operation order (bf_exact.hip).
-//   3. bf_gather_kernel: one lane per node.  Sums the node's (element, slot) contributions in
-//      ascending element order from a CSR list built at upload, in FP64: the summation order
-//      of VecSetValues(ADD_VALUES) on one rank (active_strain.c:2238-2260), deterministic run
-//      to run, without atomics.
+//   staged paths (FP64, FP32-mixed):
+//   1. bf_tile_kernel<R>: one workgroup per tile of 64 spatially close elements, one lane per
+//      (element, qp): the tile's node coordinates staged in LDS, the current midsurface
+//      geometry once per element (FP64), the staged active-strain chain of each qp in R, the
+//      qp resultants summed in qp order, the nodal contributions reduced per node in LDS in
+//      ascending element order; nodes whose contributions all lie in the tile are written,
+//      the others leave a tile partial.  Elements outside the well-conditioned regime are
+//      listed (SegList) instead of added.
+//   2. (FP32-mixed) bf_tile_tier2_kernel: the listed elements again with the FP64 staged
+//      chain; what it cannot keep is listed as (element, qp) pairs.
+//   3. the reference-order solve / finish / redo kernels over the list (bf_exact.hip).
+//   4. bf_tile_fixup_kernel (node pass): partial nodes sum their tile partials in tile order,
+//      nodes of handed-over elements add those elements' rows in ascending element order.
+//   The reference-order path: bf_exact kernels for every element, then bf_gather_kernel (one
+//   lane per node, the node's (element, slot) contributions in ascending element order from
+//   a CSR list built at upload): the summation order of VecSetValues(ADD_VALUES) on one rank
+//   (active_strain.c:2238-2260), deterministic run to run, without atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -547,15 +551,15 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       v0 = v1 = v2 = 0.0;
     }
   }
-  // FP64, and the single-context FP32-mixed path: straight into one list (the hand-over
+  // FP64, and the single-context FP32-mixed path: straight into a list (the hand-over
   // list; the second tier's input), no compaction pass; the list atomic is issued here and
   // its offset used after phase 4.  Batches on the FP32-mixed path: per-wave segments.
-  const bool direct = sizeof(R) == 8 || td.wlist2 != nullptr;
+  const bool direct = sizeof(R) == 8 || td.single;
   int* hcount = td.list_count;
   int* hlist = td.list_e;
-  if (sizeof(R) == 4 && td.wlist2) {  // this tile's region of the flat list
+  if (sizeof(R) == 4 && td.single) {  // this tile's region of the list (SegList)
     const int j = t / td.t1_tiles;
-    hcount += kT1Stride * j;
+    hcount += kCntStride * j;
     hlist += (size_t)j * td.t1_tiles * T;
   }
   unsigned long long hbal = 0ull;
@@ -709,81 +713,77 @@ __device__ __forceinline__ void push_failing_pair(int* list, int w, bool fail, i
   }
 }
 
+// Tiles per region of the tile kernel's list, and its entries in all.
+inline int t1_region_tiles(int n_tiles) { return std::max(1, (n_tiles + kRegions - 1) / kRegions); }
+inline size_t t1_list_entries(size_t n_tiles, size_t T) {
+  return (size_t)kRegions * t1_region_tiles((int)n_tiles) * T;
+}
+// The second tier's grid: two waves per tile wave, at most 32 elements per wave on average
+// over a list as long as the mesh, so a wave takes at most ceil(32 / per_wave) passes of
+// per_wave = floor(64 / n_qp) elements: 64 or fewer (n_qp <= 5).  Waves per region of its
+// element list (64 entries per wave), and that list's entries in all.
+inline int tier2_waves(int n_waves) { return 2 * std::max(n_waves, 1); }
+inline int t2_region_waves(int n_waves) { return (tier2_waves(n_waves) + kRegions - 1) / kRegions; }
+inline size_t t2_list_entries(int n_waves) { return (size_t)kRegions * t2_region_waves(n_waves) * 64; }
+
 // Second tier of the single-context FP32-mixed path: the elements the tile kernel handed
-// over (the regions of its flat list, TileDev::t1_tiles, taken in region order) are
-// re-evaluated with the FP64 staged chain,
+// over (its SegList, TileDev::t1_tiles) are re-evaluated with the FP64 staged chain,
 // floor(64 / n_qp) lane groups per pass, and what that chain cannot hold within the mixed
 // budget either is listed again (the FP32 chain keeps the well-conditioned elements:
 // rounding bound 1e-11, C33 in [0.4, 2.5]; the FP64 chain the moderately conditioned ones:
-// bound 1e-8, C33 in [0.1, 10]).  Wave k takes passes k, k + n_seg, ..: every wave has at
-// most one pass's latency unless more than n_seg passes are needed, and its own output
-// segment k (elements in td.wlist2, their failing qps in the pair segments of k, packed
-// counts in td.wcount2[k]) holds at most floor(64 / per_wave) passes, which n_seg is sized
-// for (kTier2Span).  The forces of the elements it keeps go to the element-force buffer
-// (they stay deferred: the node pass adds them).
-constexpr int kTier2Span = 60;  // min over n_qp <= 5 of per_wave * floor(64 / per_wave)
-__host__ __device__ inline int tier2_segments(int cap) { return (cap + kTier2Span - 1) / kTier2Span; }
-// Tiles per region of the FP32-mixed tile kernel's flat list, and its entries in all.
-inline int t1_region_tiles(int n_tiles) { return std::max(1, (n_tiles + kT1Regions - 1) / kT1Regions); }
-inline size_t t1_list_entries(size_t n_tiles, size_t T) {
-  return (size_t)kT1Regions * t1_region_tiles((int)n_tiles) * T;
-}
-
+// bound 1e-8, C33 in [0.1, 10]): the element in region k / region_waves of the second
+// tier's element SegList (t2_list), its failing (element, qp) pairs in the class of their
+// qp rank of m.qlist (E entries per class), at offsets from one atomic per wave and list.
+// Wave k takes passes k, k + gridDim.x, ..: with tier2_waves no wave takes more than 64
+// elements, and a short list is one pass's latency.  The forces of the elements it keeps go
+// to the element-force buffer (they stay deferred: the node pass adds them).
 __global__ void __launch_bounds__(64) bf_tile_tier2_kernel(DevMesh m, KParams kp, TileDev td,
-                                                           int n_seg,
+                                                           int* __restrict__ t2_list,
+                                                           int region_waves,
                                                            const double* __restrict__ x,
                                                            const double* __restrict__ y,
                                                            const double* __restrict__ z) {
-  const int P = kp.n_qp, per_wave = 64 / P;
+  const int P = kp.n_qp, per_wave = 64 / P, E = kp.E;
   int slot, q, base;
-  const int lane = (int)(threadIdx.x & 63u);
-  lane_group(P, lane, slot, q, base);
+  lane_group(P, (int)(threadIdx.x & 63u), slot, q, base);
   const int k = (int)blockIdx.x;
-  const int cap = td.t1_tiles * td.T;  // entries per region
-  int rc[kT1Regions];
-  int total = 0;
-#pragma unroll
-  for (int j = 0; j < kT1Regions; j++) {
-    rc[j] = td.list_count[kT1Stride * j];
-    total += rc[j];
-  }
-  const unsigned long long below = (1ull << lane) - 1ull;
-  unsigned long long word = 0ull;  // packed counts of segment k so far
-  for (int p = k; p * per_wave < total; p += n_seg) {
+  const SegList in{td.list_e, td.list_count, td.t1_tiles * td.T, kRegions};
+  const SegCount sc = seg_count(in);
+  const int j2 = k / region_waves;
+  int* ecount = m.status + kT2Base + kCntStride * j2;
+  int* elist = t2_list + (size_t)j2 * region_waves * 64;
+  for (int p = k; p * per_wave < sc.total; p += (int)gridDim.x) {
     const int i = p * per_wave + slot;
-    const bool act = slot < per_wave && i < total;
+    const bool act = slot < per_wave && i < sc.total;
     int r = kStagedKept, e = 0;
     if (act) {
-      int j = 0, off = i;  // region of entry i
-#pragma unroll
-      for (int jj = 0; jj + 1 < kT1Regions; jj++)
-        if (j == jj && off >= rc[jj]) {
-          off -= rc[jj];
-          j = jj + 1;
-        }
-      e = td.list_e[(size_t)j * cap + off];
+      e = seg_at(in, sc, i);
       r = staged_element<double>(m, kp, x, y, z, e, q, base, kC33NoiseMaxTier2);
     }
+    // the atomics of every list are issued before any offset is used
     const bool take_e = act && q == 0 && r != kStagedKept;
     const unsigned long long be = __ballot(take_e);
-    if (take_e) td.wlist2[(size_t)k * 64 + count_elems(word) + __popcll(be & below)] = e;
-    unsigned long long add = hand_count(__popcll(be));
+    const int eb = hand_list_reserve(ecount, be);
+    unsigned long long bq[kMaxQp];
+    int qb[kMaxQp];
 #pragma unroll
-    for (int j = 0; j < kMaxQp; j++) {
-      const bool tq = j < P && act && q == j && r == kStagedQpFail;
-      const unsigned long long bq = __ballot(tq);
-      if (tq) td.wqlist[pair_segment(k, j) + count_pairs(word, j) + __popcll(bq & below)] = qp_pair(e, q);
-      add |= hand_count_pairs(j, __popcll(bq));
+    for (int rk = 0; rk < kMaxQp; rk++) {
+      const bool tq = rk < P && act && r == kStagedQpFail && q == qp_of_rank(rk, P);
+      bq[rk] = __ballot(tq);
+      qb[rk] = hand_list_reserve(m.status + kClassBase + kCntStride * rk, bq[rk]);
     }
-    word += add;  // every field stays at most 64: no carries
+    hand_list_write(td, elist, be, eb, take_e, e);
+#pragma unroll
+    for (int rk = 0; rk < kMaxQp; rk++)
+      hand_list_write(td, m.qlist + (size_t)rk * E, bq[rk], qb[rk],
+                      ((bq[rk] >> (threadIdx.x & 63u)) & 1ull) != 0ull, qp_pair(e, q));
   }
-  if (lane == 0) td.wcount2[k] = word;
 }
 
-// Concatenates the staged kernel's per-wave hand-over segments (in wave order) into
-// m.fallback and stores the total in m.status[4], and its pair segments into m.qlist, qp by
-// qp in qp_of_rank order, with the total in m.status[8] and the mid-surface pairs (the last
-// class) in m.status[9].  One thread per wave, blocks of kCompactBlock waves: every block
+// Batches on the FP32-mixed path: concatenates the tile waves' hand-over segments (in wave
+// order) into one element list and their pair segments into one pair list, qp by qp in
+// qp_of_rank order, and stores the totals and the mid-surface pairs (the last class) at the
+// end (set_totals).  One thread per wave, blocks of kCompactBlock waves: every block
 // reads all the packed count words (loads in flight together) and forms the class totals
 // and the sums of the waves before its own, then a block-wide scan gives each wave its
 // offsets, and each thread copies its wave's segments with the loads of every class in
@@ -927,20 +927,6 @@ __device__ __forceinline__ void compact_all(int* s_wave, const unsigned long lon
     }
   }
   if (blockIdx.x == 0 && t == 0) set_totals(total[0], acc, has_mid_qp(P) ? total[P] : 0);
-}
-
-__global__ void __launch_bounds__(kCompactBlock) bf_compact_kernel(
-    const unsigned long long* __restrict__ cnt, const int* __restrict__ wlist,
-    const int* __restrict__ wqlist, int n_waves, int P, int* __restrict__ out,
-    int* __restrict__ qout, int* __restrict__ status) {
-  __shared__ int s_wave[2 * kCompactWaves * kCompactClasses];
-  compact_all(s_wave, cnt, wlist, wqlist, n_waves, P,
-              [&](int at, int, int v) { out[at] = v; }, [&](int at, int, int v) { qout[at] = v; },
-              [&](int ne, int np, int nm) {
-                status[4] = ne;
-                status[8] = np;
-                status[9] = nm;
-              });
 }
 
 __global__ void __launch_bounds__(256) bf_gather_kernel(const int* __restrict__ ptr,
@@ -1100,7 +1086,8 @@ __global__ void bf_status_reset(int* st) {
     st[8] = 0;
     st[9] = 0;
   }
-  if (threadIdx.x < kT1Regions) st[kT1Base + kT1Stride * threadIdx.x] = 0;
+  // the SegList counters: kRegions tile regions, kRegions second-tier regions, kMaxQp classes
+  if (threadIdx.x < 2 * kRegions + kMaxQp) st[kT1Base + kCntStride * threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------------------ host side ----------
@@ -1137,7 +1124,7 @@ struct bf_ctx {
   unsigned char* d_flags = nullptr;
   int* d_fallback = nullptr;
   int* d_qlist = nullptr;
-  int* last_qlist = nullptr;  // hand-over list of the last staged evaluation (own or
+  SegList last_list{};  // hand-over list of the last staged evaluation (own or
                               // scratch): pairs (FP32-mixed) or elements (FP64)
   double* d_qsv = nullptr;
   int* d_qst = nullptr;
@@ -1183,8 +1170,6 @@ struct bf_ctx {
   unsigned char* d_touched = nullptr;
   unsigned char* d_deferred = nullptr;
   int* d_twlist = nullptr;
-  unsigned long long* d_twcount = nullptr;
-  int* d_twqlist = nullptr;
 };
 
 namespace {
@@ -1218,7 +1203,7 @@ void free_tiles(bf_ctx* c) {
   dfree(c->d_telem); dfree(c->d_tlslot); dfree(c->d_tcptr); dfree(c->d_tcent);
   dfree(c->d_tnode_ptr); dfree(c->d_tnode); dfree(c->d_tout); dfree(c->d_tpptr);
   dfree(c->d_tcomplete); dfree(c->d_tpartial); dfree(c->d_touched); dfree(c->d_deferred);
-  dfree(c->d_twlist); dfree(c->d_twcount); dfree(c->d_twqlist);
+  dfree(c->d_twlist);
   c->n_tiles = c->n_partials = c->tileT = 0;
 }
 
@@ -1264,11 +1249,9 @@ int build_ctx_tiles(bf_ctx* c, int T) {
   TALLOC(c->d_tpartial, 3 * (size_t)std::max(th.n_partials, 1));
   TALLOC(c->d_touched, (size_t)std::max(c->n_nodes, 1));
   TALLOC(c->d_deferred, (size_t)std::max(c->E, 1));
-  // single context: the FP32-mixed tile kernel's flat list, then the second tier's segments
-  const size_t nseg2 = (size_t)tier2_segments((int)(nt * T));
-  TALLOC(c->d_twlist, t1_list_entries(nt, T) + nseg2 * 64);
-  TALLOC(c->d_twcount, nseg2);
-  TALLOC(c->d_twqlist, nseg2 * 64 * kMaxQp);
+  // the tile kernel's list (the FP64 hand-over list, the FP32-mixed second tier's input),
+  // then the second tier's element list
+  TALLOC(c->d_twlist, t1_list_entries(nt, T) + t2_list_entries((int)(nt * T / 64)));
 #undef TALLOC
   hipStream_t s = c->stream;
   if (th.n_tiles > 0) {
@@ -1365,7 +1348,7 @@ int noalloc_scratch_init(int device) {
 // Byte offsets of the per-evaluation arrays of a mesh (E elements, NS patch slots) in the
 // static scratch, 256-byte aligned; returns the total.
 struct ScratchLayout {
-  size_t Fe, fallback, qlist, qsv, qst, redo, partial, touched, deferred, twlist, twcount,
+  size_t Fe, fallback, qlist, qsv, qst, redo, partial, touched, deferred, twlist,
       twqlist, total;
 };
 ScratchLayout noalloc_layout(size_t E, size_t NS, size_t n_nodes, size_t n_tiles, size_t T,
@@ -1383,10 +1366,7 @@ ScratchLayout noalloc_layout(size_t E, size_t NS, size_t n_nodes, size_t n_tiles
   L.partial = o;  o += up(sizeof(double) * 3 * std::max<size_t>(n_partials, 1));
   L.touched = o;  o += up(std::max<size_t>(n_nodes, 1));
   L.deferred = o; o += up(En);
-  const size_t nseg2 = (size_t)tier2_segments((int)(nt * T));
-  L.twlist = o;   o += up(sizeof(int) * (t1_list_entries(nt, T) + nseg2 * 64));
-  L.twcount = o;  o += up(sizeof(unsigned long long) * nseg2);
-  L.twqlist = o;  o += up(sizeof(int) * nseg2 * 64 * kMaxQp);
+  L.twlist = o;   o += up(sizeof(int) * (t1_list_entries(nt, T) + t2_list_entries((int)(nt * T / 64))));
   L.total = o;
   return L;
 }
@@ -1755,12 +1735,11 @@ static TileDev tile_dev(const bf_ctx* c) {
   td.partial = c->d_tpartial;
   td.touched = c->d_touched;
   td.deferred = c->d_deferred;
-  td.wlist = c->d_twlist;
-  td.wcount = c->d_twcount;
-  td.wqlist = c->d_twqlist;
+  td.wlist = c->d_twlist;  // per-wave segments in batches; here the tile kernel's list
+  td.wcount = nullptr;
+  td.wqlist = nullptr;
   td.t1_tiles = t1_region_tiles(c->n_tiles);
-  td.wlist2 = c->d_twlist ? c->d_twlist + t1_list_entries(c->n_tiles, c->tileT) : nullptr;
-  td.wcount2 = c->d_twcount;
+  td.single = 1;
   return td;
 }
 
@@ -1780,8 +1759,10 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   const KParams kp = dev_params(c);
   const DevMesh dm = view ? *view : dev_mesh(c);
   TileDev td = tview ? *tview : tile_dev(c);
-  // FP64: the hand-over list (status[4]); FP32-mixed: the second tier's input list (its
-  // region counters); both zeroed by bf_status_reset
+  // the tile kernel's list (counters zeroed by bf_status_reset): FP64, the hand-over list in
+  // m.fallback (one counter, status[4]: regions spread the tiles' atomics, but the solve over
+  // a region-ordered list measured 3.5 us slower than over the atomics' order); FP32-mixed,
+  // the second tier's input (SegList of kRegions regions)
   const bool mixed = precision == BF_PREC_MIXED;
   td.list_count = dm.status + (mixed ? kT1Base : 4);
   td.list_e = mixed ? td.wlist : dm.fallback;
@@ -1822,20 +1803,20 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   c->fb_prec = -1;
   if (c->E > 0 && staged && kp.fallback) {
     const int n_waves = td.n_tiles * (td.T / 64);
-    const int n_seg = tier2_segments(n_waves * 64);
-    if (precision == BF_PREC_MIXED)
-      hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3(n_seg), dim3(64), 0, s, dm, kp, td, n_seg, x,
-                         y, z);
     const bool pairs = precision == BF_PREC_MIXED;
-    if (pairs)  // the FP64 tile kernel listed its elements itself
-      hipLaunchKernelGGL(bf_compact_kernel, dim3(std::max(1, (n_seg + kCompactBlock - 1) / kCompactBlock)), dim3(kCompactBlock), 0, s,
-                         (const unsigned long long*)td.wcount2, (const int*)td.wlist2,
-                         (const int*)td.wqlist, n_seg, kp.n_qp, dm.fallback, dm.qlist,
-                         dm.status);
+    const SegList t1 = pairs ? SegList{td.wlist, dm.status + kT1Base, td.t1_tiles * td.T, kRegions}
+                             : SegList{dm.fallback, dm.status + 4, c->E, 1};
+    int* t2_list = td.wlist + t1_list_entries(td.n_tiles, td.T);
+    const SegList t2{t2_list, dm.status + kT2Base, t2_region_waves(n_waves) * 64, kRegions};
+    const SegList cls{dm.qlist, dm.status + kClassBase, c->E, kp.n_qp};
+    if (pairs)
+      hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3(tier2_waves(n_waves)), dim3(64), 0, s, dm, kp,
+                         td, t2_list, t2_region_waves(n_waves), x, y, z);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
-                                         pairs, ev[4], ev[5], s));
+                                         pairs, pairs ? cls : t1, pairs ? t2 : t1, ev[4], ev[5],
+                                         s));
     c->fb_prec = precision;
-    c->last_qlist = pairs ? dm.qlist : dm.fallback;
+    c->last_list = pairs ? cls : t1;
   }
   if (c->fb_prec < 0) {  // no hand-over solve: an empty interval
     BF_HIP(hipEventRecord(ev[4], s));
@@ -1942,10 +1923,6 @@ int bf_force_eval_device_noalloc(bf_ctx* c, const double* d_x, const double* d_y
   td.touched = (unsigned char*)(base + L.touched);
   td.deferred = (unsigned char*)(base + L.deferred);
   td.wlist = (int*)(base + L.twlist);
-  td.wcount = (unsigned long long*)(base + L.twcount);
-  td.wqlist = (int*)(base + L.twqlist);
-  td.wlist2 = td.wlist + t1_list_entries(c->n_tiles, c->tileT);
-  td.wcount2 = td.wcount;
   // the touched marks start at zero (the node pass clears what the tile kernel sets; the
   // array is shared with other meshes' calls, so it is cleared here, by a kernel)
   if (c->n_nodes > 0)
@@ -2048,13 +2025,25 @@ int bf_last_handed_qps(bf_ctx* c, unsigned char* mask, int* n_pairs) {
   if (rc != BF_OK && rc != BF_ERR_ELEMENT) return rc;
   std::memset(mask, 0, (size_t)c->E);
   if (n_pairs) *n_pairs = 0;
-  if (c->fb_prec < 0 || !c->last_qlist) return BF_OK;
-  // FP32-mixed: a pair list; FP64: an element list, every qp solved
+  if (c->fb_prec < 0 || !c->last_list.v) return BF_OK;
+  // FP32-mixed: a pair list; FP64: an element list, every qp solved.  The list's regions
+  // (SegList) are read in order, their counts from the counters after the status words.
   const bool pairs = c->fb_prec == BF_PREC_MIXED;
-  const int n = pairs ? c->h_status[8] : c->h_status[4];
-  if (n <= 0) return BF_OK;
-  std::vector<int> list((size_t)n);
-  BF_HIP(hipMemcpyAsync(list.data(), c->last_qlist, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  const SegList& L = c->last_list;
+  std::vector<int> cnt(kStatusAlloc);
+  BF_HIP(hipMemcpyAsync(cnt.data(), c->d_status, sizeof(int) * kStatusAlloc, hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipStreamSynchronize(c->stream));
+  const size_t at = (size_t)(L.cnt - c->d_status);
+  std::vector<int> list;
+  for (int j = 0; j < L.n; j++) {
+    const int n = cnt[at + (size_t)kCntStride * j];
+    if (n < 0 || n > L.cap) return BF_ERR_DEVICE;
+    const size_t o = list.size();
+    list.resize(o + (size_t)n);
+    if (n > 0)
+      BF_HIP(hipMemcpyAsync(list.data() + o, L.v + (size_t)j * L.cap, sizeof(int) * (size_t)n,
+                            hipMemcpyDeviceToHost, c->stream));
+  }
   BF_HIP(hipStreamSynchronize(c->stream));
   const unsigned all = (1u << c->kp.n_qp) - 1u;
   int count = 0;
@@ -2364,8 +2353,7 @@ int batch_build(bf_batch* b) {
     cd.td.wlist = b->d_tlist + wave_base * 64;
     cd.td.wcount = nullptr;  // set below, once the count arrays exist
     cd.td.wqlist = b->d_tqlist + wave_base * 64 * kMaxQp;
-    cd.td.wlist2 = nullptr;  // the batch's second tier rewrites its segments in place
-    cd.td.wcount2 = nullptr;
+    cd.td.single = 0;  // per-wave segments (FP32-mixed), one (case, element) list (FP64)
     for (int t = 0; t < c->n_tiles; t++) {
       tt.push_back(Tile{i, t, 0, 0});
       for (int w = 0; w < c->tileT / 64; w++) et.push_back(Tile{i, t, w, 0});

@@ -49,7 +49,7 @@ struct DevMesh {
 // see it, plus its per-evaluation arrays.
 struct TileDev {
   int n_tiles, NS, T, lmax;     // tiles, patch slots, elements (threads) per tile, node cap
-  int t1_tiles;                 // FP32-mixed single context: tiles per region of the flat list
+  int t1_tiles;                 // single context: tiles per region of the tile kernel's list
   const int* elem;              // [n_tiles * T] element of each lane, -1 if none
   const unsigned short* lslot;  // [n_tiles][NS][T] local node of each slot, 0xFFFF unused
   const int* node_ptr;          // [n_tiles + 1] range of each tile's local nodes
@@ -69,15 +69,13 @@ struct TileDev {
   int* wqlist;                  // [n_tiles * T * kMaxQp] (element, qp) pair segments,
                                 // 64 entries per wave and qp (pair_segment)
   unsigned long long* wcount;   // [n_tiles * T / 64] packed segment counts (HandSink)
-  // Single-context FP32-mixed path: the tile kernel lists its handed-over elements in a flat
-  // list of kT1Regions regions (tiles [j t1_tiles, (j + 1) t1_tiles) append to region j of
-  // list_e = wlist, t1_tiles * T entries each, at offsets from their own counter
-  // list_count[kT1Stride j]: the tiles' atomics spread over eight addresses), and the second
-  // tier writes what it still hands over to its own segments, one per second-tier wave:
-  // elements in wlist2, pairs in wqlist, packed counts in wcount2.  Null in batches (their
-  // tiles keep per-wave segments).
-  int* wlist2;                  // [n_seg2 * 64]
-  unsigned long long* wcount2;  // [n_seg2]
+  // Single context, FP32-mixed: the tile kernel lists its handed-over elements, the second
+  // tier's input, in a SegList of kRegions regions (tiles [j t1_tiles, (j + 1) t1_tiles)
+  // append to region j of list_e = wlist, t1_tiles * T entries each, counter
+  // list_count[kCntStride j]); FP64: one list (below).  `single` is 0 in batches, whose
+  // FP32-mixed tiles keep per-wave segments (wlist, wqlist, wcount) and whose FP64 tiles
+  // append to one (case, element) list.
+  int single;
   // The FP64 path lists its handed-over elements directly in the hand-over list, at offsets
   // from one device-scope atomic per wave on *list_count (the list's order does not matter:
   // every listed element is solved and finished on its own): list_e (single context,
@@ -95,12 +93,51 @@ struct TileDev {
 // status[8]: (element, qp) pairs handed over (m.qlist, ordered by qp_of_rank)
 // status[9]: how many of them, at the end of the list, are mid-surface pairs (has_mid_qp)
 constexpr int kStatusWords = 16;
-// The FP32-mixed single-context tile kernel's region counters (TileDev::t1_tiles), after the
-// status words in the same allocation, 256 bytes apart.
-constexpr int kT1Regions = 8;
-constexpr int kT1Stride = 64;
+// Counters of the single context's segmented hand-over lists (SegList), after the status
+// words in the same allocation, 256 bytes apart: the tile kernel's regions (the FP64
+// hand-over list, or the FP32-mixed second tier's input), the second tier's element regions,
+// and its pair classes (qp ranks).  Zeroed by bf_status_reset.
+constexpr int kRegions = 8;
+constexpr int kCntStride = 64;
 constexpr int kT1Base = 64;
-constexpr int kStatusAlloc = kT1Base + kT1Regions * kT1Stride;
+constexpr int kT2Base = kT1Base + kRegions * kCntStride;
+constexpr int kClassBase = kT2Base + kRegions * kCntStride;
+constexpr int kStatusAlloc = kClassBase + kMaxQp * kCntStride;
+
+// A list kept in n <= kRegions regions: region j holds cnt[kCntStride j] entries at
+// v + j cap, and the list is the regions read in order.  Its writers take offsets in a
+// region from one atomic per wave on the region's counter: a single counter for a whole grid
+// queues those atomics (1 536 waves on one address: +8 us).  v == nullptr: no list.
+struct SegList {
+  const int* v;
+  const int* cnt;
+  int cap, n;
+};
+struct SegCount {
+  int c[kRegions];
+  int total;
+};
+__device__ __forceinline__ SegCount seg_count(const SegList& L) {
+  SegCount s;
+  s.total = 0;
+#pragma unroll
+  for (int j = 0; j < kRegions; j++) {
+    s.c[j] = j < L.n ? L.cnt[kCntStride * j] : 0;
+    s.total += s.c[j];
+  }
+  return s;
+}
+// Entry i < s.total.
+__device__ __forceinline__ int seg_at(const SegList& L, const SegCount& s, int i) {
+  int j = 0;
+#pragma unroll
+  for (int r = 0; r + 1 < kRegions; r++)
+    if (j == r && i >= s.c[r]) {
+      i -= s.c[r];
+      j = r + 1;
+    }
+  return L.v[(size_t)j * L.cap + i];
+}
 
 // An (element, qp) pair of the hand-over solve.
 __host__ __device__ inline int qp_pair(int e, int q) { return (e << 3) | q; }
@@ -432,18 +469,20 @@ __host__ __device__ inline bool split_fits(int N, int N_mid, int P, bool pairs, 
 }
 
 // Re-evaluates, in the reference's operation order, what a staged kernel handed over and
-// forms the forces of the elements it listed in m.fallback[0 .. m.status[4]): with `pairs`
-// the (element, qp) pairs of m.qlist[0 .. m.status[8]), else every qp of those elements
-// (persistent grids of 4 n_cu waves, counts read on the device; the solve takes its work
-// items from status[6], which must be zero on entry).  `wide` also
+// forms the forces of the elements of elem_list (in m.fallback): with `pairs` the (element,
+// qp) pairs of solve_list (the classes of m.qlist), else every qp of the elements of
+// solve_list (= elem_list) (persistent grids of 4 n_cu waves, counts read on the device and
+// stored in status[4], and with `pairs` status[8], status[9]; the solve takes its work items
+// from status[6], which must be zero on entry).  `wide` also
 // launches the solve kernel for short lists (split of kWideSplitMin lanes or more); without
 // it the narrow kernel takes every list -- the sums, and so the results, are the same.
 // ev_solve0 / ev_solve1 (may be null) are recorded on s right before and after the solve
 // kernels.
 hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, const double* x,
                                          const double* y, const double* z, int n_cu, bool wide,
-                                         bool pairs, hipEvent_t ev_solve0, hipEvent_t ev_solve1,
-                                         hipStream_t s);
+                                         bool pairs, const SegList& solve_list,
+                                         const SegList& elem_list, hipEvent_t ev_solve0,
+                                         hipEvent_t ev_solve1, hipStream_t s);
 // Batch forms: the reference-order path over the element tiles (plain FP64 batches) and
 // over the compacted hand-over list b.hlist[0 .. *b.htotal).  b.rtotal[0..1] must be zero
 // on entry (the prologue clears them).

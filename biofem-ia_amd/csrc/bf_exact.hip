@@ -437,14 +437,15 @@ __device__ __forceinline__ void exact_solve_k(const DevMesh& m, const KParams& k
   }
 }
 
-// 2. finish: element e from the solve's qp results; guard trips go to the redo list.
-// kFinishLanes consecutive lanes share an element (the same e, so the same branches): lane
-// `slice` writes nodes slice, slice + kFinishLanes, ...; the lanes combine their finite
-// flags and slice 0 does the element's bookkeeping.
+// 2. finish: element e from the solve's qp results; guard trips go to the redo (returns
+// true: redo(e) was called by slice 0).  kFinishLanes consecutive lanes share an element
+// (the same e, so the same branches): lane `slice` writes nodes slice, slice +
+// kFinishLanes, ...; the lanes combine their finite flags and slice 0 does the element's
+// bookkeeping.
 constexpr int kFinishLanes = 4;
 
 template <class Redo>
-__device__ __forceinline__ void exact_finish(const DevMesh& m, const KParams& kp,
+__device__ __forceinline__ bool exact_finish(const DevMesh& m, const KParams& kp,
                                              const double* __restrict__ x,
                                              const double* __restrict__ y,
                                              const double* __restrict__ z, int e,
@@ -473,7 +474,7 @@ __device__ __forceinline__ void exact_finish(const DevMesh& m, const KParams& kp
   }
   if (status & kStatCC3333) {  // the cross-qp guard rule matters: redo with coupled qps
     if (slice == 0) redo(e);
-    return;
+    return true;
   }
   exact::ElemIn in;
   const ExactRows r = exact_inputs(m, kp, x, y, z, e, in, false);
@@ -481,6 +482,7 @@ __device__ __forceinline__ void exact_finish(const DevMesh& m, const KParams& kp
 #pragma unroll
   for (int k = 1; k < kFinishLanes; k <<= 1) bad |= __shfl_xor(bad, k, kFinishLanes);
   if (slice == 0) exact_status(m, e, status, bad == 0, extra_flags);
+  return false;
 }
 
 // 3. redo: one lane group per element, lane q runs qp q; the guard of iteration it skips
@@ -650,7 +652,15 @@ struct RedoSink {
 };
 
 // list.v == nullptr: elements 0 .. n_static - 1; else the list's entries, whose length the
-// first lane stores in status[4] (the count the host reads).
+// first lane stores in status[4] (the count the host reads).  kInlineRedo (n_qp <=
+// kFinishLanes): an element whose guard tripped is redone right here by its own lanes
+// (lane q runs qp q), instead of by bf_exact_redo_kernel after every finish -- it depends on
+// no other element, and one launch fewer is ~5 us of the hand-over chain.  The redo's
+// registers cut this kernel to one wave per SIMD, which a hand-over list does not fill.
+struct NoRedo {
+  __device__ void operator()(int) const {}
+};
+template <bool kInlineRedo>
 __global__ void __launch_bounds__(256) bf_exact_finish_kernel(DevMesh m, KParams kp,
                                                               const double* __restrict__ x,
                                                               const double* __restrict__ y,
@@ -663,12 +673,19 @@ __global__ void __launch_bounds__(256) bf_exact_finish_kernel(DevMesh m, KParams
     if (blockIdx.x == 0 && threadIdx.x == 0) m.status[4] = sc.total;
   }
   const long long N = (list.v ? sc.total : n_static) * (long long)kFinishLanes;
-  const RedoSink redo{m.redo, &m.status[5]};
   // the stride is a multiple of kFinishLanes: an element's lanes stay together
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N;
        t += (long long)gridDim.x * blockDim.x) {
     const int idx = (int)(t / kFinishLanes), slice = (int)(t % kFinishLanes);
-    exact_finish(m, kp, x, y, z, list.v ? seg_at(list, sc, idx) : idx, extra_flags, redo, slice);
+    const int e = list.v ? seg_at(list, sc, idx) : idx;
+    if constexpr (kInlineRedo) {
+      if (exact_finish(m, kp, x, y, z, e, extra_flags, NoRedo{}, slice)) {
+        const QpGroup g{e, slice, (int)(threadIdx.x & 63u) - slice};
+        exact_element(m, kp, x, y, z, g, extra_flags);
+      }
+    } else {
+      exact_finish(m, kp, x, y, z, e, extra_flags, RedoSink{m.redo, &m.status[5]}, slice);
+    }
   }
 }
 
@@ -750,6 +767,7 @@ struct BatchRedoSink {
 };
 
 // Finish over the element tiles (list = 0) or the hand-over list (list = 1).
+template <bool kInlineRedo>
 __global__ void __launch_bounds__(256) bf_batch_exact_finish_kernel(BatchDev b, int from_list,
                                                                     unsigned char extra_flags) {
   // the stride is a multiple of kFinishLanes: an element's lanes stay together
@@ -761,8 +779,15 @@ __global__ void __launch_bounds__(256) bf_batch_exact_finish_kernel(BatchDev b, 
       const int idx = (int)(t / kFinishLanes), slice = (int)(t % kFinishLanes);
       const int2 v = b.hlist[idx];
       const CaseDev& cd = b.cases[v.x];
-      exact_finish(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, extra_flags,
-                   BatchRedoSink{b.rlist, b.rtotal, v.x}, slice);
+      if constexpr (kInlineRedo) {  // as bf_exact_finish_kernel
+        if (exact_finish(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, extra_flags, NoRedo{}, slice)) {
+          const QpGroup g{v.y, slice, (int)(threadIdx.x & 63u) - slice};
+          exact_element(cd.m, cd.kp, cd.x, cd.y, cd.z, g, extra_flags);
+        }
+      } else {
+        exact_finish(cd.m, cd.kp, cd.x, cd.y, cd.z, v.y, extra_flags,
+                     BatchRedoSink{b.rlist, b.rtotal, v.x}, slice);
+      }
     }
     return;
   }
@@ -803,7 +828,7 @@ hipError_t launch_batch_exact(const BatchDev& b, int n_cu, hipStream_t s) {
   hipLaunchKernelGGL(bf_batch_exact_solve_kernel, dim3((items + 1) / 2), dim3(kExactBlock), 0, s, b);
   const long long total = (long long)b.n_xtiles * 64 * kFinishLanes;
   const int fblocks = (int)std::min<long long>((total + 255) / 256, 8LL * n_cu);
-  hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(fblocks), dim3(256), 0, s, b, 0,
+  hipLaunchKernelGGL(bf_batch_exact_finish_kernel<false>, dim3(fblocks), dim3(256), 0, s, b, 0,
                      (unsigned char)0);
   hipLaunchKernelGGL(bf_batch_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b,
                      (unsigned char)0);
@@ -816,10 +841,15 @@ hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, bool pairs, hipS
   hipLaunchKernelGGL(bf_batch_exact_solve_list_kernel, dim3(4 * n_cu), dim3(64), 0, s, b,
                      pairs ? 1 : 0, 4 * n_cu);
   // 64-lane blocks: a short hand-over list still spreads over the CUs
-  hipLaunchKernelGGL(bf_batch_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, b, 1,
-                     (unsigned char)kStatFallback);
-  hipLaunchKernelGGL(bf_batch_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b,
-                     (unsigned char)kStatFallback);
+  if (b.P <= kFinishLanes) {
+    hipLaunchKernelGGL(bf_batch_exact_finish_kernel<true>, dim3(8 * n_cu), dim3(64), 0, s, b, 1,
+                       (unsigned char)kStatFallback);
+  } else {
+    hipLaunchKernelGGL(bf_batch_exact_finish_kernel<false>, dim3(8 * n_cu), dim3(64), 0, s, b,
+                       1, (unsigned char)kStatFallback);
+    hipLaunchKernelGGL(bf_batch_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, b,
+                       (unsigned char)kStatFallback);
+  }
   return hipGetLastError();
 }
 
@@ -1110,10 +1140,15 @@ hipError_t launch_element_exact_fallback(const DevMesh& m, const KParams& kp, co
     if (e != hipSuccess) return e;
   }
   // 64-lane blocks: a short hand-over list still spreads over the CUs
-  hipLaunchKernelGGL(bf_exact_finish_kernel, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x, y, z,
-                     elem_list, 0, (unsigned char)kStatFallback);
-  hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,
-                     z, (unsigned char)kStatFallback);
+  if (kp.n_qp <= kFinishLanes) {
+    hipLaunchKernelGGL(bf_exact_finish_kernel<true>, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x, y,
+                       z, elem_list, 0, (unsigned char)kStatFallback);
+  } else {
+    hipLaunchKernelGGL(bf_exact_finish_kernel<false>, dim3(8 * n_cu), dim3(64), 0, s, m, kp, x,
+                       y, z, elem_list, 0, (unsigned char)kStatFallback);
+    hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x,
+                       y, z, (unsigned char)kStatFallback);
+  }
   return hipGetLastError();
 }
 
@@ -1123,7 +1158,7 @@ hipError_t launch_element_exact(const DevMesh& m, const KParams& kp, const doubl
   const int items = ((kp.E + 63) / 64) * kp.n_qp;
   hipLaunchKernelGGL(bf_exact_solve_kernel, dim3((items + 1) / 2), dim3(kExactBlock), 0, s, m,
                      kp, x, y, z, (const int*)nullptr, (const int*)nullptr, kp.E);
-  hipLaunchKernelGGL(bf_exact_finish_kernel,
+  hipLaunchKernelGGL(bf_exact_finish_kernel<false>,
                      dim3((unsigned)(((long long)kp.E * kFinishLanes + 255) / 256)), dim3(256), 0,
                      s, m, kp, x, y, z, SegList{}, kp.E, (unsigned char)0);
   hipLaunchKernelGGL(bf_exact_redo_kernel, dim3(2 * n_cu), dim3(kExactBlock), 0, s, m, kp, x, y,

@@ -253,8 +253,14 @@ def test_cc3333_guard_case():
         e, code, warn = fe.last_error()
         assert warn == st[2]
         assert fe.elem_flags()[st[1]] & 1
+        # the staged paths hand the element over; the finish redoes it with coupled qps
+        # (its own lanes for n_qp <= 4): its C33 is the reference's bit for bit
+        g = st[1]
         fast = fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+        assert np.array_equal(fe.c33()[g], c33[g])
+        assert fe.last_error()[2] >= 1 and fe.elem_flags()[g] & 1
         mixed = fe.finternal_act(x, y, z, precision=L.BF_PREC_MIXED)
+        assert np.array_equal(fe.c33()[g], c33[g])
         assert rel(fast, ref) <= 1e-12 and rel(mixed, ref) <= 1e-5
     finally:
         fe.close()

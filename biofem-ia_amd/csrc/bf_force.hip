@@ -570,7 +570,7 @@ __device__ __forceinline__ void tile_body(const DevMesh& m, const KParams& kp, c
       hbase = hand_list_reserve(hcount, hbal);
     } else {  // per-wave segments, which the second tier re-evaluates
       const int w = t * (T >> 6) + (tid >> 6);
-      HandSink{td.wlist, td.wqlist, td.wcount, w}.push(live && handed, qfail, e, P);
+      HandSink{td.wlist, td.wqlist, td.wcount, w, td.any}.push(live && handed, qfail, e, P);
     }
   }
   lds_barrier();  // region A is read: the slot forces take its place
@@ -956,6 +956,7 @@ __global__ void __launch_bounds__(256) bf_batch_prologue(BatchDev b, int ghosts)
     b.rtotal[0] = 0;
     b.rtotal[1] = 0;
     *b.htotal = 0;  // the FP64 tile kernel appends to hlist
+    b.ptotal[2] = 0;  // TileDev::any
   }
   if (!ghosts) return;
   for (int t = blockIdx.x; t < b.n_gtiles; t += gridDim.x) {
@@ -992,6 +993,7 @@ bf_batch_tile_kernel(BatchDev b) {
 // rewrites the segment in place with what remains for the reference-order kernel (a pass
 // reads its entries before it appends, and appends never pass the read position).
 __global__ void __launch_bounds__(kStagedBlock) bf_batch_tier2_kernel(BatchDev b) {
+  if (b.ptotal[2] == 0) return;  // no tile wave handed anything over
   const int P = b.P, per_wave = 64 / P;
   int slot, q, base;
   lane_group(P, threadIdx.x & 63, slot, q, base);
@@ -1026,6 +1028,14 @@ __global__ void __launch_bounds__(kStagedBlock) bf_batch_tier2_kernel(BatchDev b
 // rank, ptotal[0] = count, ptotal[1] = mid-surface pairs at the end.
 __global__ void __launch_bounds__(kCompactBlock) bf_batch_compact_kernel(BatchDev b, int pairs) {
   __shared__ int s_wave[2 * kCompactWaves * kCompactClasses];
+  if (b.ptotal[2] == 0) {  // no tile wave handed anything over: empty lists
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *b.htotal = 0;
+      b.ptotal[0] = 0;
+      b.ptotal[1] = 0;
+    }
+    return;
+  }
   compact_all(s_wave, b.tcount, b.tlist, b.tqlist, b.n_etiles, pairs ? b.P : 0,
               [&](int at, int w, int v) { b.hlist[at] = make_int2(b.etiles[w].c, v); },
               [&](int at, int w, int v) { b.plist[at] = make_int2(b.etiles[w].c, v); },
@@ -2400,7 +2410,7 @@ int batch_build(bf_batch* b) {
   BALLOC(b->d_hlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_htotal, 1);
   BALLOC(b->d_plist, (size_t)std::max<long long>(E_pairs, 1));
-  BALLOC(b->d_ptotal, 2);
+  BALLOC(b->d_ptotal, 3);
   BALLOC(b->d_rlist, (size_t)std::max<long long>(Et, 1));
   BALLOC(b->d_rtotal, 2);
   BALLOC(b->d_cstat, 2 * cases.size());
@@ -2411,6 +2421,7 @@ int batch_build(bf_batch* b) {
     cases[i].td.list_e = nullptr;
     cases[i].td.list_ce = b->d_hlist;
     cases[i].td.list_case = i;
+    cases[i].td.any = b->d_ptotal + 2;
   }
   hipStream_t s = b->stream;
   BF_HIP(hipMemcpyAsync(b->d_cases, cases.data(), sizeof(CaseDev) * cases.size(), hipMemcpyHostToDevice, s));

@@ -69,6 +69,9 @@ struct TileDev {
   int* wqlist;                  // [n_tiles * T * kMaxQp] (element, qp) pair segments,
                                 // 64 entries per wave and qp (pair_segment)
   unsigned long long* wcount;   // [n_tiles * T / 64] packed segment counts (HandSink)
+  int* any;                     // batches, FP32-mixed: set to 1 by a tile wave that hands
+                                // anything over (the second tier and the compaction skip
+                                // their scans of the count words while it is 0)
   // Single context, FP32-mixed: the tile kernel lists its handed-over elements, the second
   // tier's input, in a SegList of kRegions regions (tiles [j t1_tiles, (j + 1) t1_tiles)
   // append to region j of list_e = wlist, t1_tiles * T entries each, counter
@@ -219,6 +222,7 @@ struct HandSink {
   int* plist;                // [n_waves * kMaxQp * 64]
   unsigned long long* count; // [n_waves]
   int wave;
+  int* any;                  // set to 1 when this wave lists anything (may be null)
   __device__ void push(bool take, unsigned qmask, int e, int P) const {
     const unsigned long long act = __ballot(true);
     const int lane = (int)(threadIdx.x & 63u);
@@ -233,7 +237,10 @@ struct HandSink {
       if (tq) plist[pair_segment(wave, q) + __popcll(bal & below)] = qp_pair(e, q);
       word |= hand_count_pairs(q, __popcll(bal));
     }
-    if (lane == leader) count[wave] = word;
+    if (lane == leader) {
+      count[wave] = word;
+      if (word != 0ull && any) *any = 1;  // every writer stores the same value
+    }
   }
 };
 
@@ -336,7 +343,8 @@ struct BatchDev {
   int* htotal;
   int* tqlist;            // [n_etiles * kMaxQp * 64] (element, qp) pair segments
   int2* plist;            // compacted pair list: (case, qp_pair(element, qp)), by qp rank
-  int* ptotal;            // [2]: pairs, mid-surface pairs at the end of the list
+  int* ptotal;            // [3]: pairs, mid-surface pairs at the end of the list, and the
+                          // flag TileDev::any of the step
   int2* rlist;            // elements whose CC3333 guard tripped: (case, element)
   int* rtotal;            // [2]: rlist count, work counter of the hand-over solve
 };

@@ -117,6 +117,7 @@ SIGNATURES = {
     "bf_batch_eval": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "bf_batch_sync": (C.c_int, [_vp]),
     "bf_batch_last_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "bf_batch_timing_enable": (C.c_int, [_vp, C.c_int]),
     "bf_batch_last_fallback_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "bf_batch_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bf_batch_case_status": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

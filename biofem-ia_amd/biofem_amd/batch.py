@@ -40,6 +40,11 @@ class Batch:
     def sync(self) -> None:
         L.check("bf_batch_sync", self.lib.bf_batch_sync(self.h))
 
+    def timing(self, on: bool) -> None:
+        """HIP events around every later eval/step call (bf_batch_timing_enable), for last_ms;
+        each record costs a few microseconds of device time."""
+        L.check("bf_batch_timing_enable", self.lib.bf_batch_timing_enable(self.h, int(bool(on))))
+
     def last_ms(self) -> float:
         ms = C.c_float()
         L.check("bf_batch_last_ms", self.lib.bf_batch_last_ms(self.h, C.byref(ms)))

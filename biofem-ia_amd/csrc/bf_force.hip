@@ -1116,6 +1116,7 @@ struct bf_ctx {
   std::vector<hipEvent_t> ring;
   int ring_n = 0;
   bool ring_on = false;
+  bool timed_last = false;  // the last evaluation recorded every timing event
   long long t_count = 0;
   double t_elem = 0.0, t_handover = 0.0, t_gather = 0.0, t_solve = 0.0;
   // mesh
@@ -1790,10 +1791,15 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     c->ring_n++;
   }
   for (int i = 0; i < bf_ctx::kEv; i++) c->ev[i] = ev[i];  // bf_last_timing reads the latest set
+  // Timing events only while timing is on: each record on the stream costs ~3.7 us of device
+  // time between the kernels (five of them: 18 us per evaluation of config 2).  ev[3], after
+  // the last kernel, is always recorded: status reads are ordered after it.
+  const bool timed = c->ring_on;
+  c->timed_last = timed;
   hipEvent_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
   double* fout = step ? c->d_fint : fint;
   const int acc = step ? 0 : accumulate;
-  BF_HIP(hipEventRecord(e0, s));
+  if (timed) BF_HIP(hipEventRecord(e0, s));
   if (c->E > 0) {
     if (!staged) {
       BF_HIP(launch_element_exact(dm, kp, x, y, z, c->n_cu, s));
@@ -1809,7 +1815,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
                            kp, td, x, y, z, fout, acc, c->n_v);
     }
   }
-  BF_HIP(hipEventRecord(e1, s));
+  if (timed) BF_HIP(hipEventRecord(e1, s));
   c->fb_prec = -1;
   if (c->E > 0 && staged && kp.fallback) {
     const int n_waves = td.n_tiles * (td.T / 64);
@@ -1823,16 +1829,16 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
       hipLaunchKernelGGL(bf_tile_tier2_kernel, dim3(tier2_waves(n_waves)), dim3(64), 0, s, dm, kp,
                          td, t2_list, t2_region_waves(n_waves), x, y, z);
     BF_HIP(launch_element_exact_fallback(dm, kp, x, y, z, c->n_cu, c->wide_hint[precision],
-                                         pairs, pairs ? cls : t1, pairs ? t2 : t1, ev[4], ev[5],
-                                         s));
+                                         pairs, pairs ? cls : t1, pairs ? t2 : t1,
+                                         timed ? ev[4] : nullptr, timed ? ev[5] : nullptr, s));
     c->fb_prec = precision;
     c->last_list = pairs ? cls : t1;
   }
-  if (c->fb_prec < 0) {  // no hand-over solve: an empty interval
+  if (c->fb_prec < 0 && timed) {  // no hand-over solve: an empty interval
     BF_HIP(hipEventRecord(ev[4], s));
     BF_HIP(hipEventRecord(ev[5], s));
   }
-  BF_HIP(hipEventRecord(e2, s));
+  if (timed) BF_HIP(hipEventRecord(e2, s));
   if (staged) {
     BF_HIP(launch_tile_fixup(td, c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, acc, fout,
                              step, c->d_x, c->d_y, c->d_z, c->d_v, c->d_winv, c->d_fixed, c->dt,
@@ -2077,7 +2083,7 @@ int bf_set_fallback(bf_ctx* c, int on) {
 
 int bf_last_timing(bf_ctx* c, float* ms_element, float* ms_gather) {
   if (!c) return BF_ERR_ARG;
-  if (!c->timing) return BF_ERR_STATE;
+  if (!c->timing || !c->timed_last) return BF_ERR_STATE;
   BF_HIP(hipEventSynchronize(c->ev[3]));
   float a = 0.f, b = 0.f;
   BF_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[2]));
@@ -2254,6 +2260,8 @@ struct bf_batch {
   int n_cu = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timing_on = false;   // ev0 / ev1 around every bf_batch_eval / bf_batch_step call
+  bool timed_last = false;  // the last call recorded them
   std::vector<bf_ctx*> cases;
   int P = 1;
   // launch shape of the element phase
@@ -2599,7 +2607,8 @@ static int batch_run(bf_batch* b, int precision, int ghosts, bool step, int n, i
     return BF_ERR_ARG;
   if (b->cases.empty()) return BF_OK;
   BF_HIP(hipSetDevice(b->device));
-  BF_HIP(hipEventRecord(b->ev0, b->stream));
+  // each event record costs a few microseconds of device time between the kernels
+  if (b->timing_on) BF_HIP(hipEventRecord(b->ev0, b->stream));
   if (use_graph) {
     if (!b->exec || b->g_prec != precision || b->g_ghosts != (ghosts ? 1 : 0) ||
         b->g_step != (step ? 1 : 0)) {
@@ -2626,7 +2635,8 @@ static int batch_run(bf_batch* b, int precision, int ghosts, bool step, int n, i
       if (rc != BF_OK) return rc;
     }
   }
-  BF_HIP(hipEventRecord(b->ev1, b->stream));
+  if (b->timing_on) BF_HIP(hipEventRecord(b->ev1, b->stream));
+  b->timed_last = b->timing_on;
   return BF_OK;
 }
 
@@ -2645,8 +2655,15 @@ int bf_batch_sync(bf_batch* b) {
   return BF_OK;
 }
 
+int bf_batch_timing_enable(bf_batch* b, int on) {
+  if (!b) return BF_ERR_ARG;
+  b->timing_on = on != 0;
+  return BF_OK;
+}
+
 int bf_batch_last_ms(bf_batch* b, float* ms) {
   if (!b || !ms) return BF_ERR_ARG;
+  if (!b->timed_last) return BF_ERR_STATE;
   BF_HIP(hipEventSynchronize(b->ev1));
   BF_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
   return BF_OK;

@@ -180,11 +180,14 @@ int bf_set_fallback(bf_ctx *ctx, int on);
 int bf_last_handed_qps(bf_ctx *ctx, unsigned char *mask, int *n_pairs);
 
 /* Wall-clock duration of the last bf_force_eval* kernels on the device, in ms
- * (HIP events on the context stream): element kernel and node-gather kernel. */
+ * (HIP events on the evaluation stream): element kernel and node-gather kernel.  The events
+ * are recorded while timing is on (bf_timing_enable); otherwise BF_ERR_STATE. */
 int bf_last_timing(bf_ctx *ctx, float *ms_element, float *ms_gather);
 
 /* Kernel timing over many evaluations: bf_timing_enable(ctx, 1) resets the totals and
- * records HIP events around the element and gather kernels of every later evaluation;
+ * records HIP events around the element and gather kernels of every later evaluation (each
+ * record costs a few microseconds of device time between the kernels, so an untimed
+ * evaluation records none but the one after its last kernel);
  * bf_timing_read() synchronises and returns the count and total milliseconds
  * (ms_element includes the hand-over kernel of the staged precisions). */
 int bf_timing_enable(bf_ctx *ctx, int on);
@@ -272,7 +275,11 @@ int bf_batch_eval(bf_batch *batch, int precision, int ghosts, int use_graph);
  * bf_batch_set_cases / bf_batch_refresh); cases without one are only evaluated. */
 int bf_batch_step(bf_batch *batch, int precision, int n_steps, int use_graph);
 int bf_batch_sync(bf_batch *batch);
-/* Device time of the last bf_batch_eval, in ms (synchronises). */
+/* Device time of the last bf_batch_eval / bf_batch_step call, in ms (synchronises): HIP
+ * events around the call, recorded while bf_batch_timing_enable(batch, 1) is on (each
+ * record costs a few microseconds of device time); otherwise BF_ERR_STATE.  No reference
+ * counterpart (instrumentation). */
+int bf_batch_timing_enable(bf_batch *batch, int on);
 int bf_batch_last_ms(bf_batch *batch, float *ms);
 /* Elements of all cases the last staged step recomputed in the reference's order. */
 int bf_batch_last_fallback_count(bf_batch *batch, int *n);

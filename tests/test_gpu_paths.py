@@ -326,3 +326,35 @@ def test_status_after_eval_on_caller_stream():
         assert np.array_equal(f.cpu().numpy(), ref, equal_nan=True)
     finally:
         fe.close()
+
+
+def test_timing_events_only_while_timing_is_on():
+    """The timing events cost device time between the kernels, so an evaluation records them
+    only while timing is on: bf_last_timing answers BF_ERR_STATE before, the split totals
+    after bf_timing_enable(1); the same for a batch's bf_batch_last_ms."""
+    from biofem_amd.batch import Batch, upload_coords
+    m, p, (x, y, z) = configs.membrane_small()
+    print("case timing events 32x24", flush=True)
+    fe = ActStrainFE(m, p)
+    bt = None
+    try:
+        fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+        a, b = C.c_float(), C.c_float()
+        assert fe.lib.bf_last_timing(fe.ctx, C.byref(a), C.byref(b)) == L.BF_ERR_STATE
+        L.check("t", fe.lib.bf_timing_enable(fe.ctx, 1))
+        fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+        assert fe.lib.bf_last_timing(fe.ctx, C.byref(a), C.byref(b)) == L.BF_OK
+        assert a.value > 0.0 and b.value > 0.0
+        L.check("t", fe.lib.bf_timing_enable(fe.ctx, 0))
+        upload_coords(fe, x, y, z)
+        bt = Batch([fe])
+        bt.eval(L.BF_PREC_FP64_FAST, graph=False)
+        ms = C.c_float()
+        assert fe.lib.bf_batch_last_ms(bt.h, C.byref(ms)) == L.BF_ERR_STATE
+        bt.timing(True)
+        bt.eval(L.BF_PREC_FP64_FAST, graph=False)
+        assert bt.last_ms() > 0.0
+    finally:
+        if bt is not None:
+            bt.close()
+        fe.close()

@@ -1116,7 +1116,8 @@ struct bf_ctx {
   std::vector<hipEvent_t> ring;
   int ring_n = 0;
   bool ring_on = false;
-  bool timed_last = false;  // the last evaluation recorded every timing event
+  bool timed_last = false;    // the last evaluation recorded every timing event
+  bool ev3_recorded = false;  // ... or at least ev[3] (timing, or a caller's stream)
   long long t_count = 0;
   double t_elem = 0.0, t_handover = 0.0, t_gather = 0.0, t_solve = 0.0;
   // mesh
@@ -1793,9 +1794,12 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   for (int i = 0; i < bf_ctx::kEv; i++) c->ev[i] = ev[i];  // bf_last_timing reads the latest set
   // Timing events only while timing is on: each record on the stream costs ~3.7 us of device
   // time between the kernels (five of them: 18 us per evaluation of config 2).  ev[3], after
-  // the last kernel, is always recorded: status reads are ordered after it.
+  // the last kernel, is also recorded for an evaluation on a caller's stream: the status
+  // reads on the context stream are ordered after it (on the context stream they follow
+  // anyway).
   const bool timed = c->ring_on;
   c->timed_last = timed;
+  c->ev3_recorded = timed || s != c->stream;
   hipEvent_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
   double* fout = step ? c->d_fint : fint;
   const int acc = step ? 0 : accumulate;
@@ -1851,7 +1855,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
     hipLaunchKernelGGL(bf_gather_kernel, dim3((c->n_nodes + 255) / 256), dim3(256), 0, s,
                        c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, accumulate, fint);
   }
-  BF_HIP(hipEventRecord(e3, s));
+  if (c->ev3_recorded) BF_HIP(hipEventRecord(e3, s));
   c->timing = true;
   return kernel_launch_check();
 }
@@ -1860,7 +1864,7 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
 // stream (bf_force_eval_device): ev[3] is recorded on the evaluation's stream after its last
 // kernel.  Every read of a per-evaluation result on the context's stream goes through here.
 static int after_last_eval(bf_ctx* c) {
-  if (c->timing) BF_HIP(hipStreamWaitEvent(c->stream, c->ev[3], 0));
+  if (c->timing && c->ev3_recorded) BF_HIP(hipStreamWaitEvent(c->stream, c->ev[3], 0));
   return BF_OK;
 }
 

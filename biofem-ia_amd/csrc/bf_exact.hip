@@ -1045,11 +1045,14 @@ __device__ __forceinline__ void step_update(int n, const double (&f)[3], double*
   z[n] = z[n] + dt * vz;
 }
 
+// st (not null): the step's status words, handed over to vis by block 0 (status_handoff).
 __global__ void __launch_bounds__(256) bf_tile_fixup_kernel(TileDev td, const int* __restrict__ ptr,
                                                             const unsigned* __restrict__ ent,
                                                             const double* __restrict__ Fe, int E,
                                                             int n_v, int n_nodes, int accumulate,
-                                                            double* __restrict__ fint) {
+                                                            double* __restrict__ fint, int* st,
+                                                            int* vis) {
+  if (st && blockIdx.x == 0) status_handoff(st, vis, (int)threadIdx.x);
   const int n = logical_block() * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   double f[3];
@@ -1060,7 +1063,8 @@ __global__ void __launch_bounds__(256) bf_tile_fixup_step_kernel(
     TileDev td, const int* __restrict__ ptr, const unsigned* __restrict__ ent,
     const double* __restrict__ Fe, int E, int n_v, int n_nodes, double* __restrict__ fint,
     double* x, double* y, double* z, double* v, const double* winv, const unsigned char* fixed,
-    double dt, double damp) {
+    double dt, double damp, int* st, int* vis) {
+  if (st && blockIdx.x == 0) status_handoff(st, vis, (int)threadIdx.x);
   const int n = logical_block() * blockDim.x + threadIdx.x;
   if (n >= n_nodes) return;
   double f[3];
@@ -1085,15 +1089,15 @@ hipError_t launch_tile_fixup(const TileDev& td, const int* ptr, const unsigned* 
                              const double* Fe, int E, int n_v, int n_nodes, int accumulate,
                              double* fint, bool step, double* x, double* y, double* z, double* v,
                              const double* winv, const unsigned char* fixed, double dt,
-                             double damp, hipStream_t s) {
+                             double damp, int* st, int* vis, hipStream_t s) {
   if (n_nodes <= 0) return hipSuccess;
   const dim3 grid((n_nodes + 255) / 256), block(256);
   if (step)
     hipLaunchKernelGGL(bf_tile_fixup_step_kernel, grid, block, 0, s, td, ptr, ent, Fe, E, n_v,
-                       n_nodes, fint, x, y, z, v, winv, fixed, dt, damp);
+                       n_nodes, fint, x, y, z, v, winv, fixed, dt, damp, st, vis);
   else
     hipLaunchKernelGGL(bf_tile_fixup_kernel, grid, block, 0, s, td, ptr, ent, Fe, E, n_v, n_nodes,
-                       accumulate, fint);
+                       accumulate, fint, st, vis);
   return hipGetLastError();
 }
 

@@ -1156,7 +1156,10 @@ struct bf_ctx {
   int* d_ptr = nullptr;
   unsigned* d_ent = nullptr;
   int* d_rules = nullptr;
-  int* d_status = nullptr;
+  int* d_status = nullptr;    // [kStatusAlloc] the step's status words and list counters
+  int* d_status_vis = nullptr;  // [kStatusAlloc] their copy after a staged step (host reads)
+  int* st_read = nullptr;       // where the host reads the last step's words
+  bool st_fresh = false;        // d_status is reset for the next step (status_handoff)
   int* h_status = nullptr;  // pinned
   double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_fint = nullptr;
   // params
@@ -1419,12 +1422,14 @@ int bf_ctx_create(int device, bf_ctx** out) {
   bool events = true;
   for (auto& e : c->own_ev) events = events && hipEventCreate(&e) == hipSuccess;
   if (!events || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc((void**)&c->d_status, kStatusAlloc * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_status, 2 * kStatusAlloc * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_status, kStatusWords * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     bf_ctx_destroy(c);
     return BF_ERR_DEVICE;
   }
   for (int i = 0; i < bf_ctx::kEv; i++) c->ev[i] = c->own_ev[i];
+  c->d_status_vis = c->d_status + kStatusAlloc;
+  c->st_read = c->d_status;
   c->kp.fallback = 1;
   *out = c;
   return BF_OK;
@@ -1781,7 +1786,10 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   td.list_ce = nullptr;
   td.list_case = 0;
   const bool staged = precision != BF_PREC_FP64;
-  hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
+  // the status words start reset: by the last staged step's node pass, or here
+  if (!c->st_fresh) hipLaunchKernelGGL(bf_status_reset, dim3(1), dim3(64), 0, s, c->d_status);
+  c->st_fresh = false;
+  c->st_read = c->d_status;
   hipEvent_t* ev = c->own_ev;
   if (c->ring_on) {
     if (c->ring_n == bf_ctx::kRing) {
@@ -1844,9 +1852,14 @@ static int launch_eval(bf_ctx* c, const double* x, const double* y, const double
   }
   if (timed) BF_HIP(hipEventRecord(e2, s));
   if (staged) {
+    const bool handoff = c->n_nodes > 0;
     BF_HIP(launch_tile_fixup(td, c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, acc, fout,
                              step, c->d_x, c->d_y, c->d_z, c->d_v, c->d_winv, c->d_fixed, c->dt,
-                             c->damp, s));
+                             c->damp, handoff ? c->d_status : nullptr, c->d_status_vis, s));
+    if (handoff) {
+      c->st_fresh = true;
+      c->st_read = c->d_status_vis;
+    }
   } else if (step) {
     BF_HIP(launch_gather_step(c->d_ptr, c->d_ent, dm.Fe, c->E, c->n_v, c->n_nodes, c->d_fint,
                               c->d_x, c->d_y, c->d_z, c->d_v, c->d_winv, c->d_fixed, c->dt,
@@ -1870,7 +1883,7 @@ static int after_last_eval(bf_ctx* c) {
 
 static int collect_status(bf_ctx* c, int* elem, int* code, int* n_warn) {
   if (int rc = after_last_eval(c)) return rc;
-  BF_HIP(hipMemcpyAsync(c->h_status, c->d_status, kStatusWords * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipMemcpyAsync(c->h_status, c->st_read, kStatusWords * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   const int* st = c->h_status;
   if (c->fb_prec >= 0)
@@ -2051,7 +2064,7 @@ int bf_last_handed_qps(bf_ctx* c, unsigned char* mask, int* n_pairs) {
   const bool pairs = c->fb_prec == BF_PREC_MIXED;
   const SegList& L = c->last_list;
   std::vector<int> cnt(kStatusAlloc);
-  BF_HIP(hipMemcpyAsync(cnt.data(), c->d_status, sizeof(int) * kStatusAlloc, hipMemcpyDeviceToHost, c->stream));
+  BF_HIP(hipMemcpyAsync(cnt.data(), c->st_read, sizeof(int) * kStatusAlloc, hipMemcpyDeviceToHost, c->stream));
   BF_HIP(hipStreamSynchronize(c->stream));
   const size_t at = (size_t)(L.cnt - c->d_status);
   std::vector<int> list;
@@ -2483,6 +2496,10 @@ BatchDev batch_dev(const bf_batch* b) {
 // step: the node phase also advances the cases that hold a time-stepping state.
 int batch_enqueue(bf_batch* b, int precision, int ghosts, bool step = false) {
   hipStream_t s = b->stream;
+  for (bf_ctx* c : b->cases) {  // the batch resets and writes the cases' own status words
+    c->st_fresh = false;
+    c->st_read = c->d_status;
+  }
   const BatchDev d = batch_dev(b);
   const int pblocks = std::max(1, std::min(b->n_cu * 2, std::max(b->n_gtiles, (d.n_cases + 255) / 256)));
   hipLaunchKernelGGL(bf_batch_prologue, dim3(pblocks), dim3(256), 0, s, d, ghosts);
@@ -2611,6 +2628,10 @@ static int batch_run(bf_batch* b, int precision, int ghosts, bool step, int n, i
     return BF_ERR_ARG;
   if (b->cases.empty()) return BF_OK;
   BF_HIP(hipSetDevice(b->device));
+  for (bf_ctx* c : b->cases) {  // also for graph replays, which skip batch_enqueue
+    c->st_fresh = false;
+    c->st_read = c->d_status;
+  }
   // each event record costs a few microseconds of device time between the kernels
   if (b->timing_on) BF_HIP(hipEventRecord(b->ev0, b->stream));
   if (use_graph) {

@@ -1,6 +1,7 @@
 // bf_kernels.h -- device-side structures shared by the kernel translation units.
 #pragma once
 
+#include <climits>
 #include <hip/hip_runtime.h>
 
 #include "bf_elem.h"
@@ -106,6 +107,22 @@ constexpr int kT1Base = 64;
 constexpr int kT2Base = kT1Base + kRegions * kCntStride;
 constexpr int kClassBase = kT2Base + kRegions * kCntStride;
 constexpr int kStatusAlloc = kClassBase + kMaxQp * kCntStride;
+
+// The status words and SegList counters as a step starts: first error / warning elements
+// INT_MAX, everything else 0.  The staged paths' node pass hands them over (status_handoff):
+// block 0 copies the step's words into a second area, which the host reads, and resets them
+// for the next step, so that step needs no reset launch (~4 us with its gap); each thread
+// copies, then resets, words of its own.
+__device__ __forceinline__ void status_handoff(int* st, int* vis, int t) {
+  if (t < kStatusWords) {
+    vis[t] = st[t];
+    st[t] = (t == 0 || t == 2) ? INT_MAX : 0;
+  } else if (t < kStatusWords + 2 * kRegions + kMaxQp) {
+    const int o = kT1Base + kCntStride * (t - kStatusWords);
+    vis[o] = st[o];
+    st[o] = 0;
+  }
+}
 
 // A list kept in n <= kRegions regions: region j holds cnt[kCntStride j] entries at
 // v + j cap, and the list is the regions read in order.  Its writers take offsets in a
@@ -506,12 +523,13 @@ hipError_t launch_batch_exact_list(const BatchDev& b, int n_cu, bool pairs, hipS
 // Node pass of the single-pass staged path: each node's force from its tile partial sums
 // (tile order) or the value its tile wrote, then the forces of handed-over elements
 // (element-force buffer, ascending element order) for nodes that have any; step != 0 also
-// applies the explicit damped update (as launch_gather_step).  Batch form over node tiles.
+// applies the explicit damped update (as launch_gather_step); st (may be null): the
+// step's status words, handed over to vis (status_handoff).  Batch form over node tiles.
 hipError_t launch_tile_fixup(const TileDev& td, const int* ptr, const unsigned* ent,
                              const double* Fe, int E, int n_v, int n_nodes, int accumulate,
                              double* fint, bool step, double* x, double* y, double* z, double* v,
                              const double* winv, const unsigned char* fixed, double dt,
-                             double damp, hipStream_t s);
+                             double damp, int* st, int* vis, hipStream_t s);
 hipError_t launch_batch_tile_fixup(const BatchDev& b, bool step, hipStream_t s);
 
 }  // namespace bf

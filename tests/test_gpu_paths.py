@@ -358,3 +358,33 @@ def test_timing_events_only_while_timing_is_on():
         if bt is not None:
             bt.close()
         fe.close()
+
+
+def test_status_words_start_reset_every_step():
+    """The staged paths' node pass hands the status words over and resets them for the next
+    step (no reset launch then): repeated evaluations report the same counts, not sums, also
+    after a batch step of the same context wrote its own."""
+    from biofem_amd.batch import Batch, upload_coords
+    m, p, (x, y, z) = _noisy_reference(128, 3e-3, 2)
+    print("case status reset guard 128x96 amp 3e-3 seed 2", flush=True)
+    fe = ActStrainFE(m, p)
+    bt = None
+    try:
+        seen = []
+        for prec in (L.BF_PREC_FP64_FAST, L.BF_PREC_FP64_FAST, L.BF_PREC_MIXED,
+                     L.BF_PREC_FP64_FAST):
+            fe.finternal_act(x, y, z, precision=prec)
+            seen.append((prec, fe.last_error(), fe.fallback_count()))
+        assert seen[0] == seen[1] == seen[3]
+        assert seen[0][1][2] >= 1  # the guard trip is reported once per step
+        upload_coords(fe, x, y, z)
+        bt = Batch([fe])
+        bt.eval(L.BF_PREC_FP64_FAST, graph=True)
+        bt.eval(L.BF_PREC_FP64_FAST, graph=True)
+        bt.sync()  # the batch's stream is not ordered with the context's
+        fe.finternal_act(x, y, z, precision=L.BF_PREC_FP64_FAST)
+        assert (L.BF_PREC_FP64_FAST, fe.last_error(), fe.fallback_count()) == seen[0]
+    finally:
+        if bt is not None:
+            bt.close()
+        fe.close()

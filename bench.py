@@ -10,7 +10,7 @@ C33 iteration leaves its well-conditioned range recomputed in the reference's op
 order.  The reference-order FP64 path (bitwise equal to the CPU oracle) and the FP32-mixed
 path are measured in the same run; every path states its error against the oracle.
 
-Printed: one JSON line (see the keys below).  --gpus N > 1 exits: this build measures one GPU.
+Printed: one JSON line (see the keys below).  --gpus N > 1 exits: the benchmark measures one GPU.
 """
 import argparse
 import ctypes as C
@@ -425,7 +425,7 @@ def main():
                     help="100 explicit steps of the 32x24 membrane (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by element partition with a halo exchange, not run here)",
               file=sys.stderr)
         sys.exit(2)

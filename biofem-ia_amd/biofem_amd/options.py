@@ -138,7 +138,7 @@ def act_params(opts: Options) -> ActParams:
     """ActParams from the database: GetUserActParams (src/active_strain.c:94-106) and the
     globals the force reads (-mu, -h0, -dt, -tisteps, -tistart, -manufactured;
     src/main.c:192-222).  -mu is the shear modulus on this path (src/active_strain.c:108).
-    -gamma_pulse_t0 / -gamma_pulse_sigma set this build's ramp type 4 (a Gaussian pulse,
+    -gamma_pulse_t0 / -gamma_pulse_sigma set the library's ramp type 4 (a Gaussian pulse,
     no reference counterpart)."""
     n_qp = opts.get_int("-num_gaussian_quad_points", 0)
     iters = opts.get_int("-C33_subitr_nums", 0)
